@@ -1,0 +1,167 @@
+/*
+ * pphpc.h -- C ABI of the MI355X-native PPHPC per-iteration step.
+ *
+ * A drop-in for the path of the reference OpenCL GPU simulator `pp_gpu`
+ * (opencl/pp/pp_gpu.c): same parameter file, same seed/iteration contract,
+ * same per-iteration statistics file. Plain C types only; no exceptions cross
+ * this boundary; every entry point returns 0 or a negative PP_* code with the
+ * values of the reference's enum pp_error_codes (opencl/pp/pp_common.in.h:87-104).
+ *
+ * Threading: a pp_sim_t handle is used by one host thread at a time. Several
+ * handles may share one GPU. The handle owns all device memory; the caller
+ * owns every host array passed in.
+ */
+#ifndef PPHPC_H
+#define PPHPC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes: the values of enum pp_error_codes (pp_common.in.h:87-104). */
+#define PP_SUCCESS 0
+#define PP_UNKNOWN_ARGS (-1)
+#define PP_INVALID_ARGS (-2)
+#define PP_LIBRARY_ERROR (-3)
+#define PP_UNABLE_TO_OPEN_PARAMS_FILE (-4)
+#define PP_INVALID_PARAMS_FILE (-5)
+#define PP_UNABLE_SAVE_STATS (-6)
+#define PP_OUT_OF_RESOURCES (-8)
+
+/* Agent type codes: SHEEP_ID / WOLF_ID of pp_common.in.h:48-52. */
+#define PP_SHEEP 0
+#define PP_WOLF 1
+
+/* Default capacity: the reference's default maximum number of agents
+ * (PPG_DEFAULT_MAX_AGENTS, pp_gpu.c:57). */
+#define PP_DEFAULT_MAX_AGENTS 16777216u
+
+/* Replaces PPParameters (pp_common.in.h:129-157): same 13 fields, same order,
+ * same meaning. grid_xy = grid_x * grid_y. */
+typedef struct pp_params {
+	uint32_t init_sheep;
+	uint32_t sheep_gain_from_food;
+	uint32_t sheep_reproduce_threshold;
+	uint32_t sheep_reproduce_prob;
+	uint32_t init_wolves;
+	uint32_t wolves_gain_from_food;
+	uint32_t wolves_reproduce_threshold;
+	uint32_t wolves_reproduce_prob;
+	uint32_t grass_restart;
+	uint32_t grid_x;
+	uint32_t grid_y;
+	uint32_t grid_xy;
+	uint32_t iters;
+} pp_params_t;
+
+/* Replaces PPStatistics (pp_common.in.h:109-124): same 7 fields, same order,
+ * widened from cl_uint to exact 64-bit counts and sums. */
+typedef struct pp_stats64 {
+	uint64_t sheep;
+	uint64_t wolves;
+	uint64_t grass;     /* cells with live grass (countdown 0) */
+	uint64_t sheep_en;  /* sum of sheep energy */
+	uint64_t wolves_en; /* sum of wolf energy */
+	uint64_t grass_en;  /* sum of grass countdowns over all cells */
+	uint64_t errors;
+} pp_stats64_t;
+
+typedef struct pp_sim pp_sim_t;
+
+/* Replaces pp_load_params (pp_common.c:39-195). Reads "KEY = value" records;
+ * each of the 12 keys exactly once. Returns PP_SUCCESS,
+ * PP_UNABLE_TO_OPEN_PARAMS_FILE (missing file) or PP_INVALID_PARAMS_FILE
+ * (unknown key, repeated key, missing key). NULL path = "config.txt"
+ * (PP_DEFAULT_PARAMS_FILE, pp_common.in.h:55). */
+int pp_load_params(const char *path, pp_params_t *out);
+
+/* Replaces pp_stats_save (pp_common.c:205-240): writes nrows lines of
+ * "%llu\t%llu\t%llu\t%f\t%f\t%f\n" (sheep, wolves, grass alive, mean sheep
+ * energy, mean wolf energy, mean countdown); a mean of an empty population is
+ * 0. Means are exact sums divided in double precision. NULL path =
+ * "stats.txt" (pp_common.in.h:58). Returns PP_SUCCESS or PP_UNABLE_SAVE_STATS. */
+int pp_stats_save(const char *path, const pp_stats64_t *rows, uint32_t nrows,
+	const pp_params_t *p);
+
+/* Message for a PP_* code. */
+const char *pp_strerror(int code);
+
+/* Replaces the set-up half of pp_gpu's main (pp_gpu.c:1765-1847): device
+ * selection (-d, pp_gpu.c:326), RNG seed (-r, :330), agent capacity (-m,
+ * :338; 0 = max(4 * initial agents, PP_DEFAULT_MAX_AGENTS)). Allocates every
+ * device buffer. device < 0 = current device. */
+int pp_sim_create(const pp_params_t *p, uint64_t seed, uint64_t capacity,
+	int device, pp_sim_t **out);
+
+/* Replaces init_cell / init_agent and the first statistics gathering of
+ * ppg_simulate (pp_gpu.cl:368-439, pp_gpu.c:619-642, :671-776): (re)sets the
+ * state to iteration 0 and writes its statistics (row 0) to *stats0 (may be
+ * NULL). */
+int pp_sim_init(pp_sim_t *s, pp_stats64_t *stats0);
+
+/* One iteration of the loop body of ppg_simulate (pp_gpu.c:671-935): grass,
+ * move, sort, cell index, actions, statistics. Writes the statistics of the
+ * new state to *out (may be NULL). Returns PP_OUT_OF_RESOURCES when the agents
+ * after births exceed the capacity (pp_gpu.c:847-852, :895-900). */
+int pp_sim_step(pp_sim_t *s, pp_stats64_t *out);
+
+/* iters iterations with no host synchronisation between them; rows 1..iters
+ * of the statistics go to out[0..iters-1] (may be NULL). On
+ * PP_OUT_OF_RESOURCES the simulation stops at the failing iteration (see
+ * pp_sim_error_step). */
+int pp_sim_run(pp_sim_t *s, uint32_t iters, pp_stats64_t *out);
+
+/* Replaces the ppg_simulate + pp_stats_save sequence of main
+ * (pp_gpu.c:1857-1866): init + p->iters iterations; rows 0..iters written to
+ * out (iters + 1 rows; may be NULL). */
+int pp_sim_simulate(pp_sim_t *s, pp_stats64_t *out);
+
+/* Completed iterations, live agents of the current state, the iteration at
+ * which an error stopped the run (or UINT32_MAX), and the sum over completed
+ * iterations of the live agents at the start of each (agent-steps). */
+uint32_t pp_sim_step_count(const pp_sim_t *s);
+uint64_t pp_sim_num_agents(const pp_sim_t *s);
+uint32_t pp_sim_error_step(const pp_sim_t *s);
+uint64_t pp_sim_agent_steps(const pp_sim_t *s);
+uint64_t pp_sim_capacity(const pp_sim_t *s);
+
+/* State dump, the counterpart of PPG_DUMP (pp_gpu.c:438-555): agents in slot
+ * order (cell = y * grid_x + x, energy, type) and the grass countdown of every
+ * cell. Each pointer may be NULL; arrays hold pp_sim_num_agents() agents and
+ * grid_xy cells. */
+int pp_sim_get_state(pp_sim_t *s, uint32_t *cell, int32_t *energy,
+	uint8_t *type, uint32_t *grass);
+
+/* Per-kernel timing, the counterpart of the reference's profiler summary and
+ * --prof-agg export (pp_gpu.c:318-321, :1868-1881; pp_common.c:248-260).
+ * When enabled, each kernel launch of init/step/run is bracketed by HIP events
+ * on the simulation's stream and no graph is used. */
+int pp_sim_set_profiling(pp_sim_t *s, int enable);
+/* Number of kernels tracked, and for kernel i its name, launch count and
+ * total / average duration in nanoseconds. */
+int pp_sim_prof_count(const pp_sim_t *s);
+int pp_sim_prof_get(pp_sim_t *s, int i, const char **name, uint64_t *launches,
+	double *total_ns);
+int pp_sim_prof_reset(pp_sim_t *s);
+/* "name"\ttotal_ns\trelative lines, as pp_export_prof_agg_info
+ * (pp_common.c:248-260). */
+int pp_sim_prof_export(pp_sim_t *s, const char *path);
+
+/* Radix passes of the agent sort for this grid (8-bit digits of the cell
+ * index) and the work-group / tile shape in use. */
+int pp_sim_info(const pp_sim_t *s, uint32_t *sort_passes, uint32_t *sort_tile,
+	uint32_t *action_tile, uint32_t *grid_move);
+
+/* Releases all device memory of the handle. NULL is accepted. */
+void pp_sim_destroy(pp_sim_t *s);
+
+/* Blocks until all work of the handle's stream has finished. */
+int pp_sim_sync(pp_sim_t *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

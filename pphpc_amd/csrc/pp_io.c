@@ -1,0 +1,110 @@
+/*
+ * pp_io.c -- parameter-file parser and statistics writer of the PPHPC step.
+ *
+ * Keeps the reference's file formats: the 12-key "KEY = value" parameter file
+ * read by pp_load_params (opencl/pp/pp_common.c:39-195) and the tab-separated
+ * per-iteration statistics written by pp_stats_save (pp_common.c:205-240).
+ */
+#include "pphpc.h"
+
+#include <stdio.h>
+#include <string.h>
+
+/* Keys in the order of the PPParameters fields (pp_common.in.h:129-157),
+ * grid_xy excluded; their bit in the "seen" mask is their index, as in the
+ * reference's check word (pp_common.c:56-169). */
+static const char *const pp_keys[12] = {
+	"INIT_SHEEP",
+	"SHEEP_GAIN_FROM_FOOD",
+	"SHEEP_REPRODUCE_THRESHOLD",
+	"SHEEP_REPRODUCE_PROB",
+	"INIT_WOLVES",
+	"WOLVES_GAIN_FROM_FOOD",
+	"WOLVES_REPRODUCE_THRESHOLD",
+	"WOLVES_REPRODUCE_PROB",
+	"GRASS_RESTART",
+	"GRID_X",
+	"GRID_Y",
+	"ITERS",
+};
+
+int pp_load_params(const char *path, pp_params_t *out) {
+	uint32_t field[12] = {0};
+	uint32_t seen = 0;
+	char key[100];
+	int value;
+	FILE *fp;
+
+	if (!out) return PP_INVALID_ARGS;
+	fp = fopen(path ? path : "config.txt", "r");
+	if (!fp) return PP_UNABLE_TO_OPEN_PARAMS_FILE;
+
+	/* One "%s = %d" record per scan, as pp_common.c:54 reads them. */
+	while (fscanf(fp, "%99s = %d", key, &value) != EOF) {
+		int idx = -1;
+		for (int k = 0; k < 12; k++) {
+			if (strcmp(key, pp_keys[k]) == 0) { idx = k; break; }
+		}
+		if (idx < 0 || (seen >> idx) & 1u) {
+			/* Unknown or repeated key (pp_common.c:60-62, :163-166). */
+			fclose(fp);
+			return PP_INVALID_PARAMS_FILE;
+		}
+		seen |= 1u << idx;
+		field[idx] = (uint32_t)value;
+	}
+	fclose(fp);
+	if (seen != 0xfffu) return PP_INVALID_PARAMS_FILE; /* pp_common.c:169 */
+
+	out->init_sheep = field[0];
+	out->sheep_gain_from_food = field[1];
+	out->sheep_reproduce_threshold = field[2];
+	out->sheep_reproduce_prob = field[3];
+	out->init_wolves = field[4];
+	out->wolves_gain_from_food = field[5];
+	out->wolves_reproduce_threshold = field[6];
+	out->wolves_reproduce_prob = field[7];
+	out->grass_restart = field[8];
+	out->grid_x = field[9];
+	out->grid_y = field[10];
+	out->grid_xy = field[9] * field[10]; /* pp_common.c:177 */
+	out->iters = field[11];
+	return PP_SUCCESS;
+}
+
+int pp_stats_save(const char *path, const pp_stats64_t *rows, uint32_t nrows,
+	const pp_params_t *p) {
+	FILE *fp;
+	if (!p || (nrows && !rows)) return PP_INVALID_ARGS;
+	fp = fopen(path ? path : "stats.txt", "w");
+	if (!fp) return PP_UNABLE_SAVE_STATS;
+	for (uint32_t i = 0; i < nrows; i++) {
+		const pp_stats64_t *r = rows + i;
+		double sheep_mean = r->sheep ? (double)r->sheep_en / (double)r->sheep : 0.0;
+		double wolf_mean = r->wolves ? (double)r->wolves_en / (double)r->wolves : 0.0;
+		double grass_mean = (double)r->grass_en / (double)p->grid_xy;
+		if (fprintf(fp, "%llu\t%llu\t%llu\t%f\t%f\t%f\n",
+				(unsigned long long)r->sheep, (unsigned long long)r->wolves,
+				(unsigned long long)r->grass, sheep_mean, wolf_mean,
+				grass_mean) < 0) {
+			fclose(fp);
+			return PP_UNABLE_SAVE_STATS;
+		}
+	}
+	if (fclose(fp) != 0) return PP_UNABLE_SAVE_STATS;
+	return PP_SUCCESS;
+}
+
+const char *pp_strerror(int code) {
+	switch (code) {
+	case PP_SUCCESS: return "success";
+	case PP_UNKNOWN_ARGS: return "unknown arguments";
+	case PP_INVALID_ARGS: return "invalid arguments";
+	case PP_LIBRARY_ERROR: return "error in HIP runtime";
+	case PP_UNABLE_TO_OPEN_PARAMS_FILE: return "unable to open parameters file";
+	case PP_INVALID_PARAMS_FILE: return "invalid parameters file";
+	case PP_UNABLE_SAVE_STATS: return "unable to save statistics";
+	case PP_OUT_OF_RESOURCES: return "agents above capacity (out of resources)";
+	default: return "unknown error code";
+	}
+}
