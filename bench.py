@@ -1,0 +1,271 @@
+"""Benchmark of the MI355X PPHPC step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+One "step" is one iteration of the model (move, sort, actions, statistics)
+over the whole population. The default workload is the 1600x1600 dense
+v2-style run: 409,600 sheep + 204,800 wolves, 4000 iterations from the seeded
+initial state (seed 0), capacity 4x the initial population. The timed region
+is exactly K iterations from iteration 0 with the state resident in HBM,
+bracketed by device synchronisation; value = agent-steps/s, i.e. the sum over
+the K iterations of the agents alive at the start of each, divided by the
+wall time.
+
+Also reported (one JSON line on stdout, progress on stderr):
+  roofline      dominant kernel of a profiled run of the same workload:
+                algorithmic bytes per launch (exact, from the run's event
+                counters) / average launch duration (HIP events on the
+                simulation's stream), against 8 TB/s
+  roofline_12800  the same for the 12800x12800 run (HBM-resident working set)
+  cpu_baseline  the single-threaded C oracle on a bounded sample of the same
+                workload, on this host's cores
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("agent-steps/s @1600² v2 plain vs batched; replicas/hr; "
+          "p99 run latency")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def v2s_params(L, iters=4000):
+    cells = L * L
+    return dict(init_sheep=cells * 1600 // 10000, sheep_gain_from_food=4,
+                sheep_reproduce_threshold=2, sheep_reproduce_prob=10,
+                init_wolves=cells * 800 // 10000, wolves_gain_from_food=20,
+                wolves_reproduce_threshold=2, wolves_reproduce_prob=5,
+                grass_restart=5, grid_x=L, grid_y=L, grid_xy=cells,
+                iters=iters)
+
+
+def v1_params(L, iters=4000):
+    cells = L * L
+    return dict(init_sheep=cells * 400 // 10000, sheep_gain_from_food=4,
+                sheep_reproduce_threshold=2, sheep_reproduce_prob=4,
+                init_wolves=cells * 200 // 10000, wolves_gain_from_food=20,
+                wolves_reproduce_threshold=2, wolves_reproduce_prob=5,
+                grass_restart=10, grid_x=L, grid_y=L, grid_xy=cells,
+                iters=iters)
+
+
+WORKLOADS = {
+    "v2s1600": lambda: v2s_params(1600),
+    "v2s12800": lambda: v2s_params(12800),
+    "v1_100": lambda: v1_params(100),
+}
+
+
+def kernel_bytes(name, c, passes):
+    """Algorithmic (compulsory) HBM bytes of all launches of one kernel over a
+    run, from the run's exact event sums c (agent-steps N, live after move L,
+    agents after actions N', occupied cells O, grass eaten E):
+      move       read cell+energy+type 9 B, write key+value 8 B per agent: 17 N
+      sort_pass  pass 0 reads N pairs and writes the L survivors; every later
+                 pass reads and writes L pairs: 8N + 8L + 16 L (P-1)
+      action     read L sorted pairs 8 B, write the new state 9 B per agent,
+                 read the grass of each occupied cell, write each eaten one:
+                 8 L + 9 N' + 4 O + 4 E
+    """
+    N, L, Np, O, E = (c["agent_steps"], c["live"], c["out"], c["occupied"],
+                      c["eaten"])
+    if name == "move":
+        return 17 * N
+    if name == "sort_pass":
+        return 8 * N + 8 * L + 16 * L * (passes - 1)
+    if name == "action":
+        return 8 * L + 9 * Np + 4 * O + 4 * E
+    return None
+
+
+def measure_roofline(params, capacity, steps, seed=0):
+    """Profiled run: per-kernel event timing on the simulation's stream."""
+    from pphpc_amd import Params, Simulation
+    sim = Simulation(Params.from_values(**params), seed=seed,
+                     capacity=capacity, device=0)
+    passes = sim.info()["sort_passes"]
+    sim.init()
+    sim.run(min(steps, 4), collect=False)      # warm the code paths
+    sim.init()
+    sim.set_profiling(True)
+    sim.profile_reset()
+    sim.run(steps, collect=False)
+    prof = sim.profile()
+    c = sim.counters()
+    sim.close()
+    step_kernels = {k: v for k, v in prof.items()
+                    if k in ("move", "sort_pass", "action", "finalize")}
+    dom = max(step_kernels, key=lambda k: step_kernels[k][1])
+    launches, ns = step_kernels[dom]
+    total_bytes = kernel_bytes(dom, c, passes)
+    per_kernel = {}
+    for k, (nl, t) in step_kernels.items():
+        b = kernel_bytes(k, c, passes)
+        per_kernel[k] = {
+            "launches": nl, "avg_us": round(t / max(nl, 1) / 1e3, 3),
+            "GBps": round(b / t, 1) if b and t else None,
+            "share": round(t / sum(v[1] for v in step_kernels.values()), 3)}
+    achieved = total_bytes / ns  # bytes/ns == GB/s
+    return {
+        "kernel": dom,
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "bytes_per_launch": int(total_bytes / launches),
+        "avg_launch_us": round(ns / launches / 1e3, 3),
+        "profiled_steps": steps,
+        "per_kernel": per_kernel,
+    }
+
+
+def cpu_baseline(params, capacity, sample_steps):
+    """The single-threaded C oracle (a port of the reference step semantics)
+    on the first `sample_steps` iterations of the same workload."""
+    exe = os.path.join(ROOT, "oracle", "build", "pp_oracle")
+    if not os.path.exists(exe):
+        return None
+    keys = ("INIT_SHEEP", "SHEEP_GAIN_FROM_FOOD", "SHEEP_REPRODUCE_THRESHOLD",
+            "SHEEP_REPRODUCE_PROB", "INIT_WOLVES", "WOLVES_GAIN_FROM_FOOD",
+            "WOLVES_REPRODUCE_THRESHOLD", "WOLVES_REPRODUCE_PROB",
+            "GRASS_RESTART", "GRID_X", "GRID_Y", "ITERS")
+    fields = ("init_sheep", "sheep_gain_from_food",
+              "sheep_reproduce_threshold", "sheep_reproduce_prob",
+              "init_wolves", "wolves_gain_from_food",
+              "wolves_reproduce_threshold", "wolves_reproduce_prob",
+              "grass_restart", "grid_x", "grid_y", "iters")
+    with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:
+        for k, fl in zip(keys, fields):
+            f.write(f"{k} = {params[fl]}\n")
+        path = f.name
+    try:
+        out = subprocess.run([exe, "-p", path, "-n", str(sample_steps), "-q",
+                              "-m", str(capacity)], capture_output=True,
+                             text=True, timeout=600, check=True).stdout
+    finally:
+        os.unlink(path)
+    kv = dict(tok.split("=") for tok in out.split())
+    return {"value": float(kv["agent_steps_per_s"]), "unit": "agent-steps/s",
+            "cores": 1, "kind": "port",
+            "sample": f"first {sample_steps} iterations of the same workload "
+                      f"(seed 0), single-threaded C oracle; "
+                      f"{float(kv['seconds']):.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="v2s1600", choices=list(WORKLOADS))
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--roofline-steps", type=int, default=400)
+    ap.add_argument("--big-roofline-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=150)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-big", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the step on a "
+              "single MI355X (no multi-GPU path); use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from pphpc_amd import Params, Simulation
+
+    params = WORKLOADS[args.workload]()
+    n0 = params["init_sheep"] + params["init_wolves"]
+    capacity = 4 * n0
+    log(f"workload {args.workload}: {params['grid_x']}x{params['grid_y']}, "
+        f"{n0} agents, capacity {capacity}")
+
+    sim = Simulation(Params.from_values(**params), seed=args.seed,
+                     capacity=capacity, device=0)
+    info = sim.info()
+    sim.init()
+    if args.warmup:
+        sim.run(args.warmup, collect=False)
+    sim.init()
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(args.steps, collect=False)
+    sim.sync()
+    t1 = time.perf_counter()
+    agent_steps = sim.agent_steps
+    n_end = sim.num_agents
+    sim.close()
+    wall = t1 - t0
+    value = agent_steps / wall
+    log(f"{args.steps} steps in {wall:.3f} s: {value:.4e} agent-steps/s "
+        f"({n_end} agents at the end)")
+
+    log("profiled run for the roofline")
+    roof = measure_roofline(params, capacity, min(args.roofline_steps,
+                                                  args.steps), args.seed)
+    log(f"dominant kernel {roof['kernel']}: {roof['achieved']} GB/s")
+
+    roof_big = None
+    if not args.no_big:
+        big = v2s_params(12800)
+        nb = big["init_sheep"] + big["init_wolves"]
+        log("profiled 12800x12800 run for the HBM roofline")
+        roof_big = measure_roofline(big, 4 * nb, args.big_roofline_steps,
+                                    args.seed)
+        log(f"12800^2 dominant kernel {roof_big['kernel']}: "
+            f"{roof_big['achieved']} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"cpu baseline: oracle, {args.cpu_steps} iterations")
+        cpu = cpu_baseline(params, capacity, args.cpu_steps)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic: seeded initial state (Philox, seed "
+                f"{args.seed}), no dataset",
+        "config": {
+            "workload": f"{args.workload}: {params['grid_x']}x"
+                        f"{params['grid_y']} torus, {params['init_sheep']} "
+                        f"sheep + {params['init_wolves']} wolves, "
+                        f"{args.steps} iterations, plain (one replica)",
+            "grid": [params["grid_x"], params["grid_y"]],
+            "capacity": capacity,
+            "seed": args.seed,
+            "sort_passes": info["sort_passes"],
+            "sort_tile": info["sort_tile"],
+            "agent_steps": agent_steps,
+            "parallelism": "single GPU",
+        },
+        "roofline": roof,
+        "roofline_12800": roof_big,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

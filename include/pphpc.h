@@ -127,6 +127,12 @@ uint32_t pp_sim_error_step(const pp_sim_t *s);
 uint64_t pp_sim_agent_steps(const pp_sim_t *s);
 uint64_t pp_sim_capacity(const pp_sim_t *s);
 
+/* Exact event sums over the completed iterations since the last init:
+ * [0] agents at the start of each iteration (agent-steps), [1] agents alive
+ * after the move, [2] agents after actions and births, [3] occupied cells,
+ * [4] grass cells eaten. Used to state each kernel's algorithmic bytes. */
+int pp_sim_counters(const pp_sim_t *s, uint64_t out[5]);
+
 /* State dump, the counterpart of PPG_DUMP (pp_gpu.c:438-555): agents in slot
  * order (cell = y * grid_x + x, energy, type) and the grass countdown of every
  * cell. Each pointer may be NULL; arrays hold pp_sim_num_agents() agents and

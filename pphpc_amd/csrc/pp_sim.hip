@@ -87,6 +87,11 @@ struct Ctl {
 	unsigned long long eaten;       /* grass eaten in this iteration */
 	unsigned long long acc[4];      /* sheep, wolves, sheep energy, wolf energy */
 	unsigned long long agent_steps; /* sum of n over completed iterations */
+	unsigned long long occupied;    /* occupied cells in this iteration */
+	unsigned long long sum_live;    /* sums over completed iterations of: */
+	unsigned long long sum_out;     /*   live, out_n, occupied, eaten */
+	unsigned long long sum_occ;
+	unsigned long long sum_eaten;
 };
 
 struct Params {
@@ -219,6 +224,11 @@ __global__ __launch_bounds__(WG) void k_init_reset(Ctl *ctl,
 		ctl->eaten = 0;
 		for (int i = 0; i < 4; i++) ctl->acc[i] = 0;
 		ctl->agent_steps = 0;
+		ctl->occupied = 0;
+		ctl->sum_live = 0;
+		ctl->sum_out = 0;
+		ctl->sum_occ = 0;
+		ctl->sum_eaten = 0;
 	}
 }
 
@@ -533,7 +543,7 @@ __global__ __launch_bounds__(WG) void k_action(
 	__shared__ uint32_t s_tmp[NWAVE];
 	__shared__ uint32_t s_tile;
 	__shared__ uint32_t s_off;
-	__shared__ unsigned long long s_red[NWAVE][5];
+	__shared__ unsigned long long s_red[NWAVE][6];
 
 	if (ctl->error) return;
 	const uint32_t L = ctl->live, step = ctl->step;
@@ -541,7 +551,7 @@ __global__ __launch_bounds__(WG) void k_action(
 	const uint32_t epoch = ctl->epoch + ACTION_SLOT + 1;
 	const uint32_t alive_at = step + 1;
 	const int tid = threadIdx.x;
-	unsigned long long acc[5] = {0, 0, 0, 0, 0};
+	unsigned long long acc[6] = {0, 0, 0, 0, 0, 0};
 	bool overflow = false;
 
 	for (;;) {
@@ -560,6 +570,7 @@ __global__ __launch_bounds__(WG) void k_action(
 			end = p + 1;
 			while (end < L && keys[end] == c) end++;
 			const uint32_t k = end - p;
+			acc[5] += 1;
 			if (k > 1) {
 				uint4 r = make_uint4(0, 0, 0, 0);
 				for (uint32_t i = k - 1, m = 0; i > 0; i--, m++) {
@@ -667,9 +678,9 @@ __global__ __launch_bounds__(WG) void k_action(
 		__syncthreads();
 	}
 	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, step);
-	unsigned long long *dst[5] = {&ctl->acc[0], &ctl->acc[1], &ctl->acc[2],
-		&ctl->acc[3], &ctl->eaten};
-	wg_add<5>(acc, s_red, dst);
+	unsigned long long *dst[6] = {&ctl->acc[0], &ctl->acc[1], &ctl->acc[2],
+		&ctl->acc[3], &ctl->eaten, &ctl->occupied};
+	wg_add<6>(acc, s_red, dst);
 }
 
 /* --------------------------------------------------------------- finalize */
@@ -705,11 +716,16 @@ __global__ __launch_bounds__(WG) void k_finalize(Ctl *ctl,
 				ctl->cd_sum = S;
 				write_row(ctl, &stats[(s + 1) % STATS_RING], P);
 				ctl->agent_steps += ctl->n;
+				ctl->sum_live += ctl->live;
+				ctl->sum_out += ctl->out_n;
+				ctl->sum_occ += ctl->occupied;
+				ctl->sum_eaten += ctl->eaten;
 				ctl->n = ctl->out_n;
 				ctl->step = s + 1;
 			}
 			for (int i = 0; i < 4; i++) ctl->acc[i] = 0;
 			ctl->eaten = 0;
+			ctl->occupied = 0;
 			ctl->live = 0;
 			ctl->out_n = 0;
 			ctl->epoch += 8;
@@ -1176,6 +1192,16 @@ uint64_t pp_sim_agent_steps(const pp_sim_t *s) {
 }
 
 uint64_t pp_sim_capacity(const pp_sim_t *s) { return s ? s->cap : 0; }
+
+int pp_sim_counters(const pp_sim_t *s, uint64_t out[5]) {
+	if (!s || !out) return PP_INVALID_ARGS;
+	out[0] = s->ctl_host->agent_steps;
+	out[1] = s->ctl_host->sum_live;
+	out[2] = s->ctl_host->sum_out;
+	out[3] = s->ctl_host->sum_occ;
+	out[4] = s->ctl_host->sum_eaten;
+	return PP_SUCCESS;
+}
 
 int pp_sim_get_state(pp_sim_t *s, uint32_t *cell, int32_t *energy,
 	uint8_t *type, uint32_t *grass) {

@@ -111,6 +111,7 @@ def _lib():
         "pp_sim_error_step": (u32, [vp]),
         "pp_sim_agent_steps": (u64, [vp]),
         "pp_sim_capacity": (u64, [vp]),
+        "pp_sim_counters": (i32, [vp, P(u64)]),
         "pp_sim_get_state": (i32, [vp, vp, vp, vp, vp]),
         "pp_sim_set_profiling": (i32, [vp, i32]),
         "pp_sim_prof_count": (i32, [vp]),
@@ -252,6 +253,13 @@ class Simulation:
     @property
     def capacity(self) -> int:
         return _lib().pp_sim_capacity(self._h)
+
+    def counters(self) -> dict:
+        """Exact sums over completed iterations since init."""
+        out = (ctypes.c_uint64 * 5)()
+        self._check(_lib().pp_sim_counters(self._h, out), "counters")
+        return dict(zip(("agent_steps", "live", "out", "occupied", "eaten"),
+                        tuple(out)))
 
     def info(self) -> dict:
         a, b, c, d = (ctypes.c_uint32() for _ in range(4))
