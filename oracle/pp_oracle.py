@@ -89,6 +89,8 @@ def params_from(p) -> OParams:
     """OParams from any object with the PPParameters field names."""
     o = OParams()
     for f in PARAM_FIELDS:
+        if f == "grid_xy":
+            continue
         setattr(o, f, int(getattr(p, f) if not isinstance(p, dict) else p[f]))
     o.grid_xy = o.grid_x * o.grid_y
     return o

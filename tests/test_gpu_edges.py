@@ -1,0 +1,203 @@
+"""GPU: golden runs, the `pp` command line, edge cases and full-size
+properties. Every comparison is bit-exact against the oracle or a fixture."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, ROOT, assert_state_equal, gpu_params, v1, v2s
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _sim(d, seed=0, capacity=0):
+    from pphpc_amd import Simulation
+    return Simulation(gpu_params(d), seed=seed, capacity=capacity, device=0)
+
+
+def _rows(sim, iters):
+    return np.array([sim.init().as_tuple()] +
+                    [r.as_tuple() for r in sim.run(iters)], dtype=np.uint64)
+
+
+@pytest.mark.parametrize("fixture", ["v1_100_seed0.npz", "v2s_1600_seed0.npz"])
+def test_golden_full_runs(fixture):
+    """Config 1 (100x100 v1) and config 2 (1600x1600 v2-style, capacity 4x
+    the initial agents): all 4001 rows equal the oracle's fixture."""
+    _case(f"golden {fixture}")
+    g = np.load(os.path.join(GOLDEN, fixture))
+    d = json.loads(str(g["params"]))
+    sim = _sim(d, seed=int(g["seed"]), capacity=int(g["capacity"]))
+    rows = _rows(sim, d["iters"])
+    sim.close()
+    bad = np.nonzero((rows != g["rows"]).any(axis=1))[0]
+    assert len(bad) == 0, f"first bad row {bad[0]}"
+
+
+def test_cli_writes_the_oracle_stats_file(tmp_path):
+    """`pp -p FILE -s FILE -r SEED` writes the same bytes as the oracle CLI."""
+    _case("cli config1")
+    params = os.path.join(ROOT, "configs", "v1_100.txt")
+    ours, ref = str(tmp_path / "gpu.txt"), str(tmp_path / "oracle.txt")
+    prof = str(tmp_path / "prof.tsv")
+    r = subprocess.run([os.path.join(ROOT, "pphpc_amd", "bin", "pp"), "-p",
+                        params, "-s", ours, "-r", "0", "-d", "0", "-i", prof],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    subprocess.run([os.path.join(ROOT, "oracle", "build", "pp_oracle"), "-p",
+                    params, "-s", ref, "-r", "0"], check=True,
+                   capture_output=True, timeout=120)
+    assert open(ours, "rb").read() == open(ref, "rb").read()
+    lines = open(prof).read().splitlines()
+    names = [ln.split("\t")[0].strip('"') for ln in lines]
+    assert {"move", "sort_pass", "action", "finalize"} <= set(names)
+    assert abs(sum(float(ln.split("\t")[2]) for ln in lines) - 1.0) < 1e-6
+
+
+def test_extinction_and_empty_population():
+    """Wolves without sheep starve: population reaches zero and stays; an
+    empty initial population is valid."""
+    _case("extinction")
+    from oracle.pp_oracle import simulate
+    d = dict(v1(30), init_sheep=0, init_wolves=50, iters=80)
+    sim = _sim(d, seed=2, capacity=1000)
+    rows = _rows(sim, 80)
+    assert rows[-1][0] == 0 and rows[-1][1] == 0
+    assert np.array_equal(rows, simulate(d, seed=2, capacity=1000, iters=80))
+    sim.close()
+    d = dict(v1(10), init_sheep=0, init_wolves=0, iters=5)
+    sim = _sim(d, capacity=16)
+    rows = _rows(sim, 5)
+    assert np.array_equal(rows, simulate(d, capacity=16, iters=5))
+    sim.close()
+
+
+def test_capacity_overflow_matches_oracle():
+    """PP_OUT_OF_RESOURCES at the same iteration as the oracle, and rows
+    before it identical (pp_gpu.c:847-852 semantics)."""
+    _case("capacity overflow")
+    from oracle.pp_oracle import Oracle
+    from pphpc_amd import PPError, PP_OUT_OF_RESOURCES
+    d = v1(100)
+    cap = 700
+    o = Oracle(d, seed=0, capacity=cap)
+    rc, ref = o.run(500)
+    assert rc == -8
+    fail_step = o.step_count          # steps completed before the failure
+    o.close()
+    sim = _sim(d, capacity=cap)
+    sim.init()
+    with pytest.raises(PPError) as e:
+        sim.run(500)
+    assert e.value.code == PP_OUT_OF_RESOURCES
+    assert sim.error_step == fail_step
+    assert sim.step_count == fail_step
+    sim.close()
+    # rows up to the failure agree
+    sim = _sim(d, capacity=cap)
+    sim.init()
+    got = np.array([r.as_tuple() for r in sim.run(fail_step)], np.uint64)
+    assert np.array_equal(got, ref[:fail_step])
+    sim.close()
+
+
+def test_large_grass_restart_uses_global_ring():
+    """R + 1 > 4096 takes the global-atomic histogram path at init."""
+    _case("large R")
+    from oracle.pp_oracle import Oracle
+    d = dict(v2s(60), grass_restart=9000)
+    sim = _sim(d, seed=4)
+    o = Oracle(d, seed=4, capacity=sim.capacity)
+    assert sim.init().as_tuple() == o.stats()
+    assert_state_equal(sim.state(), o.state(), "init R=9000")
+    for t in range(30):
+        assert sim.step().as_tuple() == (o.step(), o.stats())[1]
+    assert_state_equal(sim.state(), o.state(), "R=9000 step 30")
+    sim.close()
+    o.close()
+
+
+def test_reinit_graph_and_profiling_modes_agree():
+    """Re-init reproduces the run; graph replay, per-launch and profiled
+    modes give identical rows."""
+    _case("modes")
+    d = v2s(200)
+    sim = _sim(d, seed=3)
+    a = _rows(sim, 300)
+    b = _rows(sim, 300)                       # re-init on the same handle
+    os.environ["PPHPC_NO_GRAPH"] = "1"
+    try:
+        c = _rows(sim, 300)
+    finally:
+        del os.environ["PPHPC_NO_GRAPH"]
+    sim.set_profiling(True)
+    e = _rows(sim, 300)
+    prof = sim.profile()
+    sim.close()
+    assert np.array_equal(a, b) and np.array_equal(a, c) and \
+        np.array_equal(a, e)
+    assert prof["move"][0] == 300 and prof["action"][0] == 300
+    assert prof["sort_pass"][0] == 300 * 2     # 40000 cells: 16 bits
+
+
+def test_step_by_step_equals_run():
+    _case("step vs run")
+    d = v2s(128)
+    sim = _sim(d, seed=8)
+    a = _rows(sim, 50)
+    sim.init()
+    b = [r.as_tuple() for r in [sim.step() for _ in range(50)]]
+    assert np.array_equal(a[1:], np.array(b, np.uint64))
+    sim.close()
+
+
+def test_two_simulations_share_the_gpu():
+    """Independent handles on one device do not interfere."""
+    _case("two handles")
+    d1, d2 = v2s(100), v1(150)
+    s1, s2 = _sim(d1, seed=1), _sim(d2, seed=2)
+    s1.init()
+    s2.init()
+    r1, r2 = [], []
+    for _ in range(20):
+        r1 += [r.as_tuple() for r in s1.run(25)]
+        r2 += [r.as_tuple() for r in s2.run(25)]
+    from oracle.pp_oracle import simulate
+    assert np.array_equal(np.array(r1, np.uint64),
+                          simulate(d1, 1, s1.capacity, 500)[1:])
+    assert np.array_equal(np.array(r2, np.uint64),
+                          simulate(d2, 2, s2.capacity, 500)[1:])
+    s1.close()
+    s2.close()
+
+
+def test_full_size_12800_properties():
+    """12800x12800 (163.84 M cells, 39.3 M agents), capacity 4x: the run is
+    deterministic (two runs, same rows), the statistics agree with the state
+    (counts, energy sums, grass from the materialised countdowns), and the
+    state is ordered by cell."""
+    _case("12800^2 properties")
+    d = v2s(12800)
+    n0 = d["init_sheep"] + d["init_wolves"]
+    sim = _sim(d, seed=0, capacity=4 * n0)
+    a = _rows(sim, 4)
+    cell, energy, typ, grass = sim.state()
+    b = _rows(sim, 4)
+    sim.close()
+    assert np.array_equal(a, b)
+    last = a[-1]
+    assert last[0] == (typ == 0).sum() and last[1] == (typ == 1).sum()
+    assert last[3] == energy[typ == 0].astype(np.int64).sum()
+    assert last[4] == energy[typ == 1].astype(np.int64).sum()
+    assert last[2] == (grass == 0).sum()
+    assert last[5] == grass.astype(np.uint64).sum()
+    assert (np.diff(cell.astype(np.int64)) >= 0).all()
+    assert (energy >= 0).all() and cell.max() < 12800 * 12800
