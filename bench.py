@@ -34,6 +34,8 @@ sys.path.insert(0, ROOT)
 METRIC = ("agent-steps/s @1600² v2 plain vs batched; replicas/hr; "
           "p99 run latency")
 HBM_PEAK_GBS = 8000.0
+STEP_KERNELS = ("sort_up", "sort_scan", "sort_down", "act", "act_scan",
+                "emit_move")
 
 
 def log(msg):
@@ -71,21 +73,26 @@ def kernel_bytes(name, c, passes):
     """Algorithmic (compulsory) HBM bytes of all launches of one kernel over a
     run, from the run's exact event sums c (agent-steps N, live after move L,
     agents after actions N', occupied cells O, grass eaten E):
-      move       read cell+energy+type 9 B, write key+value 8 B per agent: 17 N
-      sort_pass  pass 0 reads N pairs and writes the L survivors; every later
-                 pass reads and writes L pairs: 8N + 8L + 16 L (P-1)
-      action     read L sorted pairs 8 B, write the new state 9 B per agent,
-                 read the grass of each occupied cell, write each eaten one:
-                 8 L + 9 N' + 4 O + 4 E
+      sort_down  pass 0 reads the N moved pairs (8 B) and writes the L
+                 survivors; every later pass reads and writes L pairs:
+                 8N + 8L + 16L(P-1)
+      sort_up    later passes read the L keys once more: 4L(P-1)
+      act        read L sorted pairs, write the L updated values back, read
+                 the grass of each occupied cell, write each eaten one:
+                 12L + 4O + 4E
+      emit_move  read L sorted pairs, write the new state (9 B) and its moved
+                 pair (8 B) per agent: 8L + 17N'
     """
     N, L, Np, O, E = (c["agent_steps"], c["live"], c["out"], c["occupied"],
                       c["eaten"])
-    if name == "move":
-        return 17 * N
-    if name == "sort_pass":
+    if name == "sort_down":
         return 8 * N + 8 * L + 16 * L * (passes - 1)
-    if name == "action":
-        return 8 * L + 9 * Np + 4 * O + 4 * E
+    if name == "sort_up":
+        return 4 * L * (passes - 1) if passes > 1 else None
+    if name == "act":
+        return 12 * L + 4 * O + 4 * E
+    if name == "emit_move":
+        return 8 * L + 17 * Np
     return None
 
 
@@ -105,8 +112,9 @@ def measure_roofline(params, capacity, steps, seed=0):
     c = sim.counters()
     sim.close()
     step_kernels = {k: v for k, v in prof.items()
-                    if k in ("move", "sort_pass", "action", "finalize")}
-    dom = max(step_kernels, key=lambda k: step_kernels[k][1])
+                    if k in STEP_KERNELS and v[0] > 0}
+    dom = max((k for k in step_kernels if kernel_bytes(k, c, passes)),
+              key=lambda k: step_kernels[k][1])
     launches, ns = step_kernels[dom]
     total_bytes = kernel_bytes(dom, c, passes)
     per_kernel = {}

@@ -9,32 +9,41 @@
  *   ---------------------------------------  -----------------------------------
  *   init_cell, init_agent (pp_gpu.cl:368)    k_init_cells, k_init_agents
  *   grass (pp_gpu.cl:448)                    lazy regrow step per cell plus an
- *                                            O(1) ring update in k_finalize
- *   move_agent (pp_gpu.cl:744)               k_move (+ fused radix histograms)
- *   cl_ops sbitonic sort (pp_gpu.c:826)      k_sort_pass: LDS-staged onesweep
- *                                            LSD radix sort, 8-bit digits of the
- *                                            cell, stable (= key cell<<32|id)
- *   find_cell_idx (pp_gpu.cl:834)            segment heads found in k_action
- *   action_agent (pp_gpu.cl:883)             k_action: one lane per occupied
- *                                            cell, shuffled order, compaction
- *                                            and spawning by a block scan with
- *                                            decoupled look-back
- *   reduce_grass1/2, reduce_agent1/2         fused reductions (k_action,
- *   (pp_gpu.cl:485-736)                      k_finalize) with exact 64-bit sums
+ *                                            O(1) ring update in k_act_scan
+ *   move_agent (pp_gpu.cl:744)               k_emit_move (fused with writing
+ *                                            the new state), k_move0 after init
+ *   cl_ops sbitonic sort (pp_gpu.c:826)      LSD radix sort of (cell, value)
+ *                                            pairs: per pass k_sort_up (tile
+ *                                            histograms), k_sort_scan (one
+ *                                            work-group per digit), k_sort_down
+ *                                            (LDS-staged wave64 ranking and
+ *                                            scatter); stable, so the order is
+ *                                            that of the key cell<<32 | id
+ *   find_cell_idx (pp_gpu.cl:834)            segment heads found in k_act
+ *   action_agent (pp_gpu.cl:883)             k_act: one lane per occupied cell,
+ *                                            shuffled order; k_act_scan: output
+ *                                            offsets; k_emit_move: compaction
+ *                                            and spawning
+ *   reduce_grass1/2, reduce_agent1/2         fused reductions (k_act,
+ *   (pp_gpu.cl:485-736)                      k_act_scan) with exact 64-bit sums
  *   stats readback every iteration           device ring of stats rows, one
  *   (pp_gpu.c:766, :840)                     copy per chunk of iterations
+ *
+ * No kernel waits on another work-group: every cross-group dependency is a
+ * kernel boundary, so the step has no spin loops.
  *
  * All arithmetic is integer. The state, its ordering and every random draw are
  * defined so that a single-threaded CPU implementation reproduces the same
  * statistics bit for bit; see the header of each kernel.
  *
- * Device state of one simulation (capacity = A agents, C cells):
+ * Device state of one simulation (capacity = A agents, C cells, T sort tiles):
  *   agents (SoA)      cell u32[A], energy i32[A], type u8[A]       9 B/agent
- *   sort pairs        key u32[A], val u32[A], two buffers         16 B/agent
+ *   sort pairs        key u32[A], val u32[A], three buffers       24 B/agent
  *   grass             regrow step u32[C]                           4 B/cell
  *   grass ring        u64[R+1] cells per regrow step (mod R+1)
- *   look-back         u64 per (sort tile, digit) and per action tile
- *   control block     counters, live count, step, error, epoch
+ *   tile histograms   u32[radix][T], digit totals u32[radix]
+ *   action tiles      u32 output count per 1024 sorted positions
+ *   control block     counters, live count, step, error
  *   stats ring        pp_stats64_t[4096]
  */
 #include <hip/hip_runtime.h>
@@ -54,13 +63,16 @@ namespace {
 constexpr int WG = 256;             /* threads per work-group (4 wave64) */
 constexpr int WAVE = 64;
 constexpr int NWAVE = WG / WAVE;
-constexpr int RADIX = 256;          /* 8-bit digits */
-constexpr int MAX_PASSES = 4;       /* 32-bit cell index */
-constexpr int ACTION_SLOT = 7;      /* tile counter / epoch slot of k_action */
+constexpr int RADIX_MAX = 2048;     /* digits of at most 11 bits */
 constexpr uint32_t DEAD_KEY = 0xFFFFFFFFu;
+constexpr uint32_t NO_DIGIT = 0xFFFFFFFFu;
 constexpr uint32_t STATS_RING = 4096;
+constexpr int M_IPT = 4;            /* agents per thread (move) */
+constexpr int A_IPT = 4;            /* sorted positions per thread (action) */
+constexpr int A_TILE = WG * A_IPT;
+constexpr int A_STAGE = 2 * A_TILE; /* staged outputs per action tile */
 
-/* Packed agent value carried through the sort and the action kernel. */
+/* Packed agent value carried through the sort and the action kernels. */
 constexpr uint32_t V_ENERGY = 0x00FFFFFFu; /* energy, 24 bits */
 constexpr uint32_t V_WOLF = 1u << 24;      /* type (0 sheep, 1 wolf) */
 constexpr uint32_t V_REPRO = 1u << 25;     /* reproduction draw < probability */
@@ -68,26 +80,20 @@ constexpr uint32_t V_KILLED = 1u << 26;    /* eaten by a wolf this step */
 constexpr uint32_t V_CHILD = 1u << 27;     /* reproduced this step */
 constexpr uint32_t V_ODD = 1u << 28;       /* energy before the split was odd */
 
-/* Look-back word: epoch (32) | flag (2) | value (30). */
-constexpr uint64_t LB_AGG = 1;
-constexpr uint64_t LB_INC = 2;
-
 struct Ctl {
 	uint32_t n;        /* agents of the current state */
 	uint32_t step;     /* completed iterations */
 	int32_t error;     /* 0 or a PP_* code; every kernel is a no-op once set */
 	uint32_t err_step; /* iteration at which error was set */
-	uint32_t live;     /* agents surviving the move of this iteration */
+	uint32_t live[2];  /* survivors of the move of iteration s, in [s & 1] */
 	uint32_t out_n;    /* agents after actions and births */
-	uint32_t epoch;    /* look-back generation, +8 per iteration and init */
 	uint32_t pad;
-	uint32_t tile_ctr[8];
 	unsigned long long dead_cells;  /* grass cells with countdown > 0 */
 	unsigned long long cd_sum;      /* sum of grass countdowns */
 	unsigned long long eaten;       /* grass eaten in this iteration */
 	unsigned long long acc[4];      /* sheep, wolves, sheep energy, wolf energy */
-	unsigned long long agent_steps; /* sum of n over completed iterations */
 	unsigned long long occupied;    /* occupied cells in this iteration */
+	unsigned long long agent_steps; /* sum of n over completed iterations */
 	unsigned long long sum_live;    /* sums over completed iterations of: */
 	unsigned long long sum_out;     /*   live, out_n, occupied, eaten */
 	unsigned long long sum_occ;
@@ -103,50 +109,16 @@ struct Params {
 	uint32_t sheep_prob, wolves_prob;
 	uint32_t init_sheep, n0;
 	uint32_t cap;
-	uint32_t passes;
+	uint32_t passes;     /* radix passes over the cell index */
+	uint32_t digit_bits; /* bits per pass (<= 11) */
+	uint32_t sort_tile;  /* positions per sort tile */
+	uint32_t hist_cols;  /* tile capacity of the histogram table */
 };
 
 /* ---------------------------------------------------------------- helpers */
 
-__device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint64_t flag,
-	uint32_t v) {
-	return ((uint64_t)epoch << 32) | (flag << 30) | (uint64_t)v;
-}
-
-__device__ __forceinline__ void lb_store(unsigned long long *p, uint64_t v) {
-	__hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELAXED,
-		__HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ uint64_t lb_load(unsigned long long *p) {
-	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-/* Exclusive prefix of the tiles before `tile` for one counter, by decoupled
- * look-back over words published by earlier tiles of the same epoch. Every
- * earlier tile was claimed (atomic counter) by a work-group that is already
- * running and publishes its aggregate before looking back itself, so the wait
- * always ends. */
-__device__ __forceinline__ uint32_t lookback(unsigned long long *base,
-	uint32_t stride, uint32_t tile, uint32_t epoch) {
-	uint32_t excl = 0;
-	int64_t t = (int64_t)tile - 1;
-	while (t >= 0) {
-		const uint64_t x = lb_load(base + (size_t)t * stride);
-		const uint32_t ep = (uint32_t)(x >> 32);
-		const uint32_t fl = (uint32_t)(x >> 30) & 3u;
-		if (ep != epoch || fl == 0) {
-			__builtin_amdgcn_s_sleep(1);
-			continue;
-		}
-		excl += (uint32_t)(x & 0x3FFFFFFFu);
-		if (fl == LB_INC) break;
-		t--;
-	}
-	return excl;
-}
-
-/* Work-group exclusive scan of one value per thread (256 threads). */
+/* Work-group exclusive scan of one value per thread (NT threads). */
+template <int NT>
 __device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t *s_tmp,
 	uint32_t &total) {
 	const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
@@ -160,7 +132,7 @@ __device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t *s_tmp,
 	__syncthreads();
 	uint32_t pre = 0, tot = 0;
 #pragma unroll
-	for (int i = 0; i < NWAVE; i++) {
+	for (int i = 0; i < NT / WAVE; i++) {
 		const uint32_t t = s_tmp[i];
 		pre += (i < w) ? t : 0u;
 		tot += t;
@@ -176,7 +148,7 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 	return v;
 }
 
-/* Work-group sum of up to 5 counters into global atomics (one per group). */
+/* Work-group sum of K counters into global atomics (one per group). */
 template <int K>
 __device__ __forceinline__ void wg_add(unsigned long long (&v)[K],
 	unsigned long long (*s_red)[K], unsigned long long *const dst[K]) {
@@ -200,31 +172,75 @@ __device__ __forceinline__ void set_error(Ctl *ctl, int32_t code,
 	if (atomicCAS(&ctl->error, 0, code) == 0) ctl->err_step = step;
 }
 
+/* Adds 1 to s_hist[d] for every lane of the wave, merging runs of equal
+ * digits in consecutive lanes first (agents arrive ordered by cell, so high
+ * digits form long runs). NO_DIGIT lanes count nothing. */
+__device__ __forceinline__ void wave_hist_add(uint32_t *s_hist, uint32_t d) {
+	const int lane = threadIdx.x & (WAVE - 1);
+	const uint32_t prev = __shfl_up(d, 1);
+	const bool head = lane == 0 || d != prev;
+	const uint64_t heads = __ballot(head);
+	if (head && d != NO_DIGIT) {
+		const uint64_t after = heads & ~((2ull << lane) - 1ull);
+		const uint32_t next = after ? (uint32_t)__builtin_ctzll(after)
+			: (uint32_t)WAVE;
+		atomicAdd(&s_hist[d], next - (uint32_t)lane);
+	}
+}
+
+/* One move: Philox block (id, step, MOVE, 0); direction U[0,5) from word 0
+ * (0 stay, 1 +x, 2 -x, 3 +y, 4 -y, torus wrap: the table of move_agent,
+ * pp_gpu.cl:780-803); reproduction draw U[0,100) from word 1, kept as one bit
+ * (draw < probability of the agent's type) for the action phase. Energy -1;
+ * the agent dies below 1 (pphpc.nlogo:66-70, pp_gpu.cl:806-815): key
+ * DEAD_KEY, dropped by the first sort pass. Returns the key; *val gets the
+ * packed value. */
+__device__ __forceinline__ uint32_t move_agent(uint32_t id, uint32_t step,
+	uint32_t c, int32_t energy, uint32_t wolf, const Params &P, uint32_t *val,
+	bool &overflow) {
+	const uint4 r = pp_philox(id, step, PP_STREAM_MOVE, 0, P.key0, P.key1);
+	const uint32_t dir = pp_bounded(r.x, 5);
+	const uint32_t dice = pp_bounded(r.y, 100);
+	const int32_t e = energy - 1;
+	*val = 0;
+	if (e < 1) return DEAD_KEY;
+	const uint32_t x = c % P.gx;
+	const uint32_t wrap_y = P.C - P.gx;
+	uint32_t nc = c;
+	if (dir == 1) nc = (x == P.gx - 1) ? c - (P.gx - 1) : c + 1;
+	else if (dir == 2) nc = (x == 0) ? c + (P.gx - 1) : c - 1;
+	else if (dir == 3) nc = (c >= wrap_y) ? c - wrap_y : c + P.gx;
+	else if (dir == 4) nc = (c < P.gx) ? c + wrap_y : c - P.gx;
+	const uint32_t prob = wolf ? P.wolves_prob : P.sheep_prob;
+	overflow |= e > (int32_t)V_ENERGY;
+	*val = ((uint32_t)e & V_ENERGY) | (wolf ? V_WOLF : 0u)
+		| (dice < prob ? V_REPRO : 0u);
+	return nc;
+}
+
 /* ------------------------------------------------------------------- init */
 
-/* Zero the iteration counters and the grass ring; bump the look-back epoch so
- * that no word of an earlier run is taken as current. */
+/* Zero the iteration counters, the grass ring and the pass-0 histograms. */
 __global__ __launch_bounds__(WG) void k_init_reset(Ctl *ctl,
-	unsigned long long *ring, uint32_t ring_n, uint32_t *ghist) {
+	unsigned long long *ring, uint32_t ring_n, uint32_t *hist,
+	uint32_t hist_words) {
 	const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
 	for (uint32_t i = gtid; i < ring_n; i += gridDim.x * WG) ring[i] = 0;
-	for (uint32_t i = gtid; i < MAX_PASSES * RADIX; i += gridDim.x * WG)
-		ghist[i] = 0;
+	for (uint32_t i = gtid; i < hist_words; i += gridDim.x * WG) hist[i] = 0;
 	if (gtid == 0) {
 		ctl->n = 0;
 		ctl->step = 0;
 		ctl->error = 0;
 		ctl->err_step = 0xFFFFFFFFu;
-		ctl->live = 0;
+		ctl->live[0] = 0;
+		ctl->live[1] = 0;
 		ctl->out_n = 0;
-		ctl->epoch += 8;
-		for (int i = 0; i < 8; i++) ctl->tile_ctr[i] = 0;
 		ctl->dead_cells = 0;
 		ctl->cd_sum = 0;
 		ctl->eaten = 0;
 		for (int i = 0; i < 4; i++) ctl->acc[i] = 0;
-		ctl->agent_steps = 0;
 		ctl->occupied = 0;
+		ctl->agent_steps = 0;
 		ctl->sum_live = 0;
 		ctl->sum_out = 0;
 		ctl->sum_occ = 0;
@@ -315,124 +331,217 @@ __global__ void k_init_done(Ctl *ctl, pp_stats64_t *stats, Params P) {
 	ctl->n = P.n0;
 }
 
-/* ------------------------------------------------------------------- move */
-
-/* One Philox block per agent, counter (id, step, MOVE, 0): direction
- * U[0,5) from word 0 (0 stay, 1 +x, 2 -x, 3 +y, 4 -y, torus wrap: the table of
- * move_agent, pp_gpu.cl:780-803), reproduction draw U[0,100) from word 1,
- * kept as one bit (draw < probability of the agent's type) for the action
- * phase. Energy -1; the agent dies below 1 (pphpc.nlogo:66-70,
- * pp_gpu.cl:806-815). Survivors leave (key = new cell, packed value); dead
- * agents get DEAD_KEY and are dropped by the first sort pass. The histograms
- * of every radix pass are counted here, in LDS, and merged once per group. */
-__global__ __launch_bounds__(WG) void k_move(const uint32_t *__restrict__ cell,
+/* Move of iteration `step` for a state stored as SoA in id order (the state
+ * right after init): pairs for the sort plus the pass-0 digit histogram of
+ * every sort tile. Each thread moves M_IPT consecutive agents with 16-byte
+ * loads and stores. */
+__global__ __launch_bounds__(WG) void k_move0(const uint32_t *__restrict__ cell,
 	const int32_t *__restrict__ energy, const uint8_t *__restrict__ type,
-	uint32_t *__restrict__ keys, uint32_t *__restrict__ vals, Ctl *ctl,
-	uint32_t *__restrict__ ghist, Params P) {
-	__shared__ uint32_t s_hist[MAX_PASSES * RADIX];
+	uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+	uint32_t *__restrict__ hist, Ctl *ctl, Params P) {
+	__shared__ uint32_t s_hist[RADIX_MAX];
 	__shared__ unsigned long long s_red[NWAVE][1];
 	if (ctl->error) return;
 	const uint32_t n = ctl->n, step = ctl->step;
-	for (uint32_t i = threadIdx.x; i < P.passes * RADIX; i += WG) s_hist[i] = 0;
-	__syncthreads();
+	const int tid = threadIdx.x;
+	const uint32_t radix = 1u << P.digit_bits, dmask = radix - 1u;
 	unsigned long long live[1] = {0};
 	bool overflow = false;
-	const uint32_t wrap_y = P.C - P.gx;
-	for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < n;
-		i += gridDim.x * WG) {
-		const uint32_t c = cell[i];
-		const int32_t e = energy[i] - 1;
-		const uint32_t wolf = type[i];
-		const uint4 r = pp_philox(i, step, PP_STREAM_MOVE, 0, P.key0, P.key1);
-		const uint32_t dir = pp_bounded(r.x, 5);
-		const uint32_t dice = pp_bounded(r.y, 100);
-		const uint32_t x = c % P.gx;
-		uint32_t nc = c;
-		if (dir == 1) nc = (x == P.gx - 1) ? c - (P.gx - 1) : c + 1;
-		else if (dir == 2) nc = (x == 0) ? c + (P.gx - 1) : c - 1;
-		else if (dir == 3) nc = (c >= wrap_y) ? c - wrap_y : c + P.gx;
-		else if (dir == 4) nc = (c < P.gx) ? c + wrap_y : c - P.gx;
-		uint32_t k = DEAD_KEY, v = 0;
-		if (e >= 1) {
-			const uint32_t prob = wolf ? P.wolves_prob : P.sheep_prob;
-			overflow |= e > (int32_t)V_ENERGY;
-			k = nc;
-			v = ((uint32_t)e & V_ENERGY) | (wolf ? V_WOLF : 0u)
-				| (dice < prob ? V_REPRO : 0u);
-			live[0] += 1;
-			for (uint32_t p = 0; p < P.passes; p++)
-				atomicAdd(&s_hist[p * RADIX + ((nc >> (8 * p)) & 0xFFu)], 1u);
+	constexpr uint32_t CHUNK = WG * M_IPT;   /* divides every sort tile */
+	for (uint32_t cb = blockIdx.x * CHUNK; cb < n; cb += gridDim.x * CHUNK) {
+		for (uint32_t i = tid; i < radix; i += WG) s_hist[i] = 0;
+		__syncthreads();
+		const uint32_t i0 = cb + tid * M_IPT;
+		uint32_t c[M_IPT], t[M_IPT];
+		int32_t e[M_IPT];
+		if (i0 + M_IPT <= n) {
+			const uint4 cv = *reinterpret_cast<const uint4 *>(cell + i0);
+			const int4 ev = *reinterpret_cast<const int4 *>(energy + i0);
+			const uint32_t tv = *reinterpret_cast<const uint32_t *>(type + i0);
+			c[0] = cv.x; c[1] = cv.y; c[2] = cv.z; c[3] = cv.w;
+			e[0] = ev.x; e[1] = ev.y; e[2] = ev.z; e[3] = ev.w;
+#pragma unroll
+			for (int j = 0; j < M_IPT; j++) t[j] = (tv >> (8 * j)) & 0xFFu;
+		} else {
+#pragma unroll
+			for (int j = 0; j < M_IPT; j++) {
+				const bool in = i0 + j < n;
+				c[j] = in ? cell[i0 + j] : 0;
+				e[j] = in ? energy[i0 + j] : 0;
+				t[j] = in ? type[i0 + j] : 0;
+			}
 		}
-		keys[i] = k;
-		vals[i] = v;
+		uint32_t k[M_IPT], v[M_IPT];
+#pragma unroll
+		for (int j = 0; j < M_IPT; j++) {
+			k[j] = move_agent(i0 + j, step, c[j], e[j], t[j], P, &v[j], overflow);
+			if (i0 + j >= n) k[j] = DEAD_KEY;
+			live[0] += k[j] != DEAD_KEY;
+		}
+		if (i0 + M_IPT <= n) {
+			*reinterpret_cast<uint4 *>(keys + i0) = make_uint4(k[0], k[1], k[2], k[3]);
+			*reinterpret_cast<uint4 *>(vals + i0) = make_uint4(v[0], v[1], v[2], v[3]);
+		} else {
+#pragma unroll
+			for (int j = 0; j < M_IPT; j++)
+				if (i0 + j < n) {
+					keys[i0 + j] = k[j];
+					vals[i0 + j] = v[j];
+				}
+		}
+#pragma unroll
+		for (int j = 0; j < M_IPT; j++)
+			wave_hist_add(s_hist, k[j] == DEAD_KEY ? NO_DIGIT : (k[j] & dmask));
+		__syncthreads();
+		const uint32_t tile = cb / P.sort_tile;
+		for (uint32_t d = tid; d < radix; d += WG)
+			if (s_hist[d]) atomicAdd(&hist[(size_t)d * P.hist_cols + tile], s_hist[d]);
+		__syncthreads();
 	}
 	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, step);
-	/* live count: one atomic per group */
-	{
-		const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-		const unsigned long long t = wave_sum(live[0]);
-		if (lane == 0) s_red[w][0] = t;
-		__syncthreads();
-		if (threadIdx.x == 0) {
-			unsigned long long s = 0;
-			for (int i = 0; i < NWAVE; i++) s += s_red[i][0];
-			if (s) atomicAdd(&ctl->live, (uint32_t)s);
-		}
-	}
-	for (uint32_t i = threadIdx.x; i < P.passes * RADIX; i += WG) {
-		const uint32_t h = s_hist[i];
-		if (h) atomicAdd(&ghist[i], h);
+	const int lane = tid & (WAVE - 1), w = tid / WAVE;
+	const unsigned long long tsum = wave_sum(live[0]);
+	if (lane == 0) s_red[w][0] = tsum;
+	__syncthreads();
+	if (tid == 0) {
+		unsigned long long sum = 0;
+		for (int i = 0; i < NWAVE; i++) sum += s_red[i][0];
+		if (sum) atomicAdd(&ctl->live[step & 1], (uint32_t)sum);
 	}
 }
 
 /* ------------------------------------------------------------------- sort */
 
-/* One LSD pass of a stable radix sort of (key, value) pairs on 8 key bits at
- * `shift`, onesweep style: work-groups claim tiles in order from an atomic
- * counter; each ranks its tile's keys in LDS with wave64 ballot matching
- * (stable: input order within the tile), publishes per-digit tile counts and
- * obtains the counts of all earlier tiles by decoupled look-back, then writes
- * the tile out digit-contiguously through LDS. The first pass also drops dead
- * agents (DEAD_KEY), so later passes see only survivors. Sorting survivors
- * stably by cell is sorting by the key (cell << 32 | agent id). */
+/* Radix sort of the (cell, value) pairs by cell, least significant digit
+ * first, digit_bits per pass. Every pass is stable, so survivors end ordered
+ * by (cell, agent id) -- the key (cell << 32 | agent id) -- and the first pass
+ * drops dead agents (DEAD_KEY). A pass is three kernels:
+ *   k_sort_up    digit histogram of every tile (pass 0: built by the move)
+ *   k_sort_scan  per digit, exclusive prefix over tiles and the digit total
+ *   k_sort_down  per tile: rank in LDS, scatter to bucket start + tile prefix
+ * Tile t covers positions [t * sort_tile, (t + 1) * sort_tile). The table
+ * hist[d * hist_cols + t] holds tile counts, then tile offsets. */
+
+__device__ __forceinline__ uint32_t pass_len(const Ctl *ctl, uint32_t pass,
+	uint32_t step) {
+	return pass == 0 ? ctl->n : ctl->live[step & 1];
+}
+
 template <int KPT>
-__global__ __launch_bounds__(WG) void k_sort_pass(
-	const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-	uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, Ctl *ctl,
-	const uint32_t *__restrict__ ghist, unsigned long long *lbuf,
-	uint32_t pass) {
+__global__ __launch_bounds__(WG) void k_sort_up(const uint32_t *__restrict__ kin,
+	uint32_t *__restrict__ hist, const Ctl *ctl, uint32_t pass, Params P) {
 	constexpr uint32_t TILE = WG * KPT;
+	__shared__ uint32_t s_hist[RADIX_MAX];
+	if (ctl->error) return;
+	const uint32_t n = pass_len(ctl, pass, ctl->step);
+	const uint32_t ntiles = (n + TILE - 1) / TILE;
+	const uint32_t shift = pass * P.digit_bits;
+	const uint32_t radix = 1u << P.digit_bits, dmask = radix - 1u;
+	const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
+	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+		for (uint32_t i = tid; i < radix; i += WG) s_hist[i] = 0;
+		__syncthreads();
+		const uint32_t base = tile * TILE + w * (WAVE * KPT);
+#pragma unroll
+		for (int j = 0; j < KPT; j++) {
+			const uint32_t idx = base + j * WAVE + lane;
+			const uint32_t k = idx < n ? kin[idx] : DEAD_KEY;
+			wave_hist_add(s_hist, k == DEAD_KEY ? NO_DIGIT : ((k >> shift) & dmask));
+		}
+		__syncthreads();
+		for (uint32_t d = tid; d < radix; d += WG)
+			hist[(size_t)d * P.hist_cols + tile] = s_hist[d];
+		__syncthreads();
+	}
+}
+
+/* One work-group per digit: hist[d][0..ntiles) -> exclusive prefix, and the
+ * digit's total. Each thread sums a contiguous run of tiles. The pass-0 scan
+ * also clears the action tiles' counters for this iteration. */
+__global__ __launch_bounds__(WG) void k_sort_scan(uint32_t *__restrict__ hist,
+	uint32_t *__restrict__ totals, const Ctl *ctl, uint32_t pass, Params P,
+	uint32_t *__restrict__ act_cnt, uint32_t act_words) {
+	__shared__ uint32_t s_tmp[NWAVE];
+	if (ctl->error) return;
+	/* The action tiles' counters of this iteration start at zero. */
+	if (pass == 0)
+		for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < act_words;
+			i += gridDim.x * WG)
+			act_cnt[i] = 0;
+	const uint32_t n = pass_len(ctl, pass, ctl->step);
+	const uint32_t ntiles = (n + P.sort_tile - 1) / P.sort_tile;
+	const uint32_t radix = 1u << P.digit_bits;
+	const uint32_t per = (ntiles + WG - 1) / WG;
+	const uint32_t t0 = threadIdx.x * per;
+	const uint32_t t1 = min(t0 + per, ntiles);
+	for (uint32_t d = blockIdx.x; d < radix; d += gridDim.x) {
+		uint32_t *col = hist + (size_t)d * P.hist_cols;
+		uint32_t sum = 0;
+		for (uint32_t t = t0; t < t1; t++) sum += col[t];
+		uint32_t total;
+		uint32_t run = wg_excl_scan<WG>(sum, s_tmp, total);
+		for (uint32_t t = t0; t < t1; t++) {
+			const uint32_t v = col[t];
+			col[t] = run;
+			run += v;
+		}
+		if (threadIdx.x == 0) totals[d] = total;
+	}
+}
+
+/* Per tile: load KPT keys per thread (wave w owns [tile*TILE + w*64*KPT,
+ * +64*KPT), element j of lane l at offset j*64 + l), rank them inside each
+ * wave with ballots over the digit bits (order (j, lane) = input order, so the
+ * ranking is stable), combine the waves, stage the tile in digit order in
+ * LDS and write it to bucket start + tile prefix. */
+template <int KPT, int RBITS>
+__global__ __launch_bounds__(WG) void k_sort_down(
+	const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+	uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+	const uint32_t *__restrict__ hist, const uint32_t *__restrict__ totals,
+	const Ctl *ctl, uint32_t pass, Params P) {
+	constexpr uint32_t TILE = WG * KPT;
+	constexpr uint32_t R = 1u << RBITS;
+	constexpr uint32_t DPT = R / WG < 1 ? 1 : R / WG;  /* digits per thread */
 	__shared__ uint32_t s_keys[TILE];
 	__shared__ uint32_t s_vals[TILE];
-	__shared__ uint32_t s_wh[NWAVE][RADIX + 1];
-	__shared__ uint32_t s_bstart[RADIX];
-	__shared__ uint32_t s_tstart[RADIX];
-	__shared__ int32_t s_dst[RADIX];
+	__shared__ uint32_t s_wh[NWAVE][R + 1];
+	__shared__ uint32_t s_bstart[R];
+	__shared__ int32_t s_dst[R];
+	__shared__ uint32_t s_tstart[R];
 	__shared__ uint32_t s_tmp[NWAVE];
-	__shared__ uint32_t s_tile;
 
 	if (ctl->error) return;
-	const uint32_t n = pass == 0 ? ctl->n : ctl->live;
+	const uint32_t n = pass_len(ctl, pass, ctl->step);
 	const uint32_t ntiles = (n + TILE - 1) / TILE;
-	const uint32_t epoch = ctl->epoch + pass + 1;
-	const uint32_t shift = 8 * pass;
+	const uint32_t shift = pass * P.digit_bits;
+	const uint32_t radix = 1u << P.digit_bits, dmask = radix - 1u;
 	const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
 	const uint64_t lt_mask = (1ull << lane) - 1ull;
 
+	/* Bucket starts: exclusive scan over digits of the digit totals. Thread
+	 * tid owns digits [tid*DPT, tid*DPT + DPT). */
 	{
+		uint32_t loc[DPT], sum = 0;
+#pragma unroll
+		for (uint32_t i = 0; i < DPT; i++) {
+			const uint32_t d = tid * DPT + i;
+			loc[i] = d < radix ? totals[d] : 0u;
+			sum += loc[i];
+		}
 		uint32_t tot;
-		s_bstart[tid] = wg_excl_scan(ghist[pass * RADIX + tid], s_tmp, tot);
+		uint32_t run = wg_excl_scan<WG>(sum, s_tmp, tot);
+#pragma unroll
+		for (uint32_t i = 0; i < DPT; i++) {
+			const uint32_t d = tid * DPT + i;
+			if (d < R) s_bstart[d] = run;
+			run += loc[i];
+		}
 	}
 
-	for (;;) {
-		if (tid == 0) s_tile = atomicAdd(&ctl->tile_ctr[pass], 1u);
-		for (int i = tid; i < NWAVE * (RADIX + 1); i += WG) (&s_wh[0][0])[i] = 0;
+	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+		for (uint32_t i = tid; i < NWAVE * (R + 1); i += WG) (&s_wh[0][0])[i] = 0;
 		__syncthreads();
-		const uint32_t tile = s_tile;
-		if (tile >= ntiles) break;
-
-		/* Load: wave w owns [tile*TILE + w*64*KPT, +64*KPT), element j of lane
-		 * l at offset j*64 + l (coalesced). */
 		const uint32_t base = tile * TILE + w * (WAVE * KPT);
 		uint32_t key[KPT], val[KPT], dig[KPT], loc[KPT];
 #pragma unroll
@@ -445,16 +554,14 @@ __global__ __launch_bounds__(WG) void k_sort_pass(
 			}
 			key[j] = k;
 			val[j] = v;
-			dig[j] = (k == DEAD_KEY) ? (uint32_t)RADIX : ((k >> shift) & 0xFFu);
+			dig[j] = (k == DEAD_KEY) ? R : ((k >> shift) & dmask);
 		}
-		/* Rank within the wave: lanes with an equal 9-bit digit (digit 256 =
-		 * dead or padding) found by 9 ballots; order (j, lane) = input order. */
 #pragma unroll
 		for (int j = 0; j < KPT; j++) {
 			const uint32_t d = dig[j];
 			uint64_t peers = ~0ull;
 #pragma unroll
-			for (int b = 0; b < 9; b++) {
+			for (int b = 0; b <= RBITS; b++) {
 				const bool bit = (d >> b) & 1u;
 				const uint64_t bal = __ballot(bit);
 				peers &= bit ? bal : ~bal;
@@ -467,35 +574,37 @@ __global__ __launch_bounds__(WG) void k_sort_pass(
 		}
 		__syncthreads();
 
-		/* Digit tid: per-wave exclusive offsets and the tile count. */
-		const uint32_t c0 = s_wh[0][tid], c1 = s_wh[1][tid];
-		const uint32_t c2 = s_wh[2][tid], c3 = s_wh[3][tid];
-		s_wh[0][tid] = 0;
-		s_wh[1][tid] = c0;
-		s_wh[2][tid] = c0 + c1;
-		s_wh[3][tid] = c0 + c1 + c2;
-		const uint32_t cnt_d = c0 + c1 + c2 + c3;
-		uint32_t tile_live;
-		const uint32_t tstart = wg_excl_scan(cnt_d, s_tmp, tile_live);
-		s_tstart[tid] = tstart;
-
-		unsigned long long *my = lbuf + (size_t)tile * RADIX + tid;
-		uint32_t excl = 0;
-		if (tile == 0) {
-			lb_store(my, lb_pack(epoch, LB_INC, cnt_d));
-		} else {
-			lb_store(my, lb_pack(epoch, LB_AGG, cnt_d));
-			excl = lookback(lbuf + tid, RADIX, tile, epoch);
-			lb_store(my, lb_pack(epoch, LB_INC, excl + cnt_d));
+		/* Per digit: exclusive offsets of the waves, tile count; then the
+		 * tile-local start of each digit (exclusive scan over digits). */
+		uint32_t cnt_d[DPT], sum = 0;
+#pragma unroll
+		for (uint32_t i = 0; i < DPT; i++) {
+			const uint32_t d = tid * DPT + i;
+			uint32_t acc = 0;
+#pragma unroll
+			for (int ww = 0; ww < NWAVE; ww++) {
+				const uint32_t c = s_wh[ww][d];
+				s_wh[ww][d] = acc;
+				acc += c;
+			}
+			cnt_d[i] = acc;
+			sum += acc;
 		}
-		s_dst[tid] = (int32_t)(s_bstart[tid] + excl) - (int32_t)tstart;
+		uint32_t tile_live;
+		uint32_t run = wg_excl_scan<WG>(sum, s_tmp, tile_live);
+#pragma unroll
+		for (uint32_t i = 0; i < DPT; i++) {
+			const uint32_t d = tid * DPT + i;
+			s_tstart[d] = run;
+			const uint32_t off = d < radix ? hist[(size_t)d * P.hist_cols + tile] : 0u;
+			s_dst[d] = (int32_t)(s_bstart[d] + off) - (int32_t)run;
+			run += cnt_d[i];
+		}
 		__syncthreads();
-
-		/* Stage the tile in digit order in LDS. */
 #pragma unroll
 		for (int j = 0; j < KPT; j++) {
 			const uint32_t d = dig[j];
-			if (d < RADIX) {
+			if (d < R) {
 				const uint32_t q = s_tstart[d] + s_wh[w][d] + loc[j];
 				s_keys[q] = key[j];
 				s_vals[q] = val[j];
@@ -504,7 +613,7 @@ __global__ __launch_bounds__(WG) void k_sort_pass(
 		__syncthreads();
 		for (uint32_t q = tid; q < tile_live; q += WG) {
 			const uint32_t k = s_keys[q];
-			const uint32_t d = (k >> shift) & 0xFFu;
+			const uint32_t d = (k >> shift) & dmask;
 			const uint32_t dst = (uint32_t)(s_dst[d] + (int32_t)q);
 			kout[dst] = k;
 			vout[dst] = s_vals[q];
@@ -515,10 +624,55 @@ __global__ __launch_bounds__(WG) void k_sort_pass(
 
 /* ----------------------------------------------------------------- action */
 
+/* Values of one cell's segment: positions of the current tile live in LDS,
+ * the few positions of a segment running into the next tile in global
+ * memory (only the lane that owns the segment touches them). */
+struct SegVals {
+	uint32_t *s_val;
+	uint32_t *g_val;
+	__device__ __forceinline__ uint32_t get(uint32_t q) const {
+		return q < (uint32_t)A_TILE ? s_val[q] : g_val[q];
+	}
+	__device__ __forceinline__ void set(uint32_t q, uint32_t v) const {
+		if (q < (uint32_t)A_TILE) s_val[q] = v;
+		else g_val[q] = v;
+	}
+};
+
+/* End (exclusive, tile-local) of the segment of cell c that starts at lp. */
+__device__ __forceinline__ uint32_t seg_end(const uint32_t *s_key,
+	const uint32_t *__restrict__ keys, uint32_t lp, uint32_t cnt,
+	uint32_t base, uint32_t L, uint32_t c) {
+	uint32_t e = lp + 1;
+	while (e < cnt && s_key[1 + e] == c) e++;
+	if (e == cnt) {
+		uint32_t g = base + e;
+		while (g < L && keys[g] == c) g++;
+		e = g - base;
+	}
+	return e;
+}
+
+/* Loads positions [base, base + A_TILE) of the sorted keys (s_key[1..], with
+ * s_key[0] the key just before the tile) and optionally the values. */
+__device__ __forceinline__ uint32_t load_tile(const uint32_t *__restrict__ keys,
+	const uint32_t *vals, uint32_t *s_key, uint32_t *s_val, uint32_t base,
+	uint32_t L) {
+	const uint32_t cnt = min((uint32_t)A_TILE, L - base);
+	for (uint32_t i = threadIdx.x; i < (uint32_t)A_TILE; i += WG) {
+		const bool in = i < cnt;
+		s_key[1 + i] = in ? keys[base + i] : DEAD_KEY;
+		if (s_val) s_val[i] = in ? vals[base + i] : 0u;
+	}
+	if (threadIdx.x == 0) s_key[0] = base ? keys[base - 1] : DEAD_KEY;
+	__syncthreads();
+	return cnt;
+}
+
 /* Agent actions for the agents sorted by cell (ascending agent id within a
- * cell). Thread p handles sorted position p; when p starts a run of equal
- * cells (the segment that find_cell_idx, pp_gpu.cl:834-867, delimits), the
- * thread acts for the whole cell in sequence:
+ * cell). A work-group stages A_TILE sorted positions in LDS; the thread owning
+ * a position that starts a run of equal cells (the segment that find_cell_idx,
+ * pp_gpu.cl:834-867, delimits) acts for the whole cell in sequence:
  *   1. Fisher-Yates permutation of the segment, the draw for position i being
  *      word m%4 of Philox block (cell, step, SHUFFLE, m/4), m = k-1-i (random
  *      acting order: Cell.java:246, pp_cpu.cl:236-259);
@@ -530,46 +684,54 @@ __global__ __launch_bounds__(WG) void k_sort_pass(
  *      reproduction draw passed, the energy is split, the child getting
  *      floor(e/2) (AbstractAgent.java:139-167). Newborns are not in the
  *      segment, so they neither act nor get eaten (pp_cpu.cl:841-845).
- * The next state is written compacted: cells ascending, within a cell the
- * surviving agents in acting order each followed by its child. Output offsets
- * come from a work-group scan of the per-cell output counts plus decoupled
- * look-back across tiles. Counts and energy sums of the new state are reduced
- * here; grass eaten is counted for the ring update in k_finalize. */
-__global__ __launch_bounds__(WG) void k_action(
-	const uint32_t *__restrict__ keys, uint32_t *vals,
-	uint32_t *__restrict__ ocell, int32_t *__restrict__ oenergy,
-	uint8_t *__restrict__ otype, uint32_t *grass_at, Ctl *ctl,
-	unsigned long long *lbuf, Params P) {
+ * The permuted, updated values are written back in place, so the next state
+ * is the sorted positions in order, each emitting its survivor then its
+ * newborn; act_cnt[t] (zeroed by the pass-0 scan) receives the number of
+ * agents the positions of tile t emit. Counts and energy sums of the new state
+ * are reduced here. This kernel also clears the pass-0 histogram table that
+ * k_emit_move fills for the next iteration. */
+__global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
+	uint32_t *vals, uint32_t *grass_at, uint32_t *act_cnt,
+	uint32_t *__restrict__ hist, uint32_t hist_words, Ctl *ctl, Params P) {
+	__shared__ uint32_t s_key[A_TILE + 1];
+	__shared__ uint32_t s_val[A_TILE];
 	__shared__ uint32_t s_tmp[NWAVE];
-	__shared__ uint32_t s_tile;
-	__shared__ uint32_t s_off;
 	__shared__ unsigned long long s_red[NWAVE][6];
 
 	if (ctl->error) return;
-	const uint32_t L = ctl->live, step = ctl->step;
-	const uint32_t ntiles = (L + WG - 1) / WG;
-	const uint32_t epoch = ctl->epoch + ACTION_SLOT + 1;
-	const uint32_t alive_at = step + 1;
 	const int tid = threadIdx.x;
+	for (uint32_t i = blockIdx.x * WG + tid; i < hist_words; i += gridDim.x * WG)
+		hist[i] = 0;
+	const uint32_t step = ctl->step;
+	const uint32_t L = ctl->live[step & 1];
+	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
+	const uint32_t alive_at = step + 1;
 	unsigned long long acc[6] = {0, 0, 0, 0, 0, 0};
 	bool overflow = false;
 
-	for (;;) {
-		if (tid == 0) s_tile = atomicAdd(&ctl->tile_ctr[ACTION_SLOT], 1u);
-		__syncthreads();
-		const uint32_t tile = s_tile;
-		if (tile >= ntiles) break;
-		const uint32_t p = tile * WG + tid;
-		uint32_t c = 0, end = 0, nout = 0;
-		bool head = false;
-		if (p < L) {
-			c = keys[p];
-			head = (p == 0) || (keys[p - 1] != c);
+	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+		const uint32_t base = tile * A_TILE;
+		const uint32_t cnt = load_tile(keys, vals, s_key, s_val, base, L);
+		const SegVals sv{s_val, vals + base};
+		/* Heads of this thread's positions; their grass loads are issued
+		 * together before any segment is processed. */
+		uint32_t cj[A_IPT], gj[A_IPT];
+		bool hj[A_IPT];
+#pragma unroll
+		for (int j = 0; j < A_IPT; j++) {
+			const uint32_t lp = tid * A_IPT + j;
+			cj[j] = s_key[1 + lp];
+			hj[j] = lp < cnt && s_key[lp] != cj[j];
+			gj[j] = hj[j] ? grass_at[cj[j]] : 0u;
 		}
-		if (head) {
-			end = p + 1;
-			while (end < L && keys[end] == c) end++;
-			const uint32_t k = end - p;
+		uint32_t nout = 0;
+#pragma unroll 1
+		for (int j = 0; j < A_IPT; j++) {
+			if (!hj[j]) continue;
+			const uint32_t lp = tid * A_IPT + j;
+			const uint32_t c = cj[j];
+			const uint32_t end = seg_end(s_key, keys, lp, cnt, base, L, c);
+			const uint32_t k = end - lp;
 			acc[5] += 1;
 			if (k > 1) {
 				uint4 r = make_uint4(0, 0, 0, 0);
@@ -579,18 +741,18 @@ __global__ __launch_bounds__(WG) void k_action(
 							P.key1);
 					const uint32_t word = (m & 3u) == 0 ? r.x
 						: (m & 3u) == 1 ? r.y : (m & 3u) == 2 ? r.z : r.w;
-					const uint32_t j = pp_bounded(word, i + 1);
-					if (j != i) {
-						const uint32_t a = vals[p + i];
-						vals[p + i] = vals[p + j];
-						vals[p + j] = a;
+					const uint32_t jj = pp_bounded(word, i + 1);
+					if (jj != i) {
+						const uint32_t a = sv.get(lp + i);
+						sv.set(lp + i, sv.get(lp + jj));
+						sv.set(lp + jj, a);
 					}
 				}
 			}
-			bool grass = grass_at[c] <= alive_at;
+			bool grass = gj[j] <= alive_at;
 			bool ate = false;
-			for (uint32_t q = p; q < end; q++) {
-				const uint32_t v = vals[q];
+			for (uint32_t q = lp; q < end; q++) {
+				const uint32_t v = sv.get(q);
 				if (v & V_KILLED) continue;
 				int32_t e = (int32_t)(v & V_ENERGY);
 				uint32_t thr;
@@ -602,10 +764,10 @@ __global__ __launch_bounds__(WG) void k_action(
 					}
 					thr = P.sheep_thr;
 				} else {
-					for (uint32_t o = p; o < end; o++) {
-						const uint32_t u = vals[o];
+					for (uint32_t o = lp; o < end; o++) {
+						const uint32_t u = sv.get(o);
 						if (!(u & (V_WOLF | V_KILLED))) {
-							vals[o] = u | V_KILLED;
+							sv.set(o, u | V_KILLED);
 							e += (int32_t)P.wolves_gain;
 							break;
 						}
@@ -618,64 +780,37 @@ __global__ __launch_bounds__(WG) void k_action(
 					e -= e / 2;
 				}
 				overflow |= e > (int32_t)V_ENERGY;
-				vals[q] = nv | ((uint32_t)e & V_ENERGY);
+				sv.set(q, nv | ((uint32_t)e & V_ENERGY));
 			}
 			if (ate) {
 				grass_at[c] = step + 1 + P.R;
 				acc[4] += 1;
 			}
-			for (uint32_t q = p; q < end; q++) {
-				const uint32_t v = vals[q];
-				nout += ((v & V_KILLED) == 0) + ((v & V_CHILD) != 0);
-			}
-		}
-
-		uint32_t tile_total;
-		const uint32_t local = wg_excl_scan(nout, s_tmp, tile_total);
-		if (tid == 0) {
-			unsigned long long *my = lbuf + tile;
-			uint32_t excl = 0;
-			if (tile == 0) {
-				lb_store(my, lb_pack(epoch, LB_INC, tile_total));
-			} else {
-				lb_store(my, lb_pack(epoch, LB_AGG, tile_total));
-				excl = lookback(lbuf, 1, tile, epoch);
-				lb_store(my, lb_pack(epoch, LB_INC, excl + tile_total));
-			}
-			s_off = excl;
-			if (tile == ntiles - 1) ctl->out_n = excl + tile_total;
-		}
-		__syncthreads();
-		if (head) {
-			uint32_t o = s_off + local;
-			for (uint32_t q = p; q < end; q++) {
-				const uint32_t v = vals[q];
-				const int32_t e = (int32_t)(v & V_ENERGY);
-				const uint8_t t = (v & V_WOLF) ? PP_WOLF : PP_SHEEP;
+			for (uint32_t q = lp; q < end; q++) {
+				const uint32_t v = sv.get(q);
+				const uint32_t e = v & V_ENERGY;
+				const uint32_t t = (v & V_WOLF) ? 1u : 0u;
+				const uint32_t outs = ((v & V_KILLED) == 0) + ((v & V_CHILD) != 0);
 				if (!(v & V_KILLED)) {
-					if (o < P.cap) {
-						ocell[o] = c;
-						oenergy[o] = e;
-						otype[o] = t;
-					}
-					o++;
 					acc[t] += 1;
-					acc[2 + t] += (unsigned long long)e;
+					acc[2 + t] += e;
 				}
 				if (v & V_CHILD) {
-					const int32_t ce = (v & V_ODD) ? e - 1 : e;
-					if (o < P.cap) {
-						ocell[o] = c;
-						oenergy[o] = ce;
-						otype[o] = t;
-					}
-					o++;
 					acc[t] += 1;
-					acc[2 + t] += (unsigned long long)ce;
+					acc[2 + t] += (v & V_ODD) ? e - 1 : e;
 				}
+				if (q < (uint32_t)A_TILE) nout += outs;
+				else if (outs) atomicAdd(&act_cnt[tile + q / A_TILE], outs);
 			}
 		}
 		__syncthreads();
+		/* Write back the positions this tile owns (not those of the previous
+		 * tile's last segment, which its lane wrote). */
+		for (uint32_t i = tid; i < cnt; i += WG)
+			if (s_key[1 + i] != s_key[0]) vals[base + i] = s_val[i];
+		uint32_t total;
+		(void)wg_excl_scan<WG>(nout, s_tmp, total);
+		if (tid == 0 && total) atomicAdd(&act_cnt[tile], total);
 	}
 	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, step);
 	unsigned long long *dst[6] = {&ctl->acc[0], &ctl->acc[1], &ctl->acc[2],
@@ -683,56 +818,199 @@ __global__ __launch_bounds__(WG) void k_action(
 	wg_add<6>(acc, s_red, dst);
 }
 
-/* --------------------------------------------------------------- finalize */
+/* One work-group: exclusive scan of the action tiles' output counts (the
+ * output offsets), then the end of iteration s -> s+1. Grass statistics come
+ * from the regrow-step encoding without touching the grid: with D dead cells
+ * and countdown sum S at state s, every dead countdown drops by one (S -= D),
+ * the cells whose regrow step is s+1 come alive (ring slot (s+1) mod (R+1)),
+ * and the E cells eaten this iteration restart at R (slot s mod (R+1)):
+ * exactly the grass kernel (pp_gpu.cl:448-476) plus the sums of
+ * reduce_grass1/2 (pp_gpu.cl:485-583). Then the statistics row s+1 is
+ * written, the capacity checked (pp_gpu.c:847-852) and the per-iteration
+ * counters cleared. */
+constexpr int SCAN_WG = 1024;
+constexpr int SCAN_UNROLL = 8;
 
-/* End of an iteration s -> s+1 (one work-group). Grass statistics come from
- * the regrow-step encoding without touching the grid: with D dead cells and
- * countdown sum S at state s, every dead countdown drops by one (S -= D), the
- * cells whose regrow step is s+1 come alive (ring slot (s+1) mod (R+1)), and
- * the E cells eaten this iteration restart at R (slot s mod (R+1)): exactly
- * the grass kernel (pp_gpu.cl:448-476) plus the sums of reduce_grass1/2
- * (pp_gpu.cl:485-583). Then the statistics row s+1 is written, the capacity
- * checked (pp_gpu.c:847-852) and the per-iteration counters cleared. */
-__global__ __launch_bounds__(WG) void k_finalize(Ctl *ctl,
-	unsigned long long *ring, uint32_t *ghist, pp_stats64_t *stats,
-	Params P) {
-	if (threadIdx.x == 0) {
-		if (ctl->error == 0) {
-			const uint32_t s = ctl->step;
-			if (ctl->out_n > P.cap) {
-				set_error(ctl, PP_OUT_OF_RESOURCES, s);
-			} else {
-				const uint32_t M = P.R + 1;
-				unsigned long long D = ctl->dead_cells, S = ctl->cd_sum;
-				S -= D;
-				const uint32_t expire = (s + 1) % M;
-				D -= ring[expire];
-				ring[expire] = 0;
-				const unsigned long long E = ctl->eaten;
-				ring[s % M] += E;
-				D += E;
-				S += E * (unsigned long long)P.R;
-				ctl->dead_cells = D;
-				ctl->cd_sum = S;
-				write_row(ctl, &stats[(s + 1) % STATS_RING], P);
-				ctl->agent_steps += ctl->n;
-				ctl->sum_live += ctl->live;
-				ctl->sum_out += ctl->out_n;
-				ctl->sum_occ += ctl->occupied;
-				ctl->sum_eaten += ctl->eaten;
-				ctl->n = ctl->out_n;
-				ctl->step = s + 1;
-			}
-			for (int i = 0; i < 4; i++) ctl->acc[i] = 0;
-			ctl->eaten = 0;
-			ctl->occupied = 0;
-			ctl->live = 0;
-			ctl->out_n = 0;
-			ctl->epoch += 8;
-			for (int i = 0; i < 8; i++) ctl->tile_ctr[i] = 0;
+__global__ __launch_bounds__(SCAN_WG) void k_act_scan(
+	uint32_t *__restrict__ act_cnt, Ctl *ctl, unsigned long long *ring,
+	pp_stats64_t *stats, Params P) {
+	__shared__ uint32_t s_tmp[SCAN_WG / WAVE];
+	if (ctl->error) return;
+	const uint32_t step = ctl->step;
+	const uint32_t L = ctl->live[step & 1];
+	const uint32_t nt = (L + A_TILE - 1) / A_TILE;
+	const uint32_t per = (nt + SCAN_WG - 1) / SCAN_WG;
+	const uint32_t t0 = threadIdx.x * per, t1 = min(t0 + per, nt);
+	uint32_t sum = 0;
+	for (uint32_t t = t0; t < t1; t += SCAN_UNROLL) {
+		uint32_t v[SCAN_UNROLL];
+#pragma unroll
+		for (int u = 0; u < SCAN_UNROLL; u++)
+			v[u] = t + u < t1 ? act_cnt[t + u] : 0u;
+#pragma unroll
+		for (int u = 0; u < SCAN_UNROLL; u++) sum += v[u];
+	}
+	uint32_t total;
+	uint32_t run = wg_excl_scan<SCAN_WG>(sum, s_tmp, total);
+	for (uint32_t t = t0; t < t1; t += SCAN_UNROLL) {
+		uint32_t v[SCAN_UNROLL];
+#pragma unroll
+		for (int u = 0; u < SCAN_UNROLL; u++)
+			v[u] = t + u < t1 ? act_cnt[t + u] : 0u;
+#pragma unroll
+		for (int u = 0; u < SCAN_UNROLL; u++) {
+			if (t + u < t1) act_cnt[t + u] = run;
+			run += v[u];
 		}
 	}
-	for (uint32_t i = threadIdx.x; i < MAX_PASSES * RADIX; i += WG) ghist[i] = 0;
+	if (threadIdx.x != 0) return;
+	const uint32_t s = step;
+	if (total > P.cap) {
+		set_error(ctl, PP_OUT_OF_RESOURCES, s);
+		return;
+	}
+	const uint32_t M = P.R + 1;
+	unsigned long long D = ctl->dead_cells, S = ctl->cd_sum;
+	S -= D;
+	const uint32_t expire = (s + 1) % M;
+	D -= ring[expire];
+	ring[expire] = 0;
+	const unsigned long long E = ctl->eaten;
+	ring[s % M] += E;
+	D += E;
+	S += E * (unsigned long long)P.R;
+	ctl->dead_cells = D;
+	ctl->cd_sum = S;
+	write_row(ctl, &stats[(s + 1) % STATS_RING], P);
+	ctl->agent_steps += ctl->n;
+	ctl->sum_live += L;
+	ctl->sum_out += total;
+	ctl->sum_occ += ctl->occupied;
+	ctl->sum_eaten += E;
+	ctl->n = total;
+	ctl->out_n = total;
+	for (int i = 0; i < 4; i++) ctl->acc[i] = 0;
+	ctl->eaten = 0;
+	ctl->occupied = 0;
+	ctl->live[(s + 1) & 1] = 0;
+	ctl->step = s + 1;
+}
+
+/* Emits the new state and moves it for the next iteration. Sorted position
+ * p emits its agent unless eaten, then its newborn if any; the agents' ids in
+ * the new state are these outputs in position order (tile offset from
+ * k_act_scan plus a work-group scan). Each output is staged in LDS, then
+ * written with coalesced stores to the SoA state and immediately moved for
+ * iteration step (= s+1): its (key, value) pair goes to slot id and its
+ * pass-0 digit is counted for the sort tile holding id. The outputs of one
+ * action tile (at most 2 x A_TILE) span at most two sort tiles. */
+template <int RBITS>
+__global__ __launch_bounds__(WG) void k_emit_move(
+	const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+	const uint32_t *__restrict__ act_off, uint32_t *__restrict__ ocell,
+	int32_t *__restrict__ oenergy, uint8_t *__restrict__ otype,
+	uint32_t *__restrict__ mkeys, uint32_t *__restrict__ mvals,
+	uint32_t *__restrict__ hist, Ctl *ctl, Params P) {
+	constexpr uint32_t R = 1u << RBITS;
+	__shared__ uint32_t s_oc[A_STAGE];
+	__shared__ uint32_t s_ov[A_STAGE];
+	__shared__ uint32_t s_h[2 * R];
+	__shared__ uint32_t s_tmp[NWAVE];
+	__shared__ unsigned long long s_red[NWAVE][1];
+
+	if (ctl->error) return;
+	const int tid = threadIdx.x;
+	const uint32_t step = ctl->step;            /* iteration being moved */
+	const uint32_t L = ctl->live[(step - 1) & 1];
+	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
+	const uint32_t dmask = (1u << P.digit_bits) - 1u;
+	unsigned long long live[1] = {0};
+	bool overflow = false;
+
+	for (uint32_t i = tid; i < 2 * R; i += WG) s_h[i] = 0;
+	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+		const uint32_t base = tile * A_TILE;
+		const uint32_t goff = act_off[tile];
+		const uint32_t p0 = base + tid * A_IPT;
+		uint32_t c[A_IPT], v[A_IPT];
+		if (p0 + A_IPT <= L) {
+			const uint4 kv = *reinterpret_cast<const uint4 *>(keys + p0);
+			const uint4 vv = *reinterpret_cast<const uint4 *>(vals + p0);
+			c[0] = kv.x; c[1] = kv.y; c[2] = kv.z; c[3] = kv.w;
+			v[0] = vv.x; v[1] = vv.y; v[2] = vv.z; v[3] = vv.w;
+		} else {
+#pragma unroll
+			for (int j = 0; j < A_IPT; j++) {
+				const bool in = p0 + j < L;
+				c[j] = in ? keys[p0 + j] : 0u;
+				v[j] = in ? vals[p0 + j] : V_KILLED;
+			}
+		}
+		uint32_t nout = 0;
+#pragma unroll
+		for (int j = 0; j < A_IPT; j++)
+			nout += ((v[j] & V_KILLED) == 0) + ((v[j] & V_CHILD) != 0);
+		uint32_t total;
+		uint32_t o = wg_excl_scan<WG>(nout, s_tmp, total);
+#pragma unroll
+		for (int j = 0; j < A_IPT; j++) {
+			const uint32_t e = v[j] & V_ENERGY;
+			const uint32_t t = (v[j] & V_WOLF) ? 1u : 0u;
+			if (!(v[j] & V_KILLED)) {
+				s_oc[o] = c[j];
+				s_ov[o] = e | (t << 31);
+				o++;
+			}
+			if (v[j] & V_CHILD) {
+				s_oc[o] = c[j];
+				s_ov[o] = ((v[j] & V_ODD) ? e - 1 : e) | (t << 31);
+				o++;
+			}
+		}
+		__syncthreads();
+		const uint32_t to0 = goff / P.sort_tile;
+		for (uint32_t i0 = 0; i0 < total; i0 += WG) {
+			const uint32_t i = i0 + tid;
+			uint32_t d = NO_DIGIT;
+			if (i < total) {
+				const uint32_t id = goff + i;
+				const uint32_t cc = s_oc[i], ov = s_ov[i];
+				const uint32_t en = ov & V_ENERGY, t = ov >> 31;
+				ocell[id] = cc;
+				oenergy[id] = (int32_t)en;
+				otype[id] = (uint8_t)t;
+				uint32_t mv;
+				const uint32_t mk = move_agent(id, step, cc, (int32_t)en, t, P,
+					&mv, overflow);
+				mkeys[id] = mk;
+				mvals[id] = mv;
+				if (mk != DEAD_KEY) {
+					live[0] += 1;
+					d = (id / P.sort_tile - to0) * R + (mk & dmask);
+				}
+			}
+			wave_hist_add(s_h, d);
+		}
+		__syncthreads();
+		for (uint32_t i = tid; i < 2 * R; i += WG) {
+			const uint32_t h = s_h[i];
+			if (h) {
+				atomicAdd(&hist[(size_t)(i % R) * P.hist_cols + to0 + i / R], h);
+				s_h[i] = 0;
+			}
+		}
+		__syncthreads();
+	}
+	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, step);
+	const int lane = tid & (WAVE - 1), w = tid / WAVE;
+	const unsigned long long tsum = wave_sum(live[0]);
+	if (lane == 0) s_red[w][0] = tsum;
+	__syncthreads();
+	if (tid == 0) {
+		unsigned long long sum = 0;
+		for (int i = 0; i < NWAVE; i++) sum += s_red[i][0];
+		if (sum) atomicAdd(&ctl->live[step & 1], (uint32_t)sum);
+	}
 }
 
 /* ------------------------------------------------------------ host layer */
@@ -742,16 +1020,19 @@ enum KernelId {
 	KI_INIT_CELLS,
 	KI_INIT_AGENTS,
 	KI_INIT_DONE,
-	KI_MOVE,
-	KI_SORT_PASS,
-	KI_ACTION,
-	KI_FINALIZE,
+	KI_MOVE0,
+	KI_SORT_UP,
+	KI_SORT_SCAN,
+	KI_SORT_DOWN,
+	KI_ACT,
+	KI_ACT_SCAN,
+	KI_EMIT_MOVE,
 	KI_COUNT
 };
 
 const char *const kernel_names[KI_COUNT] = {
-	"init_reset", "init_cells", "init_agents", "init_done",
-	"move", "sort_pass", "action", "finalize",
+	"init_reset", "init_cells", "init_agents", "init_done", "move0",
+	"sort_up", "sort_scan", "sort_down", "act", "act_scan", "emit_move",
 };
 
 struct PendingEvent {
@@ -773,22 +1054,25 @@ struct pp_sim {
 	uint32_t *cell;
 	int32_t *energy;
 	uint8_t *type;
-	uint32_t *keys[2];
-	uint32_t *vals[2];
+	uint32_t *keys[3];
+	uint32_t *vals[3];
 	uint32_t *grass_at;
 	unsigned long long *ring;
 	uint32_t ring_n;
-	uint32_t *ghist;
-	unsigned long long *lb_sort;
-	unsigned long long *lb_act;
+	uint32_t *hist;
+	uint32_t hist_words;
+	uint32_t *totals;
+	uint32_t *act_cnt;
+	uint32_t act_words;
 	Ctl *ctl;
 	pp_stats64_t *stats_dev;
 	pp_stats64_t *stats_host; /* pinned, STATS_RING rows */
 	Ctl *ctl_host;            /* pinned mirror */
 
 	int kpt;
-	uint32_t sort_tile;
-	uint32_t grid_move, grid_sort, grid_act, grid_cells, grid_agents;
+	int rbits;                /* template radix bits of the sort kernels */
+	uint32_t grid_move, grid_up, grid_down, grid_act, grid_emit, grid_cells;
+	uint32_t grid_agents;
 
 	hipGraphExec_t graph_exec;
 	int graph_steps;
@@ -853,39 +1137,66 @@ static int prof_collect(pp_sim_t *s) {
 	return PP_SUCCESS;
 }
 
-template <int KPT>
-static void launch_sort_pass(pp_sim_t *s, uint32_t pass) {
-	const int src = pass & 1, dst = src ^ 1;
-	k_sort_pass<KPT><<<s->grid_sort, WG, 0, s->stream>>>(s->keys[src],
-		s->vals[src], s->keys[dst], s->vals[dst], s->ctl, s->ghist, s->lb_sort,
-		pass);
-}
+/* Pair buffers: the move writes buffer 0; pass p reads buffer 0 (p = 0) or
+ * 1 + ((p - 1) & 1) and writes 1 + (p & 1). */
+static int pass_src(uint32_t p) { return p == 0 ? 0 : 1 + (int)((p - 1) & 1); }
+static int pass_dst(uint32_t p) { return 1 + (int)(p & 1); }
 
-/* Enqueue one iteration (no synchronisation). */
-static int enqueue_step(pp_sim_t *s) {
+template <int KPT, int RBITS>
+static void launch_sort(pp_sim_t *s, uint32_t pass) {
 	PendingEvent pe;
-	prof_begin(s, KI_MOVE, pe);
-	k_move<<<s->grid_move, WG, 0, s->stream>>>(s->cell, s->energy, s->type,
-		s->keys[0], s->vals[0], s->ctl, s->ghist, s->P);
-	prof_end(s, pe);
-	for (uint32_t pass = 0; pass < s->P.passes; pass++) {
-		prof_begin(s, KI_SORT_PASS, pe);
-		switch (s->kpt) {
-		case 4: launch_sort_pass<4>(s, pass); break;
-		case 16: launch_sort_pass<16>(s, pass); break;
-		default: launch_sort_pass<8>(s, pass); break;
-		}
+	const int src = pass_src(pass), dst = pass_dst(pass);
+	if (pass > 0) {
+		prof_begin(s, KI_SORT_UP, pe);
+		k_sort_up<KPT><<<s->grid_up, WG, 0, s->stream>>>(s->keys[src], s->hist,
+			s->ctl, pass, s->P);
 		prof_end(s, pe);
 	}
-	const int sorted = s->P.passes & 1;
-	prof_begin(s, KI_ACTION, pe);
-	k_action<<<s->grid_act, WG, 0, s->stream>>>(s->keys[sorted],
-		s->vals[sorted], s->cell, s->energy, s->type, s->grass_at, s->ctl,
-		s->lb_act, s->P);
+	prof_begin(s, KI_SORT_SCAN, pe);
+	k_sort_scan<<<1u << s->P.digit_bits, WG, 0, s->stream>>>(s->hist,
+		s->totals, s->ctl, pass, s->P, s->act_cnt, s->act_words);
 	prof_end(s, pe);
-	prof_begin(s, KI_FINALIZE, pe);
-	k_finalize<<<1, WG, 0, s->stream>>>(s->ctl, s->ring, s->ghist,
+	prof_begin(s, KI_SORT_DOWN, pe);
+	k_sort_down<KPT, RBITS><<<s->grid_down, WG, 0, s->stream>>>(s->keys[src],
+		s->vals[src], s->keys[dst], s->vals[dst], s->hist, s->totals, s->ctl,
+		pass, s->P);
+	prof_end(s, pe);
+}
+
+static void launch_sort_pass(pp_sim_t *s, uint32_t pass) {
+	if (s->kpt == 8) {
+		if (s->rbits == 11) launch_sort<8, 11>(s, pass);
+		else launch_sort<8, 8>(s, pass);
+	} else {
+		if (s->rbits == 11) launch_sort<16, 11>(s, pass);
+		else launch_sort<16, 8>(s, pass);
+	}
+}
+
+/* Enqueue one iteration (no synchronisation): the sort of the pairs the
+ * previous iteration's emit (or the move after init) produced, the actions,
+ * the scan + end of iteration, and the emit + move of the next iteration. */
+static int enqueue_step(pp_sim_t *s) {
+	PendingEvent pe;
+	for (uint32_t pass = 0; pass < s->P.passes; pass++) launch_sort_pass(s, pass);
+	const int sorted = pass_dst(s->P.passes - 1);
+	prof_begin(s, KI_ACT, pe);
+	k_act<<<s->grid_act, WG, 0, s->stream>>>(s->keys[sorted], s->vals[sorted],
+		s->grass_at, s->act_cnt, s->hist, s->hist_words, s->ctl, s->P);
+	prof_end(s, pe);
+	prof_begin(s, KI_ACT_SCAN, pe);
+	k_act_scan<<<1, SCAN_WG, 0, s->stream>>>(s->act_cnt, s->ctl, s->ring,
 		s->stats_dev, s->P);
+	prof_end(s, pe);
+	prof_begin(s, KI_EMIT_MOVE, pe);
+	if (s->rbits == 11)
+		k_emit_move<11><<<s->grid_emit, WG, 0, s->stream>>>(s->keys[sorted],
+			s->vals[sorted], s->act_cnt, s->cell, s->energy, s->type,
+			s->keys[0], s->vals[0], s->hist, s->ctl, s->P);
+	else
+		k_emit_move<8><<<s->grid_emit, WG, 0, s->stream>>>(s->keys[sorted],
+			s->vals[sorted], s->act_cnt, s->cell, s->energy, s->type,
+			s->keys[0], s->vals[0], s->hist, s->ctl, s->P);
 	prof_end(s, pe);
 	HIP_OK(hipGetLastError());
 	return PP_SUCCESS;
@@ -916,15 +1227,15 @@ static void sim_free(pp_sim_t *s) {
 	(void)hipFree(s->cell);
 	(void)hipFree(s->energy);
 	(void)hipFree(s->type);
-	for (int i = 0; i < 2; i++) {
+	for (int i = 0; i < 3; i++) {
 		(void)hipFree(s->keys[i]);
 		(void)hipFree(s->vals[i]);
 	}
 	(void)hipFree(s->grass_at);
 	(void)hipFree(s->ring);
-	(void)hipFree(s->ghist);
-	(void)hipFree(s->lb_sort);
-	(void)hipFree(s->lb_act);
+	(void)hipFree(s->hist);
+	(void)hipFree(s->totals);
+	(void)hipFree(s->act_cnt);
 	(void)hipFree(s->ctl);
 	(void)hipFree(s->stats_dev);
 	if (s->stats_host) (void)hipHostFree(s->stats_host);
@@ -935,32 +1246,31 @@ static void sim_free(pp_sim_t *s) {
 
 static int sim_alloc(pp_sim_t *s) {
 	const uint64_t A = s->cap, C = s->P.C;
-	HIP_OK(hipMalloc(&s->cell, A * 4));
-	HIP_OK(hipMalloc(&s->energy, A * 4));
-	HIP_OK(hipMalloc(&s->type, A));
-	for (int i = 0; i < 2; i++) {
-		HIP_OK(hipMalloc(&s->keys[i], A * 4));
-		HIP_OK(hipMalloc(&s->vals[i], A * 4));
+	/* Vector accesses of the move read/write 16 bytes at multiples of 4. */
+	const uint64_t A4 = (A + 3) & ~3ull;
+	HIP_OK(hipMalloc(&s->cell, A4 * 4));
+	HIP_OK(hipMalloc(&s->energy, A4 * 4));
+	HIP_OK(hipMalloc(&s->type, A4));
+	for (int i = 0; i < 3; i++) {
+		HIP_OK(hipMalloc(&s->keys[i], A4 * 4));
+		HIP_OK(hipMalloc(&s->vals[i], A4 * 4));
 	}
 	HIP_OK(hipMalloc(&s->grass_at, C * 4));
 	s->ring_n = s->P.R + 1;
 	HIP_OK(hipMalloc(&s->ring, (size_t)s->ring_n * 8));
-	HIP_OK(hipMalloc(&s->ghist, MAX_PASSES * RADIX * 4));
-	const uint64_t sort_tiles = ceil_div(A, s->sort_tile);
-	const uint64_t act_tiles = ceil_div(A, WG);
-	HIP_OK(hipMalloc(&s->lb_sort, sort_tiles * RADIX * 8));
-	HIP_OK(hipMalloc(&s->lb_act, act_tiles * 8));
+	s->hist_words = (1u << s->P.digit_bits) * s->P.hist_cols;
+	HIP_OK(hipMalloc(&s->hist, (size_t)s->hist_words * 4));
+	HIP_OK(hipMalloc(&s->totals, RADIX_MAX * 4));
+	s->act_words = ceil_div(A, A_TILE) + 1;
+	HIP_OK(hipMalloc(&s->act_cnt, (size_t)s->act_words * 4));
 	HIP_OK(hipMalloc(&s->ctl, sizeof(Ctl)));
 	HIP_OK(hipMalloc(&s->stats_dev, STATS_RING * sizeof(pp_stats64_t)));
 	HIP_OK(hipHostMalloc(&s->stats_host, STATS_RING * sizeof(pp_stats64_t),
 		hipHostMallocDefault));
 	HIP_OK(hipHostMalloc(&s->ctl_host, sizeof(Ctl), hipHostMallocDefault));
-	/* Zeroed once at creation, outside any graph: epoch 0 never matches. */
-	HIP_OK(hipMemset(s->lb_sort, 0, sort_tiles * RADIX * 8));
-	HIP_OK(hipMemset(s->lb_act, 0, act_tiles * 8));
 	HIP_OK(hipMemset(s->ctl, 0, sizeof(Ctl)));
-	HIP_OK(hipMemset(s->ghist, 0, MAX_PASSES * RADIX * 4));
 	HIP_OK(hipDeviceSynchronize());
+	memset(s->ctl_host, 0, sizeof(Ctl));
 	return PP_SUCCESS;
 }
 
@@ -982,8 +1292,8 @@ int pp_sim_create(const pp_params_t *p, uint64_t seed, uint64_t capacity,
 		capacity = 4 * n0;
 		if (capacity < PP_DEFAULT_MAX_AGENTS) capacity = PP_DEFAULT_MAX_AGENTS;
 	}
-	/* Look-back words carry 30-bit counts; an iteration can at most double
-	 * the survivors, so the capacity stays below 2^29. */
+	/* Output offsets and ids are 32-bit, and an iteration can at most double
+	 * the survivors. */
 	if (capacity < n0 || capacity >= (1ull << 29)) return PP_INVALID_ARGS;
 
 	pp_sim_t *s = new pp_sim();
@@ -1029,27 +1339,34 @@ int pp_sim_create(const pp_params_t *p, uint64_t seed, uint64_t capacity,
 	P.init_sheep = p->init_sheep;
 	P.n0 = (uint32_t)n0;
 	P.cap = s->cap;
+	/* Radix passes over the significant bits of the largest cell index, the
+	 * bits spread evenly: up to 22 bits in passes of at most 11 bits (2048
+	 * buckets; 1600 x 1600 = 2 x 11), wider indices in passes of at most 8
+	 * bits (12800 x 12800 = 4 x 7), where longer runs per bucket keep the
+	 * scatter coalesced. */
 	uint32_t bits = 0;
 	while (bits < 32 && ((C - 1) >> bits) != 0) bits++;
-	P.passes = bits == 0 ? 1 : (bits + 7) / 8;
+	if (bits == 0) bits = 1;
+	const uint32_t max_bits = bits <= 22 ? 11 : 8;
+	P.passes = (bits + max_bits - 1) / max_bits;
+	P.digit_bits = (bits + P.passes - 1) / P.passes;
+	s->rbits = P.digit_bits > 8 ? 11 : 8;
 
-	s->kpt = 8;
+	s->kpt = 16;
 	const char *kpt_env = getenv("PPHPC_SORT_KPT");
-	if (kpt_env) {
-		int v = atoi(kpt_env);
-		if (v == 4 || v == 8 || v == 16) s->kpt = v;
-	}
-	s->sort_tile = WG * s->kpt;
-	/* Persistent grids: enough groups to fill every CU several times, never
-	 * more than the tiles of a full-capacity population. */
-	s->grid_move = min_u32(ceil_div(s->cap, WG), 8 * s->num_cu);
-	s->grid_sort = min_u32(ceil_div(s->cap, s->sort_tile), 4 * s->num_cu);
-	s->grid_act = min_u32(ceil_div(s->cap, WG), 8 * s->num_cu);
-	s->grid_cells = min_u32(ceil_div(C, WG), 8 * s->num_cu);
-	s->grid_agents = min_u32(ceil_div(n0 ? n0 : 1, WG), 8 * s->num_cu);
-	if (s->grid_move == 0) s->grid_move = 1;
-	if (s->grid_sort == 0) s->grid_sort = 1;
-	if (s->grid_act == 0) s->grid_act = 1;
+	if (kpt_env && atoi(kpt_env) == 8) s->kpt = 8;
+	P.sort_tile = WG * s->kpt;
+	P.hist_cols = ceil_div(s->cap, P.sort_tile);
+	/* Grids: enough groups to fill every CU several times, never more than the
+	 * tiles of a full-capacity population; all loops are grid-strided. */
+	const uint32_t cu = (uint32_t)s->num_cu;
+	s->grid_move = min_u32(ceil_div(s->cap, WG * M_IPT), 8 * cu);
+	s->grid_up = min_u32(P.hist_cols, 8 * cu);
+	s->grid_down = min_u32(P.hist_cols, (s->rbits == 11 ? 1 : 2) * cu);
+	s->grid_act = min_u32(ceil_div(s->cap, A_TILE), 8 * cu);
+	s->grid_emit = min_u32(ceil_div(s->cap, A_TILE), 8 * cu);
+	s->grid_cells = min_u32(ceil_div(C, WG), 8 * cu);
+	s->grid_agents = min_u32(ceil_div(n0 ? n0 : 1, WG), 8 * cu);
 
 	int rc = sim_alloc(s);
 	if (rc != PP_SUCCESS) {
@@ -1067,8 +1384,8 @@ int pp_sim_init(pp_sim_t *s, pp_stats64_t *stats0) {
 	HIP_OK(hipSetDevice(s->device));
 	PendingEvent pe;
 	prof_begin(s, KI_INIT_RESET, pe);
-	k_init_reset<<<min_u32(ceil_div(s->ring_n, WG), 1024), WG, 0, s->stream>>>(
-		s->ctl, s->ring, s->ring_n, s->ghist);
+	k_init_reset<<<min_u32(ceil_div(s->ring_n + s->hist_words, WG), 2048), WG, 0,
+		s->stream>>>(s->ctl, s->ring, s->ring_n, s->hist, s->hist_words);
 	prof_end(s, pe);
 	prof_begin(s, KI_INIT_CELLS, pe);
 	k_init_cells<<<s->grid_cells, WG, 0, s->stream>>>(s->grass_at, s->ring,
@@ -1080,6 +1397,10 @@ int pp_sim_init(pp_sim_t *s, pp_stats64_t *stats0) {
 	prof_end(s, pe);
 	prof_begin(s, KI_INIT_DONE, pe);
 	k_init_done<<<1, 64, 0, s->stream>>>(s->ctl, s->stats_dev, s->P);
+	prof_end(s, pe);
+	prof_begin(s, KI_MOVE0, pe);
+	k_move0<<<s->grid_move, WG, 0, s->stream>>>(s->cell, s->energy, s->type,
+		s->keys[0], s->vals[0], s->hist, s->ctl, s->P);
 	prof_end(s, pe);
 	HIP_OK(hipGetLastError());
 	HIP_OK(hipMemcpyAsync(s->stats_host, s->stats_dev, sizeof(pp_stats64_t),
@@ -1271,8 +1592,8 @@ int pp_sim_info(const pp_sim_t *s, uint32_t *sort_passes, uint32_t *sort_tile,
 	uint32_t *action_tile, uint32_t *grid_move) {
 	if (!s) return PP_INVALID_ARGS;
 	if (sort_passes) *sort_passes = s->P.passes;
-	if (sort_tile) *sort_tile = s->sort_tile;
-	if (action_tile) *action_tile = WG;
+	if (sort_tile) *sort_tile = s->P.sort_tile;
+	if (action_tile) *action_tile = A_TILE;
 	if (grid_move) *grid_move = s->grid_move;
 	return PP_SUCCESS;
 }

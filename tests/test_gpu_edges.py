@@ -58,7 +58,8 @@ def test_cli_writes_the_oracle_stats_file(tmp_path):
     assert open(ours, "rb").read() == open(ref, "rb").read()
     lines = open(prof).read().splitlines()
     names = [ln.split("\t")[0].strip('"') for ln in lines]
-    assert {"move", "sort_pass", "action", "finalize"} <= set(names)
+    assert {"move0", "sort_scan", "sort_down", "act", "act_scan",
+            "emit_move"} <= set(names)
     assert abs(sum(float(ln.split("\t")[2]) for ln in lines) - 1.0) < 1e-6
 
 
@@ -144,8 +145,11 @@ def test_reinit_graph_and_profiling_modes_agree():
     sim.close()
     assert np.array_equal(a, b) and np.array_equal(a, c) and \
         np.array_equal(a, e)
-    assert prof["move"][0] == 300 and prof["action"][0] == 300
-    assert prof["sort_pass"][0] == 300 * 2     # 40000 cells: 16 bits
+    assert prof["act"][0] == 300 and prof["emit_move"][0] == 300
+    assert prof["act_scan"][0] == 300 and prof["move0"][0] == 1
+    # 40000 cells: 16 bits = 2 passes of 8; only pass 1 needs an upsweep
+    assert prof["sort_down"][0] == 600 and prof["sort_scan"][0] == 600
+    assert prof["sort_up"][0] == 300
 
 
 def test_step_by_step_equals_run():
