@@ -39,7 +39,8 @@
  * Device state of one simulation (capacity = A agents, C cells, T sort tiles):
  *   agents (SoA)      cell u32[A], energy i32[A], type u8[A]       9 B/agent
  *   sort pairs        key u32[A], val u32[A], three buffers       24 B/agent
- *   grass             regrow step u32[C]                           4 B/cell
+ *   grass             regrow step modulo 2^8 (R <= 191), 2^16 or 2^32,
+ *                     1, 2 or 4 B/cell
  *   grass ring        u64[R+1] cells per regrow step (mod R+1)
  *   tile histograms   u32[radix][T], digit totals u32[radix]
  *   action tiles      u32 output count per 1024 sorted positions
@@ -71,6 +72,10 @@ constexpr int M_IPT = 4;            /* agents per thread (move) */
 constexpr int A_IPT = 4;            /* sorted positions per thread (action) */
 constexpr int A_TILE = WG * A_IPT;
 constexpr int A_STAGE = 2 * A_TILE; /* staged outputs per action tile */
+/* Grass regrow steps are stored modulo 2^(8 sizeof(GT)); alive cells are
+ * re-stamped once every REFRESH iterations (1/REFRESH of the grid per
+ * iteration), so every stored value is within R + REFRESH + 1 of the step. */
+constexpr uint32_t REFRESH = 64;
 
 /* Packed agent value carried through the sort and the action kernels. */
 constexpr uint32_t V_ENERGY = 0x00FFFFFFu; /* energy, 24 bits */
@@ -172,6 +177,18 @@ __device__ __forceinline__ void set_error(Ctl *ctl, int32_t code,
 	if (atomicCAS(&ctl->error, 0, code) == 0) ctl->err_step = step;
 }
 
+/* Grass of a cell, stored as its regrow step a (the cell is alive at state
+ * s iff a <= s; its countdown is max(0, a - s)) modulo 2^bits of GT. */
+template <typename GT>
+__device__ __forceinline__ bool grass_alive_at(GT g, uint32_t s, uint32_t R) {
+	/* dead iff a in [s + 1, s + R], i.e. (s - a) mod 2^w in [2^w - R, 2^w) */
+	const uint32_t w = 8u * sizeof(GT);
+	const uint32_t delta = (s - (uint32_t)g) & (w == 32 ? 0xFFFFFFFFu
+		: ((1u << w) - 1u));
+	const uint32_t dead_from = (w == 32 ? 0u : (1u << w)) - R;
+	return delta < dead_from;
+}
+
 /* Adds 1 to s_hist[d] for every lane of the wave, merging runs of equal
  * digits in consecutive lanes first (agents arrive ordered by cell, so high
  * digits form long runs). NO_DIGIT lanes count nothing. */
@@ -252,7 +269,8 @@ __global__ __launch_bounds__(WG) void k_init_reset(Ctl *ctl,
  * 1 + U[0, R) (word 1), Philox counter (cell, 0, INIT_CELL, 0). Follows
  * init_cell (pp_gpu.cl:368-398) and pphpc.nlogo:27-35. The grass of a cell is
  * stored as its regrow step: countdown(step s) = max(0, regrow - s). */
-__global__ __launch_bounds__(WG) void k_init_cells(uint32_t *grass_at,
+template <typename GT>
+__global__ __launch_bounds__(WG) void k_init_cells(GT *grass_at,
 	unsigned long long *ring, Ctl *ctl, Params P) {
 	__shared__ uint32_t s_hist[4096];
 	__shared__ unsigned long long s_red[NWAVE][2];
@@ -266,7 +284,7 @@ __global__ __launch_bounds__(WG) void k_init_cells(uint32_t *grass_at,
 		const uint4 r = pp_philox(c, 0, PP_STREAM_INIT_CELL, 0, P.key0, P.key1);
 		uint32_t g = 0;
 		if (pp_bounded(r.x, 2) != 1) g = 1 + pp_bounded(r.y, P.R);
-		grass_at[c] = g;
+		grass_at[c] = (GT)g;
 		if (g) {
 			v[0] += 1;
 			v[1] += g;
@@ -457,17 +475,32 @@ __global__ __launch_bounds__(WG) void k_sort_up(const uint32_t *__restrict__ kin
 
 /* One work-group per digit: hist[d][0..ntiles) -> exclusive prefix, and the
  * digit's total. Each thread sums a contiguous run of tiles. The pass-0 scan
- * also clears the action tiles' counters for this iteration. */
+ * also clears the action tiles' counters for this iteration and re-stamps
+ * one grass slice. */
+template <typename GT>
 __global__ __launch_bounds__(WG) void k_sort_scan(uint32_t *__restrict__ hist,
 	uint32_t *__restrict__ totals, const Ctl *ctl, uint32_t pass, Params P,
-	uint32_t *__restrict__ act_cnt, uint32_t act_words) {
+	uint32_t *__restrict__ act_cnt, uint32_t act_words, GT *grass_at) {
 	__shared__ uint32_t s_tmp[NWAVE];
 	if (ctl->error) return;
-	/* The action tiles' counters of this iteration start at zero. */
-	if (pass == 0)
+	if (pass == 0) {
+		/* The action tiles' counters of this iteration start at zero. */
 		for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < act_words;
 			i += gridDim.x * WG)
 			act_cnt[i] = 0;
+		/* Re-stamp the alive cells of slice (s mod REFRESH) with s, which
+		 * keeps their stored regrow step within the modular window; the
+		 * state does not change (alive stays alive, countdown 0). */
+		const uint32_t s = ctl->step;
+		const uint32_t per = (P.C + REFRESH - 1) / REFRESH;
+		const uint32_t c0 = (s % REFRESH) * per;
+		const uint32_t c1 = min(c0 + per, P.C);
+		for (uint32_t c = c0 + blockIdx.x * WG + threadIdx.x; c < c1;
+			c += gridDim.x * WG) {
+			const GT g = grass_at[c];
+			if (grass_alive_at<GT>(g, s, P.R) && g != (GT)s) grass_at[c] = (GT)s;
+		}
+	}
 	const uint32_t n = pass_len(ctl, pass, ctl->step);
 	const uint32_t ntiles = (n + P.sort_tile - 1) / P.sort_tile;
 	const uint32_t radix = 1u << P.digit_bits;
@@ -690,11 +723,15 @@ __device__ __forceinline__ uint32_t load_tile(const uint32_t *__restrict__ keys,
  * agents the positions of tile t emit. Counts and energy sums of the new state
  * are reduced here. This kernel also clears the pass-0 histogram table that
  * k_emit_move fills for the next iteration. */
+template <typename GT>
 __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
-	uint32_t *vals, uint32_t *grass_at, uint32_t *act_cnt,
+	uint32_t *vals, GT *grass_at, uint32_t *act_cnt,
 	uint32_t *__restrict__ hist, uint32_t hist_words, Ctl *ctl, Params P) {
 	__shared__ uint32_t s_key[A_TILE + 1];
 	__shared__ uint32_t s_val[A_TILE];
+	__shared__ uint32_t s_slow_lp[A_TILE];
+	__shared__ GT s_slow_g[A_TILE];
+	__shared__ uint32_t s_nslow;
 	__shared__ uint32_t s_tmp[NWAVE];
 	__shared__ unsigned long long s_red[NWAVE][6];
 
@@ -706,6 +743,7 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 	const uint32_t L = ctl->live[step & 1];
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
 	const uint32_t alive_at = step + 1;
+	const GT eaten_until = (GT)(step + 1 + P.R);
 	unsigned long long acc[6] = {0, 0, 0, 0, 0, 0};
 	bool overflow = false;
 
@@ -715,21 +753,71 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 		const SegVals sv{s_val, vals + base};
 		/* Heads of this thread's positions; their grass loads are issued
 		 * together before any segment is processed. */
-		uint32_t cj[A_IPT], gj[A_IPT];
+		uint32_t cj[A_IPT];
+		GT gj[A_IPT];
 		bool hj[A_IPT];
 #pragma unroll
 		for (int j = 0; j < A_IPT; j++) {
 			const uint32_t lp = tid * A_IPT + j;
 			cj[j] = s_key[1 + lp];
 			hj[j] = lp < cnt && s_key[lp] != cj[j];
-			gj[j] = hj[j] ? grass_at[cj[j]] : 0u;
+			gj[j] = hj[j] ? grass_at[cj[j]] : (GT)0;
 		}
+		/* Cells with one agent (most of them) are handled in place; heads of
+		 * longer segments are listed and then processed by consecutive
+		 * lanes, so the long serial path runs once per wave, not once per
+		 * position of every lane. Segments are independent: the order of the
+		 * list does not change any result. */
+		if (tid == 0) s_nslow = 0;
+		__syncthreads();
 		uint32_t nout = 0;
-#pragma unroll 1
+#pragma unroll
 		for (int j = 0; j < A_IPT; j++) {
 			if (!hj[j]) continue;
 			const uint32_t lp = tid * A_IPT + j;
 			const uint32_t c = cj[j];
+			if (lp + 1 < cnt && s_key[2 + lp] != c) {
+				/* A cell with one agent (most of them): nothing to shuffle, a
+				 * lone wolf finds no sheep, a lone agent is never eaten. */
+				const uint32_t v = s_val[lp];
+				int32_t e = (int32_t)(v & V_ENERGY);
+				const uint32_t t = (v & V_WOLF) ? 1u : 0u;
+				uint32_t thr = P.wolves_thr;
+				if (!t) {
+					thr = P.sheep_thr;
+					if (grass_alive_at<GT>(gj[j], alive_at, P.R)) {
+						e += (int32_t)P.sheep_gain;
+						grass_at[c] = eaten_until;
+						acc[4] += 1;
+					}
+				}
+				uint32_t nv = v & V_WOLF;
+				uint32_t ce = 0;
+				if (e > (int32_t)thr && (v & V_REPRO)) {
+					nv |= V_CHILD | ((e & 1) ? V_ODD : 0u);
+					ce = (uint32_t)(e / 2);
+					e -= e / 2;
+				}
+				overflow |= e > (int32_t)V_ENERGY;
+				s_val[lp] = nv | ((uint32_t)e & V_ENERGY);
+				const uint32_t kids = (nv & V_CHILD) ? 1u : 0u;
+				acc[5] += 1;
+				acc[t] += 1 + kids;
+				acc[2 + t] += (uint32_t)e + ce;
+				nout += 1 + kids;
+				continue;
+			}
+			const uint32_t slot = atomicAdd(&s_nslow, 1u);
+			s_slow_lp[slot] = lp;
+			s_slow_g[slot] = gj[j];
+		}
+		__syncthreads();
+		const uint32_t nslow = s_nslow;
+#pragma unroll 1
+		for (uint32_t si = tid; si < nslow; si += WG) {
+			const uint32_t lp = s_slow_lp[si];
+			const uint32_t c = s_key[1 + lp];
+			const GT gslow = s_slow_g[si];
 			const uint32_t end = seg_end(s_key, keys, lp, cnt, base, L, c);
 			const uint32_t k = end - lp;
 			acc[5] += 1;
@@ -749,7 +837,7 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 					}
 				}
 			}
-			bool grass = gj[j] <= alive_at;
+			bool grass = grass_alive_at<GT>(gslow, alive_at, P.R);
 			bool ate = false;
 			for (uint32_t q = lp; q < end; q++) {
 				const uint32_t v = sv.get(q);
@@ -783,7 +871,7 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 				sv.set(q, nv | ((uint32_t)e & V_ENERGY));
 			}
 			if (ate) {
-				grass_at[c] = step + 1 + P.R;
+				grass_at[c] = eaten_until;
 				acc[4] += 1;
 			}
 			for (uint32_t q = lp; q < end; q++) {
@@ -1056,7 +1144,8 @@ struct pp_sim {
 	uint8_t *type;
 	uint32_t *keys[3];
 	uint32_t *vals[3];
-	uint32_t *grass_at;
+	void *grass_at;
+	uint32_t grass_bytes;     /* 1, 2 or 4 bytes per cell */
 	unsigned long long *ring;
 	uint32_t ring_n;
 	uint32_t *hist;
@@ -1153,8 +1242,19 @@ static void launch_sort(pp_sim_t *s, uint32_t pass) {
 		prof_end(s, pe);
 	}
 	prof_begin(s, KI_SORT_SCAN, pe);
-	k_sort_scan<<<1u << s->P.digit_bits, WG, 0, s->stream>>>(s->hist,
-		s->totals, s->ctl, pass, s->P, s->act_cnt, s->act_words);
+	const uint32_t nscan = 1u << s->P.digit_bits;
+	if (s->grass_bytes == 1)
+		k_sort_scan<uint8_t><<<nscan, WG, 0, s->stream>>>(s->hist, s->totals,
+			s->ctl, pass, s->P, s->act_cnt, s->act_words,
+			(uint8_t *)s->grass_at);
+	else if (s->grass_bytes == 2)
+		k_sort_scan<uint16_t><<<nscan, WG, 0, s->stream>>>(s->hist, s->totals,
+			s->ctl, pass, s->P, s->act_cnt, s->act_words,
+			(uint16_t *)s->grass_at);
+	else
+		k_sort_scan<uint32_t><<<nscan, WG, 0, s->stream>>>(s->hist, s->totals,
+			s->ctl, pass, s->P, s->act_cnt, s->act_words,
+			(uint32_t *)s->grass_at);
 	prof_end(s, pe);
 	prof_begin(s, KI_SORT_DOWN, pe);
 	k_sort_down<KPT, RBITS><<<s->grid_down, WG, 0, s->stream>>>(s->keys[src],
@@ -1181,8 +1281,18 @@ static int enqueue_step(pp_sim_t *s) {
 	for (uint32_t pass = 0; pass < s->P.passes; pass++) launch_sort_pass(s, pass);
 	const int sorted = pass_dst(s->P.passes - 1);
 	prof_begin(s, KI_ACT, pe);
-	k_act<<<s->grid_act, WG, 0, s->stream>>>(s->keys[sorted], s->vals[sorted],
-		s->grass_at, s->act_cnt, s->hist, s->hist_words, s->ctl, s->P);
+	if (s->grass_bytes == 1)
+		k_act<uint8_t><<<s->grid_act, WG, 0, s->stream>>>(s->keys[sorted],
+			s->vals[sorted], (uint8_t *)s->grass_at, s->act_cnt, s->hist,
+			s->hist_words, s->ctl, s->P);
+	else if (s->grass_bytes == 2)
+		k_act<uint16_t><<<s->grid_act, WG, 0, s->stream>>>(s->keys[sorted],
+			s->vals[sorted], (uint16_t *)s->grass_at, s->act_cnt, s->hist,
+			s->hist_words, s->ctl, s->P);
+	else
+		k_act<uint32_t><<<s->grid_act, WG, 0, s->stream>>>(s->keys[sorted],
+			s->vals[sorted], (uint32_t *)s->grass_at, s->act_cnt, s->hist,
+			s->hist_words, s->ctl, s->P);
 	prof_end(s, pe);
 	prof_begin(s, KI_ACT_SCAN, pe);
 	k_act_scan<<<1, SCAN_WG, 0, s->stream>>>(s->act_cnt, s->ctl, s->ring,
@@ -1255,7 +1365,7 @@ static int sim_alloc(pp_sim_t *s) {
 		HIP_OK(hipMalloc(&s->keys[i], A4 * 4));
 		HIP_OK(hipMalloc(&s->vals[i], A4 * 4));
 	}
-	HIP_OK(hipMalloc(&s->grass_at, C * 4));
+	HIP_OK(hipMalloc(&s->grass_at, C * s->grass_bytes));
 	s->ring_n = s->P.R + 1;
 	HIP_OK(hipMalloc(&s->ring, (size_t)s->ring_n * 8));
 	s->hist_words = (1u << s->P.digit_bits) * s->P.hist_cols;
@@ -1352,6 +1462,15 @@ int pp_sim_create(const pp_params_t *p, uint64_t seed, uint64_t capacity,
 	P.digit_bits = (bits + P.passes - 1) / P.passes;
 	s->rbits = P.digit_bits > 8 ? 11 : 8;
 
+	/* Narrowest grass storage whose modular window holds R + REFRESH + 1. */
+	s->grass_bytes = P.R + REFRESH + 1 <= 256 ? 1
+		: P.R + REFRESH + 1 <= 65536 ? 2 : 4;
+	const char *gb_env = getenv("PPHPC_GRASS_BYTES");
+	if (gb_env) {
+		const int v = atoi(gb_env);
+		if ((v == 2 && P.R + REFRESH + 1 <= 65536) || v == 4) s->grass_bytes = v;
+	}
+
 	s->kpt = 16;
 	const char *kpt_env = getenv("PPHPC_SORT_KPT");
 	if (kpt_env && atoi(kpt_env) == 8) s->kpt = 8;
@@ -1388,8 +1507,15 @@ int pp_sim_init(pp_sim_t *s, pp_stats64_t *stats0) {
 		s->stream>>>(s->ctl, s->ring, s->ring_n, s->hist, s->hist_words);
 	prof_end(s, pe);
 	prof_begin(s, KI_INIT_CELLS, pe);
-	k_init_cells<<<s->grid_cells, WG, 0, s->stream>>>(s->grass_at, s->ring,
-		s->ctl, s->P);
+	if (s->grass_bytes == 1)
+		k_init_cells<uint8_t><<<s->grid_cells, WG, 0, s->stream>>>(
+			(uint8_t *)s->grass_at, s->ring, s->ctl, s->P);
+	else if (s->grass_bytes == 2)
+		k_init_cells<uint16_t><<<s->grid_cells, WG, 0, s->stream>>>(
+			(uint16_t *)s->grass_at, s->ring, s->ctl, s->P);
+	else
+		k_init_cells<uint32_t><<<s->grid_cells, WG, 0, s->stream>>>(
+			(uint32_t *)s->grass_at, s->ring, s->ctl, s->P);
 	prof_end(s, pe);
 	prof_begin(s, KI_INIT_AGENTS, pe);
 	k_init_agents<<<s->grid_agents, WG, 0, s->stream>>>(s->cell, s->energy,
@@ -1536,11 +1662,22 @@ int pp_sim_get_state(pp_sim_t *s, uint32_t *cell, int32_t *energy,
 		HIP_OK(hipMemcpy(energy, s->energy, n * 4, hipMemcpyDeviceToHost));
 	if (type && n) HIP_OK(hipMemcpy(type, s->type, n, hipMemcpyDeviceToHost));
 	if (grass) {
-		HIP_OK(hipMemcpy(grass, s->grass_at, (size_t)s->P.C * 4,
+		/* Countdown at state st from the regrow step a stored modulo 2^w:
+		 * d = (a - st) mod 2^w is the countdown if 1 <= d <= R, else 0. */
+		const uint32_t st = s->ctl_host->step, R = s->P.R, C = s->P.C;
+		const size_t gb = s->grass_bytes;
+		std::vector<uint8_t> raw((size_t)C * gb);
+		HIP_OK(hipMemcpy(raw.data(), s->grass_at, raw.size(),
 			hipMemcpyDeviceToHost));
-		const uint32_t st = s->ctl_host->step;
-		for (uint32_t c = 0; c < s->P.C; c++)
-			grass[c] = grass[c] > st ? grass[c] - st : 0;
+		const uint32_t mask = gb == 4 ? 0xFFFFFFFFu : ((1u << (8 * gb)) - 1u);
+		for (uint32_t c = 0; c < C; c++) {
+			uint32_t a;
+			if (gb == 1) a = raw[c];
+			else if (gb == 2) a = reinterpret_cast<const uint16_t *>(raw.data())[c];
+			else a = reinterpret_cast<const uint32_t *>(raw.data())[c];
+			const uint32_t d = (a - st) & mask;
+			grass[c] = (d >= 1 && d <= R) ? d : 0;
+		}
 	}
 	return PP_SUCCESS;
 }

@@ -205,3 +205,26 @@ def test_full_size_12800_properties():
     assert last[5] == grass.astype(np.uint64).sum()
     assert (np.diff(cell.astype(np.int64)) >= 0).all()
     assert (energy >= 0).all() and cell.max() < 12800 * 12800
+
+
+@pytest.mark.parametrize("width", ["1", "2", "4"])
+def test_grass_storage_widths_agree(width):
+    """The regrow step stored modulo 2^8, 2^16 or 2^32 (with the periodic
+    re-stamp of alive cells) gives the oracle's rows over 1200 iterations
+    (many re-stamp periods) and its final grass countdowns."""
+    _case(f"grass width {width}")
+    from oracle.pp_oracle import Oracle
+    d = v2s(150)
+    os.environ["PPHPC_GRASS_BYTES"] = width
+    try:
+        sim = _sim(d, seed=6)
+    finally:
+        del os.environ["PPHPC_GRASS_BYTES"]
+    o = Oracle(d, seed=6, capacity=sim.capacity)
+    rows = _rows(sim, 1200)
+    ref = [o.stats()]
+    _, more = o.run(1200)
+    assert np.array_equal(rows, np.vstack([np.array(ref, np.uint64), more]))
+    assert_state_equal(sim.state(), o.state(), f"width {width} final")
+    sim.close()
+    o.close()
