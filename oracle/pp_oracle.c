@@ -365,7 +365,8 @@ int ppo_step(ppo_sim *s) {
 				/* Sheep eat live grass (Sheep.java:66-83). */
 				if (s->grass[c] == 0) {
 					s->grass[c] = p->grass_restart;
-					e += (int32_t)p->sheep_gain_from_food;
+					e += (int32_t)p->sheep_gain_from_food
+						* (s->fault == 4 ? 2 : 1);
 					ev[1]++;
 				}
 				thr = p->sheep_reproduce_threshold;

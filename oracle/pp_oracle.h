@@ -103,7 +103,7 @@ uint64_t ppo_agent_steps(const ppo_sim *s);
 /* Fault injection for testing the tests: 0 = none. A non-zero mode plants a
  * deliberate mistake in the step so that parity tests can show they catch
  * it (1: wolf gain off by one, 2: move direction 1 and 2 swapped,
- * 3: shuffle skipped). */
+ * 3: shuffle skipped, 4: sheep gain doubled). */
 void ppo_set_fault(ppo_sim *s, int mode);
 
 #ifdef __cplusplus

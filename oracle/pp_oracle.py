@@ -198,3 +198,51 @@ def simulate(params, seed=0, capacity=0, iters=None):
     if rc:
         raise RuntimeError(f"oracle stopped with {rc}")
     return np.vstack([row0, rows])
+
+
+# ------------------------------------------------------------------------
+# Reference side of the statistical tests (oracle/pp_statref.c)
+
+_SREF = None
+
+
+def statref_lib():
+    global _SREF
+    if _SREF is None:
+        path = os.path.join(BUILD, "libpp_statref.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.ppr_run.restype = ctypes.c_int
+        L.ppr_run.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
+                              ctypes.c_uint32, ctypes.c_void_p]
+        L.ppr_mt_outputs.restype = None
+        L.ppr_mt_outputs.argtypes = [ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.c_void_p]
+        _SREF = L
+    return _SREF
+
+
+def statref_run(params, seed, iters=None):
+    """(iters+1, 7) uint64 rows of one NetLogo-rules replica (MT19937)."""
+    p = params_from(params)
+    it = p.iters if iters is None else iters
+    arr = (ctypes.c_uint32 * 11)(p.init_sheep, p.sheep_gain_from_food,
+                                 p.sheep_reproduce_threshold,
+                                 p.sheep_reproduce_prob, p.init_wolves,
+                                 p.wolves_gain_from_food,
+                                 p.wolves_reproduce_threshold,
+                                 p.wolves_reproduce_prob, p.grass_restart,
+                                 p.grid_x, p.grid_y)
+    rows = np.zeros((it + 1, 7), np.uint64)
+    rc = statref_lib().ppr_run(arr, seed, it,
+                               rows.ctypes.data_as(ctypes.c_void_p))
+    if rc:
+        raise ValueError("statref rejected the parameters")
+    return rows
+
+
+def mt_outputs(seed, n):
+    out = np.zeros(n, np.uint32)
+    statref_lib().ppr_mt_outputs(seed, n, out.ctypes.data_as(ctypes.c_void_p))
+    return out
