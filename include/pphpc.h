@@ -166,6 +166,109 @@ void pp_sim_destroy(pp_sim_t *s);
 /* Blocks until all work of the handle's stream has finished. */
 int pp_sim_sync(pp_sim_t *s);
 
+/* ------------------------------------------------------------------------
+ * Batches of replicas. No counterpart in the reference, which runs one
+ * simulation per process with its parameters compiled into the kernels as
+ * -D constants (pp_gpu.c:1522-1574); here they are per-replica table
+ * entries, so replicas of different grids, parameters, seeds and iteration
+ * counts advance together, one set of kernel launches per tick. A replica in
+ * a batch produces exactly the statistics rows it produces alone
+ * (pp_sim_simulate with the same parameters and seed). A pp_sim_t is a batch
+ * of one.
+ *
+ * Admission is by device footprint: a batch allocates its buffers once
+ * (pp_batch_footprint bytes); a replica needs a free slot, grid_xy
+ * contiguous free cells and an agent reservation (the most agents it may
+ * hold after births; above it the replica fails with PP_OUT_OF_RESOURCES,
+ * alone). pp_batch_add returns PP_OUT_OF_RESOURCES when these are not free
+ * now; slots, cells and reservations come back when replicas finish.
+ * ---------------------------------------------------------------------- */
+
+typedef struct pp_batch pp_batch_t;
+
+/* Replica states. */
+#define PP_REPLICA_RUNNING 1
+#define PP_REPLICA_DONE 2
+#define PP_REPLICA_FAILED 3
+
+/* Device bytes a batch with these limits allocates. 0 for invalid limits. */
+uint64_t pp_batch_footprint(uint64_t agent_capacity, uint64_t cell_capacity,
+	uint32_t max_replicas, uint32_t max_grass_restart);
+
+/* A batch holding up to agent_capacity agents (< 2^29) over all replicas,
+ * cell_capacity cells, max_replicas replicas at once, each with
+ * grass_restart <= max_grass_restart. device < 0 = current device. */
+int pp_batch_create(uint64_t agent_capacity, uint64_t cell_capacity,
+	uint32_t max_replicas, uint32_t max_grass_restart, int device,
+	pp_batch_t **out);
+
+/* Admits a replica for p->iters iterations with RNG seed `seed` and agent
+ * reservation agent_reserve (0 = 4 x initial agents). Its row 0 is written
+ * at admission; *id names it until pp_batch_release. */
+int pp_batch_add(pp_batch_t *b, const pp_params_t *p, uint64_t seed,
+	uint64_t agent_reserve, uint32_t *id);
+
+/* Advances every running replica by up to `ticks` iterations (each stops at
+ * its own count), collecting their statistics rows; returns early once no
+ * replica runs. */
+int pp_batch_run(pp_batch_t *b, uint32_t ticks);
+
+/* State (PP_REPLICA_*), completed iterations, error code (0 unless failed)
+ * and agent-steps of replica id. */
+int pp_batch_replica(pp_batch_t *b, uint32_t id, int *state, uint32_t *steps,
+	int *error, uint64_t *agent_steps);
+
+/* Statistics rows first .. first + n - 1 of replica id (row 0 = initial
+ * state), as pp_sim_simulate writes them. */
+int pp_batch_rows(pp_batch_t *b, uint32_t id, uint32_t first, uint32_t n,
+	pp_stats64_t *out);
+
+/* Current state of a running replica, as pp_sim_get_state (cells local to
+ * its grid); *n_agents receives its agent count. */
+int pp_batch_get_state(pp_batch_t *b, uint32_t id, uint32_t *n_agents,
+	uint32_t *cell, int32_t *energy, uint8_t *type, uint32_t *grass);
+
+/* Forgets a finished (done or failed) replica and its rows. */
+int pp_batch_release(pp_batch_t *b, uint32_t id);
+
+/* Free agent reservation, longest free run of cells, free slots, and
+ * running replicas. */
+int pp_batch_resources(const pp_batch_t *b, uint64_t *agents_free,
+	uint32_t *cells_free_run, uint32_t *slots_free, uint32_t *active);
+
+/* Iterations the batch has run. */
+uint32_t pp_batch_tick(const pp_batch_t *b);
+
+/* Batch-wide event sums, as pp_sim_counters. */
+int pp_batch_counters(const pp_batch_t *b, uint64_t out[5]);
+
+/* Sort passes, sort tile, action tile and emit grid in use. */
+int pp_batch_info(const pp_batch_t *b, uint32_t *sort_passes,
+	uint32_t *sort_tile, uint32_t *action_tile, uint32_t *grid_move);
+
+/* Sort launch shape: keys per thread (8 or 16; tile = 256 x that) and the
+ * widest radix digit (8 or 11 bits). Results do not depend on it. */
+int pp_batch_set_shape(pp_batch_t *b, uint32_t keys_per_thread,
+	uint32_t max_digit_bits);
+
+/* Autotuner: runs trial_ticks iterations of the batch under each candidate
+ * sort shape (the iterations count as normal progress), keeps the fastest
+ * and reports it with its seconds per tick. */
+int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks,
+	uint32_t *keys_per_thread, uint32_t *max_digit_bits, double *sec_per_tick);
+
+/* Per-kernel timing of the batch, as pp_sim_set_profiling / prof_get. */
+int pp_batch_set_profiling(pp_batch_t *b, int enable);
+int pp_batch_prof_count(const pp_batch_t *b);
+int pp_batch_prof_get(pp_batch_t *b, int i, const char **name,
+	uint64_t *launches, double *total_ns);
+int pp_batch_prof_reset(pp_batch_t *b);
+
+int pp_batch_sync(pp_batch_t *b);
+
+/* Releases all device memory of the batch. NULL is accepted. */
+void pp_batch_destroy(pp_batch_t *b);
+
 #ifdef __cplusplus
 }
 #endif

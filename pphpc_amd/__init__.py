@@ -5,13 +5,18 @@ Python mirror of the C ABI in ``include/pphpc.h`` (the library itself is
 layer). The names follow the reference simulator's interface:
 ``load_params`` / ``stats_save`` restate ``pp_load_params`` / ``pp_stats_save``
 (opencl/pp/pp_common.c:39-240) and :class:`Simulation` wraps the loop of
-``ppg_simulate`` (opencl/pp/pp_gpu.c:573-987).
+``ppg_simulate`` (opencl/pp/pp_gpu.c:573-987). :class:`Batch` runs many
+replicas of that loop together on one GPU.
 """
 from .sim import (  # noqa: F401
     PPError,
     Params,
     Stats,
     Simulation,
+    Batch,
+    RUNNING,
+    DONE,
+    FAILED,
     load_params,
     stats_save,
     stats_rows_to_array,

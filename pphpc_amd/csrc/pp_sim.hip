@@ -26,8 +26,22 @@
  *                                            and spawning
  *   reduce_grass1/2, reduce_agent1/2         fused reductions (k_act,
  *   (pp_gpu.cl:485-736)                      k_act_scan) with exact 64-bit sums
- *   stats readback every iteration           device ring of stats rows, one
+ *   stats readback every iteration           device rings of stats rows, one
  *   (pp_gpu.c:766, :840)                     copy per chunk of iterations
+ *
+ * Replicas. A batch holds independent simulations (replicas, each with its
+ * own grid, parameters, seed and iteration count) in one set of buffers:
+ * replica r owns the cells [cb_r, cb_r + C_r) of a global cell space, so one
+ * sort orders every replica's agents at once and every kernel serves all
+ * replicas in the same launch, finding a cell's replica in a table sorted by
+ * first cell. Random draws use replica-local agent ids, cells and iteration
+ * numbers, so a replica evolves exactly as it would alone. The parameters the
+ * reference compiles in as -D constants (pp_gpu.c:1522-1574) are per-replica
+ * table entries here. A single simulation (pp_sim_*) is a batch of one.
+ *
+ * Time. The batch counts iterations in ticks. Replica r, admitted at tick
+ * t0_r, is at its local iteration s = tick - t0_r. Grass regrow steps are
+ * stored in ticks, so the grass of every replica uses one clock.
  *
  * No kernel waits on another work-group: every cross-group dependency is a
  * kernel boundary, so the step has no spin loops.
@@ -36,16 +50,16 @@
  * defined so that a single-threaded CPU implementation reproduces the same
  * statistics bit for bit; see the header of each kernel.
  *
- * Device state of one simulation (capacity = A agents, C cells, T sort tiles):
+ * Device state (capacity = A agents, C cells over all replicas, T sort tiles):
  *   agents (SoA)      cell u32[A], energy i32[A], type u8[A]       9 B/agent
  *   sort pairs        key u32[A], val u32[A], three buffers       24 B/agent
- *   grass             regrow step modulo 2^8 (R <= 191), 2^16 or 2^32,
+ *   grass             regrow tick modulo 2^8 (R <= 191), 2^16 or 2^32,
  *                     1, 2 or 4 B/cell
- *   grass ring        u64[R+1] cells per regrow step (mod R+1)
+ *   grass rings       u64[R_max + 1] per replica slot
  *   tile histograms   u32[radix][T], digit totals u32[radix]
  *   action tiles      u32 output count per 1024 sorted positions
- *   control block     counters, live count, step, error
- *   stats ring        pp_stats64_t[4096]
+ *   replica tables    parameters, control words, statistics rings
+ *   control block     batch counters, tick, error
  */
 #include <hip/hip_runtime.h>
 
@@ -53,7 +67,10 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
+#include <algorithm>
+#include <map>
 #include <vector>
 
 #include "pphpc.h"
@@ -67,14 +84,17 @@ constexpr int NWAVE = WG / WAVE;
 constexpr int RADIX_MAX = 2048;     /* digits of at most 11 bits */
 constexpr uint32_t DEAD_KEY = 0xFFFFFFFFu;
 constexpr uint32_t NO_DIGIT = 0xFFFFFFFFu;
-constexpr uint32_t STATS_RING = 4096;
+constexpr uint32_t NO_REP = 0xFFFFFFFFu;
+constexpr uint32_t STATS_RING = 4096; /* stats rows kept per plain simulation */
+constexpr uint32_t ITERS_UNLIMITED = 0xFFFFFFFFu; /* iterations of a pp_sim */
 constexpr int M_IPT = 4;            /* agents per thread (move) */
 constexpr int A_IPT = 4;            /* sorted positions per thread (action) */
 constexpr int A_TILE = WG * A_IPT;
 constexpr int A_STAGE = 2 * A_TILE; /* staged outputs per action tile */
-/* Grass regrow steps are stored modulo 2^(8 sizeof(GT)); alive cells are
- * re-stamped once every REFRESH iterations (1/REFRESH of the grid per
- * iteration), so every stored value is within R + REFRESH + 1 of the step. */
+constexpr int MIX_SLOTS = 16;       /* replicas per action tile summed in LDS */
+/* Grass regrow ticks are stored modulo 2^(8 sizeof(GT)); alive cells are
+ * re-stamped once every REFRESH ticks (1/REFRESH of the cells per tick), so
+ * every stored value is within R + REFRESH of the tick. */
 constexpr uint32_t REFRESH = 64;
 
 /* Packed agent value carried through the sort and the action kernels. */
@@ -85,39 +105,81 @@ constexpr uint32_t V_KILLED = 1u << 26;    /* eaten by a wolf this step */
 constexpr uint32_t V_CHILD = 1u << 27;     /* reproduced this step */
 constexpr uint32_t V_ODD = 1u << 28;       /* energy before the split was odd */
 
-struct Ctl {
-	uint32_t n;        /* agents of the current state */
-	uint32_t step;     /* completed iterations */
-	int32_t error;     /* 0 or a PP_* code; every kernel is a no-op once set */
-	uint32_t err_step; /* iteration at which error was set */
-	uint32_t live[2];  /* survivors of the move of iteration s, in [s & 1] */
-	uint32_t out_n;    /* agents after actions and births */
-	uint32_t pad;
-	unsigned long long dead_cells;  /* grass cells with countdown > 0 */
-	unsigned long long cd_sum;      /* sum of grass countdowns */
-	unsigned long long eaten;       /* grass eaten in this iteration */
-	unsigned long long acc[4];      /* sheep, wolves, sheep energy, wolf energy */
-	unsigned long long occupied;    /* occupied cells in this iteration */
-	unsigned long long agent_steps; /* sum of n over completed iterations */
-	unsigned long long sum_live;    /* sums over completed iterations of: */
-	unsigned long long sum_out;     /*   live, out_n, occupied, eaten */
-	unsigned long long sum_occ;
-	unsigned long long sum_eaten;
-};
+/* Replica states (the values of PP_REPLICA_* in pphpc.h). */
+constexpr uint32_t RS_FREE = 0;
+constexpr uint32_t RS_ACTIVE = 1;
+constexpr uint32_t RS_DONE = 2;
+constexpr uint32_t RS_FAILED = 3;
 
-struct Params {
-	uint32_t key0, key1;
-	uint32_t gx, gy, C;
-	uint32_t R;
+/* Counters k_act accumulates per replica for the statistics row. */
+enum { AC_SHEEP = 0, AC_WOLVES, AC_SHEEP_EN, AC_WOLVES_EN, AC_EATEN, AC_OCC,
+	AC_OUT, NACC };
+
+/* Per-replica constants, written by the host at admission. */
+struct Rep {
+	uint32_t cb;               /* first global cell */
+	uint32_t C, gx, gy, R;
 	uint32_t sheep_gain, wolves_gain;
 	uint32_t sheep_thr, wolves_thr;
 	uint32_t sheep_prob, wolves_prob;
 	uint32_t init_sheep, n0;
-	uint32_t cap;
-	uint32_t passes;     /* radix passes over the cell index */
+	uint32_t key0, key1;
+	uint32_t t0;               /* tick of admission */
+	uint32_t iters;            /* done after this many iterations */
+	uint32_t cap;              /* agents allowed after births */
+	uint32_t stats_rows;       /* length of its stats ring */
+	uint32_t pad;
+	unsigned long long *ring;  /* R + 1 counters of dead cells per regrow tick */
+	pp_stats64_t *stats;       /* stats ring */
+};
+
+/* Per-replica control words, updated by the kernels. */
+struct RepCtl {
+	uint32_t n;        /* agents of its current state */
+	uint32_t base;     /* position of its first agent in the batch state */
+	uint32_t step;     /* completed iterations */
+	uint32_t status;   /* RS_* */
+	int32_t error;     /* PP_* code of a failed replica */
+	uint32_t err_step; /* iteration at which it failed */
+	uint32_t pad[2];
+	unsigned long long dead_cells;  /* grass cells with countdown > 0 */
+	unsigned long long cd_sum;      /* sum of grass countdowns */
+	unsigned long long acc[NACC];   /* this iteration, AC_* */
+	unsigned long long agent_steps; /* agents at the start, summed */
+};
+
+struct Ctl {
+	uint32_t n;        /* positions of the batch state */
+	uint32_t live[2];  /* survivors of the move, indexed by tick parity */
+	uint32_t tick;     /* iterations run by this batch */
+	int32_t error;     /* 0 or a PP_* code; every kernel is a no-op once set */
+	uint32_t err_step; /* tick at which error was set */
+	int32_t err_rep;   /* slot of the replica involved, -1 for the batch */
+	uint32_t nrep;     /* replicas in the lookup table */
+	/* sums over completed ticks of: agents at the start, alive after the
+	 * move, after actions, occupied cells, grass eaten */
+	unsigned long long agent_steps, sum_live, sum_out, sum_occ, sum_eaten;
+};
+
+/* Batch-wide launch constants. */
+struct Params {
+	uint32_t cap;        /* agent capacity of the buffers */
+	uint32_t passes;     /* radix passes over the global cell index */
 	uint32_t digit_bits; /* bits per pass (<= 11) */
 	uint32_t sort_tile;  /* positions per sort tile */
 	uint32_t hist_cols;  /* tile capacity of the histogram table */
+	uint32_t C;          /* cells of the global cell space */
+	uint32_t R;          /* largest grass restart allowed */
+};
+
+/* Replica tables in device memory: constants and control words by slot, and
+ * the replicas of the batch sorted by first cell (cbs ascending, order[k] =
+ * slot of the k-th). */
+struct Tables {
+	const Rep *rep;
+	RepCtl *rctl;
+	const uint32_t *cbs;
+	const uint32_t *order;
 };
 
 /* ---------------------------------------------------------------- helpers */
@@ -153,39 +215,60 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 	return v;
 }
 
-/* Work-group sum of K counters into global atomics (one per group). */
-template <int K>
+/* Work-group sums of K counters (NT threads), added with one atomic each to
+ * dst[k]; v is cleared. */
+template <int K, int NT = WG>
 __device__ __forceinline__ void wg_add(unsigned long long (&v)[K],
-	unsigned long long (*s_red)[K], unsigned long long *const dst[K]) {
+	unsigned long long (*s_red)[K], unsigned long long *dst) {
 	const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
 #pragma unroll
-	for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
-	if (lane == 0) {
-#pragma unroll
-		for (int k = 0; k < K; k++) s_red[w][k] = v[k];
+	for (int k = 0; k < K; k++) {
+		v[k] = wave_sum(v[k]);
+		if (lane == 0) s_red[w][k] = v[k];
+		v[k] = 0;
 	}
 	__syncthreads();
 	if (threadIdx.x < K) {
 		unsigned long long t = 0;
-		for (int i = 0; i < NWAVE; i++) t += s_red[i][threadIdx.x];
-		if (t) atomicAdd(dst[threadIdx.x], t);
+		for (int i = 0; i < NT / WAVE; i++) t += s_red[i][threadIdx.x];
+		if (t) atomicAdd(&dst[threadIdx.x], t);
 	}
+	__syncthreads();
 }
 
 __device__ __forceinline__ void set_error(Ctl *ctl, int32_t code,
-	uint32_t step) {
-	if (atomicCAS(&ctl->error, 0, code) == 0) ctl->err_step = step;
+	uint32_t tick, int32_t rep) {
+	if (atomicCAS(&ctl->error, 0, code) == 0) {
+		ctl->err_step = tick;
+		ctl->err_rep = rep;
+	}
 }
 
-/* Grass of a cell, stored as its regrow step a (the cell is alive at state
- * s iff a <= s; its countdown is max(0, a - s)) modulo 2^bits of GT. */
+/* Table index of the replica owning global cell c (a cell of some replica):
+ * the largest k with cbs[k] <= c. */
+__device__ __forceinline__ uint32_t rep_index(const Tables &T, uint32_t nrep,
+	uint32_t c) {
+	if (nrep <= 1) return 0;
+	uint32_t lo = 0, hi = nrep;
+	while (hi - lo > 1) {
+		const uint32_t mid = (lo + hi) >> 1;
+		if (T.cbs[mid] <= c) lo = mid;
+		else hi = mid;
+	}
+	return lo;
+}
+
+/* Grass of a cell, stored as its regrow tick a (the cell is alive at tick t
+ * iff a <= t; its countdown is max(0, a - t)) modulo 2^bits of GT. Exact for
+ * every replica with R <= Rmax: alive cells are within REFRESH ticks of their
+ * stored value, dead ones within R. */
 template <typename GT>
-__device__ __forceinline__ bool grass_alive_at(GT g, uint32_t s, uint32_t R) {
-	/* dead iff a in [s + 1, s + R], i.e. (s - a) mod 2^w in [2^w - R, 2^w) */
+__device__ __forceinline__ bool grass_alive_at(GT g, uint32_t t, uint32_t Rmax) {
+	/* dead iff a in [t + 1, t + R], i.e. (t - a) mod 2^w >= 2^w - R */
 	const uint32_t w = 8u * sizeof(GT);
-	const uint32_t delta = (s - (uint32_t)g) & (w == 32 ? 0xFFFFFFFFu
+	const uint32_t delta = (t - (uint32_t)g) & (w == 32 ? 0xFFFFFFFFu
 		: ((1u << w) - 1u));
-	const uint32_t dead_from = (w == 32 ? 0u : (1u << w)) - R;
+	const uint32_t dead_from = (w == 32 ? 0u : (1u << w)) - Rmax;
 	return delta < dead_from;
 }
 
@@ -205,229 +288,238 @@ __device__ __forceinline__ void wave_hist_add(uint32_t *s_hist, uint32_t d) {
 	}
 }
 
-/* One move: Philox block (id, step, MOVE, 0); direction U[0,5) from word 0
- * (0 stay, 1 +x, 2 -x, 3 +y, 4 -y, torus wrap: the table of move_agent,
- * pp_gpu.cl:780-803); reproduction draw U[0,100) from word 1, kept as one bit
- * (draw < probability of the agent's type) for the action phase. Energy -1;
- * the agent dies below 1 (pphpc.nlogo:66-70, pp_gpu.cl:806-815): key
- * DEAD_KEY, dropped by the first sort pass. Returns the key; *val gets the
- * packed value. */
-__device__ __forceinline__ uint32_t move_agent(uint32_t id, uint32_t step,
-	uint32_t c, int32_t energy, uint32_t wolf, const Params &P, uint32_t *val,
+/* What the move needs of a replica. */
+struct RM {
+	uint32_t cb, C, gx, sheep_prob, wolves_prob, key0, key1;
+	uint32_t step;   /* local iteration the moved agents act in */
+	uint32_t base;   /* batch position of its local agent id 0 */
+	bool active;
+};
+
+__device__ __forceinline__ RM load_rm(const Rep &rp, const RepCtl &rc,
+	uint32_t tick) {
+	RM m;
+	m.cb = rp.cb;
+	m.C = rp.C;
+	m.gx = rp.gx;
+	m.sheep_prob = rp.sheep_prob;
+	m.wolves_prob = rp.wolves_prob;
+	m.key0 = rp.key0;
+	m.key1 = rp.key1;
+	m.step = tick - rp.t0;
+	m.base = rc.base;
+	m.active = rc.status == RS_ACTIVE;
+	return m;
+}
+
+/* One move of a replica's agent with local id `id` at local cell c: Philox
+ * block (id, step, MOVE, 0) under the replica's key; direction U[0,5) from
+ * word 0 (0 stay, 1 +x, 2 -x, 3 +y, 4 -y, torus wrap: the table of
+ * move_agent, pp_gpu.cl:780-803); reproduction draw U[0,100) from word 1,
+ * kept as one bit (draw < probability of the agent's type) for the action
+ * phase. Energy -1; the agent dies below 1 (pphpc.nlogo:66-70,
+ * pp_gpu.cl:806-815): key DEAD_KEY, dropped by the first sort pass. Returns
+ * the global key (cb + new local cell); *val gets the packed value. */
+__device__ __forceinline__ uint32_t move_agent(uint32_t id, uint32_t c,
+	int32_t energy, uint32_t wolf, const RM &m, uint32_t *val,
 	bool &overflow) {
-	const uint4 r = pp_philox(id, step, PP_STREAM_MOVE, 0, P.key0, P.key1);
+	const uint4 r = pp_philox(id, m.step, PP_STREAM_MOVE, 0, m.key0, m.key1);
 	const uint32_t dir = pp_bounded(r.x, 5);
 	const uint32_t dice = pp_bounded(r.y, 100);
 	const int32_t e = energy - 1;
 	*val = 0;
 	if (e < 1) return DEAD_KEY;
-	const uint32_t x = c % P.gx;
-	const uint32_t wrap_y = P.C - P.gx;
+	const uint32_t x = c % m.gx;
+	const uint32_t wrap_y = m.C - m.gx;
 	uint32_t nc = c;
-	if (dir == 1) nc = (x == P.gx - 1) ? c - (P.gx - 1) : c + 1;
-	else if (dir == 2) nc = (x == 0) ? c + (P.gx - 1) : c - 1;
-	else if (dir == 3) nc = (c >= wrap_y) ? c - wrap_y : c + P.gx;
-	else if (dir == 4) nc = (c < P.gx) ? c + wrap_y : c - P.gx;
-	const uint32_t prob = wolf ? P.wolves_prob : P.sheep_prob;
+	if (dir == 1) nc = (x == m.gx - 1) ? c - (m.gx - 1) : c + 1;
+	else if (dir == 2) nc = (x == 0) ? c + (m.gx - 1) : c - 1;
+	else if (dir == 3) nc = (c >= wrap_y) ? c - wrap_y : c + m.gx;
+	else if (dir == 4) nc = (c < m.gx) ? c + wrap_y : c - m.gx;
+	const uint32_t prob = wolf ? m.wolves_prob : m.sheep_prob;
 	overflow |= e > (int32_t)V_ENERGY;
 	*val = ((uint32_t)e & V_ENERGY) | (wolf ? V_WOLF : 0u)
 		| (dice < prob ? V_REPRO : 0u);
-	return nc;
+	return m.cb + nc;
 }
 
 /* ------------------------------------------------------------------- init */
 
-/* Zero the iteration counters, the grass ring and the pass-0 histograms. */
-__global__ __launch_bounds__(WG) void k_init_reset(Ctl *ctl,
-	unsigned long long *ring, uint32_t ring_n, uint32_t *hist,
+/* Empties the batch: no agents, no replicas, tick 0, the pass-0 histograms
+ * zeroed. */
+__global__ __launch_bounds__(WG) void k_batch_reset(Ctl *ctl, uint32_t *hist,
 	uint32_t hist_words) {
 	const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
-	for (uint32_t i = gtid; i < ring_n; i += gridDim.x * WG) ring[i] = 0;
 	for (uint32_t i = gtid; i < hist_words; i += gridDim.x * WG) hist[i] = 0;
-	if (gtid == 0) {
-		ctl->n = 0;
-		ctl->step = 0;
-		ctl->error = 0;
-		ctl->err_step = 0xFFFFFFFFu;
-		ctl->live[0] = 0;
-		ctl->live[1] = 0;
-		ctl->out_n = 0;
-		ctl->dead_cells = 0;
-		ctl->cd_sum = 0;
-		ctl->eaten = 0;
-		for (int i = 0; i < 4; i++) ctl->acc[i] = 0;
-		ctl->occupied = 0;
-		ctl->agent_steps = 0;
-		ctl->sum_live = 0;
-		ctl->sum_out = 0;
-		ctl->sum_occ = 0;
-		ctl->sum_eaten = 0;
-	}
+	if (gtid == 0) memset(ctl, 0, sizeof(Ctl));
 }
 
-/* Cells: grass alive with probability 1/2 (draw word 0), else countdown
- * 1 + U[0, R) (word 1), Philox counter (cell, 0, INIT_CELL, 0). Follows
- * init_cell (pp_gpu.cl:368-398) and pphpc.nlogo:27-35. The grass of a cell is
- * stored as its regrow step: countdown(step s) = max(0, regrow - s). */
+/* Clears one replica slot's control words and grass ring. */
+__global__ __launch_bounds__(WG) void k_rep_reset(Tables T, uint32_t slot) {
+	const Rep &rp = T.rep[slot];
+	for (uint32_t i = threadIdx.x; i <= rp.R; i += WG) rp.ring[i] = 0;
+	if (threadIdx.x == 0) memset(&T.rctl[slot], 0, sizeof(RepCtl));
+}
+
+/* Cells of one replica: grass alive with probability 1/2 (draw word 0), else
+ * countdown 1 + U[0, R) (word 1), Philox counter (local cell, 0, INIT_CELL,
+ * 0). Follows init_cell (pp_gpu.cl:368-398) and pphpc.nlogo:27-35. A cell with
+ * countdown g at local iteration 0 is stored as regrow tick t0 + g; dead
+ * cells are counted in ring slot (t0 + g) mod (R + 1). */
 template <typename GT>
-__global__ __launch_bounds__(WG) void k_init_cells(GT *grass_at,
-	unsigned long long *ring, Ctl *ctl, Params P) {
+__global__ __launch_bounds__(WG) void k_init_cells(GT *grass_at, Tables T,
+	uint32_t slot) {
 	__shared__ uint32_t s_hist[4096];
 	__shared__ unsigned long long s_red[NWAVE][2];
-	const bool lds_hist = P.R + 1 <= 4096;
+	const Rep rp = T.rep[slot];
+	const uint32_t M = rp.R + 1;
+	const bool lds_hist = M <= 4096;
 	if (lds_hist)
-		for (uint32_t i = threadIdx.x; i <= P.R; i += WG) s_hist[i] = 0;
+		for (uint32_t i = threadIdx.x; i < M; i += WG) s_hist[i] = 0;
 	__syncthreads();
 	unsigned long long v[2] = {0, 0};
-	for (uint32_t c = blockIdx.x * WG + threadIdx.x; c < P.C;
+	for (uint32_t c = blockIdx.x * WG + threadIdx.x; c < rp.C;
 		c += gridDim.x * WG) {
-		const uint4 r = pp_philox(c, 0, PP_STREAM_INIT_CELL, 0, P.key0, P.key1);
+		const uint4 r = pp_philox(c, 0, PP_STREAM_INIT_CELL, 0, rp.key0, rp.key1);
 		uint32_t g = 0;
-		if (pp_bounded(r.x, 2) != 1) g = 1 + pp_bounded(r.y, P.R);
-		grass_at[c] = (GT)g;
+		if (pp_bounded(r.x, 2) != 1) g = 1 + pp_bounded(r.y, rp.R);
+		grass_at[rp.cb + c] = (GT)(rp.t0 + g);
 		if (g) {
 			v[0] += 1;
 			v[1] += g;
-			if (lds_hist) atomicAdd(&s_hist[g], 1u);
-			else atomicAdd(&ring[g], 1ull);
+			const uint32_t k = (rp.t0 + g) % M;
+			if (lds_hist) atomicAdd(&s_hist[k], 1u);
+			else atomicAdd(&rp.ring[k], 1ull);
 		}
 	}
-	unsigned long long *dst[2] = {&ctl->dead_cells, &ctl->cd_sum};
-	wg_add<2>(v, s_red, dst);
-	if (lds_hist) {
-		__syncthreads();
-		for (uint32_t i = threadIdx.x; i <= P.R; i += WG)
-			if (s_hist[i]) atomicAdd(&ring[i], (unsigned long long)s_hist[i]);
-	}
+	wg_add<2>(v, s_red, &T.rctl[slot].dead_cells);
+	if (lds_hist)
+		for (uint32_t i = threadIdx.x; i < M; i += WG)
+			if (s_hist[i]) atomicAdd(&rp.ring[i], (unsigned long long)s_hist[i]);
 }
 
-/* Agents: ids 0..init_sheep-1 are sheep, the rest wolves; x = U[0,GX) (word 0),
+/* Agents of one replica at batch positions [pos, pos + n0): local ids
+ * 0..init_sheep-1 are sheep, the rest wolves; x = U[0,GX) (word 0),
  * y = U[0,GY) (word 1), energy 1 + U[0, 2*gain) (word 2), Philox counter
- * (id, 0, INIT_AGENT, 0). Follows init_agent (pp_gpu.cl:406-439) and
+ * (local id, 0, INIT_AGENT, 0). Follows init_agent (pp_gpu.cl:406-439) and
  * pphpc.nlogo:38-53. */
 __global__ __launch_bounds__(WG) void k_init_agents(uint32_t *cell,
-	int32_t *energy, uint8_t *type, Ctl *ctl, Params P) {
+	int32_t *energy, uint8_t *type, Tables T, uint32_t slot, uint32_t pos) {
 	__shared__ unsigned long long s_red[NWAVE][4];
+	const Rep rp = T.rep[slot];
 	unsigned long long v[4] = {0, 0, 0, 0};
-	for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < P.n0;
+	for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < rp.n0;
 		i += gridDim.x * WG) {
-		const uint4 r = pp_philox(i, 0, PP_STREAM_INIT_AGENT, 0, P.key0, P.key1);
-		const bool wolf = i >= P.init_sheep;
-		const uint32_t gain = wolf ? P.wolves_gain : P.sheep_gain;
-		const uint32_t x = pp_bounded(r.x, P.gx);
-		const uint32_t y = pp_bounded(r.y, P.gy);
+		const uint4 r = pp_philox(i, 0, PP_STREAM_INIT_AGENT, 0, rp.key0, rp.key1);
+		const bool wolf = i >= rp.init_sheep;
+		const uint32_t gain = wolf ? rp.wolves_gain : rp.sheep_gain;
+		const uint32_t x = pp_bounded(r.x, rp.gx);
+		const uint32_t y = pp_bounded(r.y, rp.gy);
 		const int32_t e = 1 + (int32_t)pp_bounded(r.z, 2 * gain);
-		cell[i] = y * P.gx + x;
-		energy[i] = e;
-		type[i] = wolf ? PP_WOLF : PP_SHEEP;
+		cell[pos + i] = rp.cb + y * rp.gx + x;
+		energy[pos + i] = e;
+		type[pos + i] = wolf ? PP_WOLF : PP_SHEEP;
 		if (wolf) { v[1] += 1; v[3] += (unsigned long long)e; }
 		else { v[0] += 1; v[2] += (unsigned long long)e; }
 	}
-	unsigned long long *dst[4] = {&ctl->acc[0], &ctl->acc[1], &ctl->acc[2],
-		&ctl->acc[3]};
-	wg_add<4>(v, s_red, dst);
+	wg_add<4>(v, s_red, &T.rctl[slot].acc[0]);
 }
 
-__device__ void write_row(Ctl *ctl, pp_stats64_t *row, const Params &P) {
-	row->sheep = ctl->acc[0];
-	row->wolves = ctl->acc[1];
-	row->grass = (unsigned long long)P.C - ctl->dead_cells;
-	row->sheep_en = ctl->acc[2];
-	row->wolves_en = ctl->acc[3];
-	row->grass_en = ctl->cd_sum;
+__device__ void write_row(const RepCtl &rc, const Rep &rp, pp_stats64_t *row) {
+	row->sheep = rc.acc[AC_SHEEP];
+	row->wolves = rc.acc[AC_WOLVES];
+	row->grass = (unsigned long long)rp.C - rc.dead_cells;
+	row->sheep_en = rc.acc[AC_SHEEP_EN];
+	row->wolves_en = rc.acc[AC_WOLVES_EN];
+	row->grass_en = rc.cd_sum;
 	row->errors = 0;
 }
 
-__global__ void k_init_done(Ctl *ctl, pp_stats64_t *stats, Params P) {
+/* Row 0 of one replica placed at batch position pos = ctl->n; its agents join
+ * the batch state. */
+__global__ void k_rep_init_done(Ctl *ctl, Tables T, uint32_t slot,
+	uint32_t pos) {
 	if (threadIdx.x != 0 || blockIdx.x != 0) return;
-	if (P.n0 > P.cap) {
-		set_error(ctl, PP_OUT_OF_RESOURCES, 0);
-		return;
-	}
-	write_row(ctl, &stats[0], P);
-	for (int i = 0; i < 4; i++) ctl->acc[i] = 0;
-	ctl->n = P.n0;
+	const Rep &rp = T.rep[slot];
+	RepCtl &rc = T.rctl[slot];
+	write_row(rc, rp, &rp.stats[0]);
+	for (int i = 0; i < NACC; i++) rc.acc[i] = 0;
+	rc.n = rp.iters ? rp.n0 : 0;
+	rc.base = pos;
+	rc.step = 0;
+	rc.status = rp.iters ? RS_ACTIVE : RS_DONE;
+	ctl->n = pos + rp.n0;
 }
 
-/* Move of iteration `step` for a state stored as SoA in id order (the state
- * right after init): pairs for the sort plus the pass-0 digit histogram of
- * every sort tile. Each thread moves M_IPT consecutive agents with 16-byte
- * loads and stores. */
+/* Move for one replica's freshly initialised agents at batch positions
+ * [pos, pos + n0) (SoA in local id order): their pairs, the pass-0 digit
+ * histogram of every sort tile, and the live count of this tick. Each thread
+ * moves M_IPT consecutive agents. */
 __global__ __launch_bounds__(WG) void k_move0(const uint32_t *__restrict__ cell,
 	const int32_t *__restrict__ energy, const uint8_t *__restrict__ type,
 	uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-	uint32_t *__restrict__ hist, Ctl *ctl, Params P) {
-	__shared__ uint32_t s_hist[RADIX_MAX];
-	__shared__ unsigned long long s_red[NWAVE][1];
+	uint32_t *__restrict__ hist, Ctl *ctl, Tables T, uint32_t slot,
+	uint32_t pos, Params P) {
+	__shared__ uint32_t s_hist[2 * RADIX_MAX];
+	__shared__ unsigned long long s_red[NWAVE];
 	if (ctl->error) return;
-	const uint32_t n = ctl->n, step = ctl->step;
+	const uint32_t tick = ctl->tick;
+	const Rep &rp = T.rep[slot];
+	const RM m = load_rm(rp, T.rctl[slot], tick);
+	const uint32_t n = rp.n0;
 	const int tid = threadIdx.x;
 	const uint32_t radix = 1u << P.digit_bits, dmask = radix - 1u;
-	unsigned long long live[1] = {0};
+	unsigned long long live = 0;
 	bool overflow = false;
-	constexpr uint32_t CHUNK = WG * M_IPT;   /* divides every sort tile */
+	constexpr uint32_t CHUNK = WG * M_IPT;
 	for (uint32_t cb = blockIdx.x * CHUNK; cb < n; cb += gridDim.x * CHUNK) {
-		for (uint32_t i = tid; i < radix; i += WG) s_hist[i] = 0;
+		for (uint32_t i = tid; i < 2 * radix; i += WG) s_hist[i] = 0;
 		__syncthreads();
-		const uint32_t i0 = cb + tid * M_IPT;
-		uint32_t c[M_IPT], t[M_IPT];
-		int32_t e[M_IPT];
-		if (i0 + M_IPT <= n) {
-			const uint4 cv = *reinterpret_cast<const uint4 *>(cell + i0);
-			const int4 ev = *reinterpret_cast<const int4 *>(energy + i0);
-			const uint32_t tv = *reinterpret_cast<const uint32_t *>(type + i0);
-			c[0] = cv.x; c[1] = cv.y; c[2] = cv.z; c[3] = cv.w;
-			e[0] = ev.x; e[1] = ev.y; e[2] = ev.z; e[3] = ev.w;
-#pragma unroll
-			for (int j = 0; j < M_IPT; j++) t[j] = (tv >> (8 * j)) & 0xFFu;
-		} else {
-#pragma unroll
-			for (int j = 0; j < M_IPT; j++) {
-				const bool in = i0 + j < n;
-				c[j] = in ? cell[i0 + j] : 0;
-				e[j] = in ? energy[i0 + j] : 0;
-				t[j] = in ? type[i0 + j] : 0;
-			}
-		}
-		uint32_t k[M_IPT], v[M_IPT];
+		const uint32_t l0 = cb + tid * M_IPT;
+		uint32_t k[M_IPT];
 #pragma unroll
 		for (int j = 0; j < M_IPT; j++) {
-			k[j] = move_agent(i0 + j, step, c[j], e[j], t[j], P, &v[j], overflow);
-			if (i0 + j >= n) k[j] = DEAD_KEY;
-			live[0] += k[j] != DEAD_KEY;
+			const uint32_t l = l0 + j;
+			k[j] = DEAD_KEY;
+			if (l < n) {
+				const uint32_t g = pos + l;
+				uint32_t v = 0;
+				if (m.active)
+					k[j] = move_agent(l, cell[g] - m.cb, energy[g], type[g], m, &v,
+						overflow);
+				keys[g] = k[j];
+				vals[g] = v;
+				live += k[j] != DEAD_KEY;
+			}
 		}
-		if (i0 + M_IPT <= n) {
-			*reinterpret_cast<uint4 *>(keys + i0) = make_uint4(k[0], k[1], k[2], k[3]);
-			*reinterpret_cast<uint4 *>(vals + i0) = make_uint4(v[0], v[1], v[2], v[3]);
-		} else {
+		/* positions [pos + cb, pos + cb + CHUNK) span at most two sort tiles
+		 * (CHUNK <= sort_tile) */
+		const uint32_t t0 = (pos + cb) / P.sort_tile;
 #pragma unroll
-			for (int j = 0; j < M_IPT; j++)
-				if (i0 + j < n) {
-					keys[i0 + j] = k[j];
-					vals[i0 + j] = v[j];
-				}
+		for (int j = 0; j < M_IPT; j++) {
+			uint32_t d = NO_DIGIT;
+			if (k[j] != DEAD_KEY)
+				d = ((pos + l0 + j) / P.sort_tile - t0) * radix + (k[j] & dmask);
+			wave_hist_add(s_hist, d);
 		}
-#pragma unroll
-		for (int j = 0; j < M_IPT; j++)
-			wave_hist_add(s_hist, k[j] == DEAD_KEY ? NO_DIGIT : (k[j] & dmask));
 		__syncthreads();
-		const uint32_t tile = cb / P.sort_tile;
-		for (uint32_t d = tid; d < radix; d += WG)
-			if (s_hist[d]) atomicAdd(&hist[(size_t)d * P.hist_cols + tile], s_hist[d]);
+		for (uint32_t i = tid; i < 2 * radix; i += WG)
+			if (s_hist[i])
+				atomicAdd(&hist[(size_t)(i % radix) * P.hist_cols + t0 + i / radix],
+					s_hist[i]);
 		__syncthreads();
 	}
-	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, step);
+	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, (int32_t)slot);
 	const int lane = tid & (WAVE - 1), w = tid / WAVE;
-	const unsigned long long tsum = wave_sum(live[0]);
-	if (lane == 0) s_red[w][0] = tsum;
+	const unsigned long long tsum = wave_sum(live);
+	if (lane == 0) s_red[w] = tsum;
 	__syncthreads();
 	if (tid == 0) {
 		unsigned long long sum = 0;
-		for (int i = 0; i < NWAVE; i++) sum += s_red[i][0];
-		if (sum) atomicAdd(&ctl->live[step & 1], (uint32_t)sum);
+		for (int i = 0; i < NWAVE; i++) sum += s_red[i];
+		if (sum) atomicAdd(&ctl->live[tick & 1], (uint32_t)sum);
 	}
 }
-
 /* ------------------------------------------------------------------- sort */
 
 /* Radix sort of the (cell, value) pairs by cell, least significant digit
@@ -440,9 +532,8 @@ __global__ __launch_bounds__(WG) void k_move0(const uint32_t *__restrict__ cell,
  * Tile t covers positions [t * sort_tile, (t + 1) * sort_tile). The table
  * hist[d * hist_cols + t] holds tile counts, then tile offsets. */
 
-__device__ __forceinline__ uint32_t pass_len(const Ctl *ctl, uint32_t pass,
-	uint32_t step) {
-	return pass == 0 ? ctl->n : ctl->live[step & 1];
+__device__ __forceinline__ uint32_t pass_len(const Ctl *ctl, uint32_t pass) {
+	return pass == 0 ? ctl->n : ctl->live[ctl->tick & 1];
 }
 
 template <int KPT>
@@ -451,7 +542,7 @@ __global__ __launch_bounds__(WG) void k_sort_up(const uint32_t *__restrict__ kin
 	constexpr uint32_t TILE = WG * KPT;
 	__shared__ uint32_t s_hist[RADIX_MAX];
 	if (ctl->error) return;
-	const uint32_t n = pass_len(ctl, pass, ctl->step);
+	const uint32_t n = pass_len(ctl, pass);
 	const uint32_t ntiles = (n + TILE - 1) / TILE;
 	const uint32_t shift = pass * P.digit_bits;
 	const uint32_t radix = 1u << P.digit_bits, dmask = radix - 1u;
@@ -488,10 +579,11 @@ __global__ __launch_bounds__(WG) void k_sort_scan(uint32_t *__restrict__ hist,
 		for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < act_words;
 			i += gridDim.x * WG)
 			act_cnt[i] = 0;
-		/* Re-stamp the alive cells of slice (s mod REFRESH) with s, which
-		 * keeps their stored regrow step within the modular window; the
-		 * state does not change (alive stays alive, countdown 0). */
-		const uint32_t s = ctl->step;
+		/* Re-stamp the alive cells of slice (t mod REFRESH) with t, which
+		 * keeps their stored regrow tick within the modular window; the
+		 * state does not change (alive stays alive, countdown 0). Cells of
+		 * no replica are stamped too, harmlessly. */
+		const uint32_t s = ctl->tick;
 		const uint32_t per = (P.C + REFRESH - 1) / REFRESH;
 		const uint32_t c0 = (s % REFRESH) * per;
 		const uint32_t c1 = min(c0 + per, P.C);
@@ -501,7 +593,7 @@ __global__ __launch_bounds__(WG) void k_sort_scan(uint32_t *__restrict__ hist,
 			if (grass_alive_at<GT>(g, s, P.R) && g != (GT)s) grass_at[c] = (GT)s;
 		}
 	}
-	const uint32_t n = pass_len(ctl, pass, ctl->step);
+	const uint32_t n = pass_len(ctl, pass);
 	const uint32_t ntiles = (n + P.sort_tile - 1) / P.sort_tile;
 	const uint32_t radix = 1u << P.digit_bits;
 	const uint32_t per = (ntiles + WG - 1) / WG;
@@ -532,7 +624,7 @@ __global__ __launch_bounds__(WG) void k_sort_down(
 	const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
 	uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
 	const uint32_t *__restrict__ hist, const uint32_t *__restrict__ totals,
-	const Ctl *ctl, uint32_t pass, Params P) {
+	Ctl *ctl, uint32_t pass, Params P) {
 	constexpr uint32_t TILE = WG * KPT;
 	constexpr uint32_t R = 1u << RBITS;
 	constexpr uint32_t DPT = R / WG < 1 ? 1 : R / WG;  /* digits per thread */
@@ -545,12 +637,13 @@ __global__ __launch_bounds__(WG) void k_sort_down(
 	__shared__ uint32_t s_tmp[NWAVE];
 
 	if (ctl->error) return;
-	const uint32_t n = pass_len(ctl, pass, ctl->step);
+	const uint32_t n = pass_len(ctl, pass);
 	const uint32_t ntiles = (n + TILE - 1) / TILE;
 	const uint32_t shift = pass * P.digit_bits;
 	const uint32_t radix = 1u << P.digit_bits, dmask = radix - 1u;
 	const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
 	const uint64_t lt_mask = (1ull << lane) - 1ull;
+	bool bad = false;
 
 	/* Bucket starts: exclusive scan over digits of the digit totals. Thread
 	 * tid owns digits [tid*DPT, tid*DPT + DPT). */
@@ -648,11 +741,19 @@ __global__ __launch_bounds__(WG) void k_sort_down(
 			const uint32_t k = s_keys[q];
 			const uint32_t d = (k >> shift) & dmask;
 			const uint32_t dst = (uint32_t)(s_dst[d] + (int32_t)q);
-			kout[dst] = k;
-			vout[dst] = s_vals[q];
+			/* dst < n whenever the histograms describe the input; the guard
+			 * turns an inconsistent table into an error, never a stray
+			 * write */
+			if (dst < n) {
+				kout[dst] = k;
+				vout[dst] = s_vals[q];
+			} else {
+				bad = true;
+			}
 		}
 		__syncthreads();
 	}
+	if (bad) set_error(ctl, PP_LIBRARY_ERROR, ctl->tick, -1);
 }
 
 /* ----------------------------------------------------------------- action */
@@ -702,13 +803,48 @@ __device__ __forceinline__ uint32_t load_tile(const uint32_t *__restrict__ keys,
 	return cnt;
 }
 
+/* What the actions need of a replica. */
+struct RA {
+	uint32_t cb, step, R;
+	uint32_t sheep_gain, wolves_gain, sheep_thr, wolves_thr;
+	uint32_t key0, key1;
+};
+
+__device__ __forceinline__ RA load_ra(const Tables &T, uint32_t k,
+	uint32_t tick) {
+	const Rep &rp = T.rep[T.order[k]];
+	RA a;
+	a.cb = rp.cb;
+	a.step = tick - rp.t0;
+	a.R = rp.R;
+	a.sheep_gain = rp.sheep_gain;
+	a.wolves_gain = rp.wolves_gain;
+	a.sheep_thr = rp.sheep_thr;
+	a.wolves_thr = rp.wolves_thr;
+	a.key0 = rp.key0;
+	a.key1 = rp.key1;
+	return a;
+}
+
+/* Adds v to counter idx of the replica with table index k, in an action tile
+ * whose positions belong to several replicas (the first with index k_first):
+ * LDS slots for the first MIX_SLOTS of them, global atomics beyond. */
+__device__ __forceinline__ void mix_add(unsigned long long (*s_mix)[NACC],
+	const Tables &T, uint32_t k, uint32_t k_first, int idx,
+	unsigned long long v) {
+	if (!v) return;
+	const uint32_t d = k - k_first;
+	if (d < (uint32_t)MIX_SLOTS) atomicAdd(&s_mix[d][idx], v);
+	else atomicAdd(&T.rctl[T.order[k]].acc[idx], v);
+}
+
 /* Agent actions for the agents sorted by cell (ascending agent id within a
  * cell). A work-group stages A_TILE sorted positions in LDS; the thread owning
  * a position that starts a run of equal cells (the segment that find_cell_idx,
  * pp_gpu.cl:834-867, delimits) acts for the whole cell in sequence:
  *   1. Fisher-Yates permutation of the segment, the draw for position i being
- *      word m%4 of Philox block (cell, step, SHUFFLE, m/4), m = k-1-i (random
- *      acting order: Cell.java:246, pp_cpu.cl:236-259);
+ *      word m%4 of Philox block (local cell, local iteration, SHUFFLE, m/4),
+ *      m = k-1-i (random acting order: Cell.java:246, pp_cpu.cl:236-259);
  *   2. in that order, each agent not yet eaten: a sheep eats the grass if the
  *      grass is alive (countdown 0 after this iteration's regrowth) and gains
  *      SHEEP_GAIN (Sheep.java:66-83); a wolf eats the first sheep of the
@@ -720,37 +856,58 @@ __device__ __forceinline__ uint32_t load_tile(const uint32_t *__restrict__ keys,
  * The permuted, updated values are written back in place, so the next state
  * is the sorted positions in order, each emitting its survivor then its
  * newborn; act_cnt[t] (zeroed by the pass-0 scan) receives the number of
- * agents the positions of tile t emit. Counts and energy sums of the new state
- * are reduced here. This kernel also clears the pass-0 histogram table that
- * k_emit_move fills for the next iteration. */
+ * agents the positions of tile t emit. Per replica, the counts and energy
+ * sums of the new state, the grass eaten, the occupied cells and the outputs
+ * are summed into its control words: in registers over the tiles of one
+ * replica, through LDS slots in the few tiles shared by several. This kernel
+ * also clears the pass-0 histogram table that k_emit_move fills for the next
+ * iteration. */
 template <typename GT>
 __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 	uint32_t *vals, GT *grass_at, uint32_t *act_cnt,
-	uint32_t *__restrict__ hist, uint32_t hist_words, Ctl *ctl, Params P) {
+	uint32_t *__restrict__ hist, uint32_t hist_words, Ctl *ctl, Tables T,
+	Params P) {
 	__shared__ uint32_t s_key[A_TILE + 1];
 	__shared__ uint32_t s_val[A_TILE];
 	__shared__ uint32_t s_slow_lp[A_TILE];
 	__shared__ GT s_slow_g[A_TILE];
 	__shared__ uint32_t s_nslow;
 	__shared__ uint32_t s_tmp[NWAVE];
-	__shared__ unsigned long long s_red[NWAVE][6];
+	__shared__ unsigned long long s_red[NWAVE][NACC];
+	__shared__ unsigned long long s_mix[MIX_SLOTS][NACC];
 
 	if (ctl->error) return;
 	const int tid = threadIdx.x;
 	for (uint32_t i = blockIdx.x * WG + tid; i < hist_words; i += gridDim.x * WG)
 		hist[i] = 0;
-	const uint32_t step = ctl->step;
-	const uint32_t L = ctl->live[step & 1];
+	const uint32_t tick = ctl->tick, nrep = ctl->nrep;
+	const uint32_t L = ctl->live[tick & 1];
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
-	const uint32_t alive_at = step + 1;
-	const GT eaten_until = (GT)(step + 1 + P.R);
-	unsigned long long acc[6] = {0, 0, 0, 0, 0, 0};
+	const uint32_t alive_at = tick + 1;
+	unsigned long long acc[NACC] = {0, 0, 0, 0, 0, 0, 0};
+	uint32_t acc_k = NO_REP;   /* replica the registers sum for */
 	bool overflow = false;
+	/* A batch of one replica (every plain simulation) needs no lookups. */
+	RA ra_one{};
+	if (nrep == 1 && ntiles > blockIdx.x) ra_one = load_ra(T, 0, tick);
 
 	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 		const uint32_t base = tile * A_TILE;
 		const uint32_t cnt = load_tile(keys, vals, s_key, s_val, base, L);
 		const SegVals sv{s_val, vals + base};
+		/* Replicas of this tile: keys ascend, so they are the table entries
+		 * k_first..k_last. The branch below is uniform over the group. */
+		const uint32_t k_first = rep_index(T, nrep,
+			__builtin_amdgcn_readfirstlane(s_key[1]));
+		const uint32_t k_last = rep_index(T, nrep,
+			__builtin_amdgcn_readfirstlane(s_key[cnt]));
+		const bool mixed = k_first != k_last;
+		if (!mixed && k_first != acc_k) {
+			if (acc_k != NO_REP)
+				wg_add<NACC>(acc, s_red, &T.rctl[T.order[acc_k]].acc[0]);
+			acc_k = k_first;
+		}
+		const RA ra_t = nrep == 1 ? ra_one : load_ra(T, k_first, tick);
 		/* Heads of this thread's positions; their grass loads are issued
 		 * together before any segment is processed. */
 		uint32_t cj[A_IPT];
@@ -769,6 +926,8 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 		 * position of every lane. Segments are independent: the order of the
 		 * list does not change any result. */
 		if (tid == 0) s_nslow = 0;
+		if (mixed)
+			for (int i = tid; i < MIX_SLOTS * NACC; i += WG) (&s_mix[0][0])[i] = 0;
 		__syncthreads();
 		uint32_t nout = 0;
 #pragma unroll
@@ -776,19 +935,22 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 			if (!hj[j]) continue;
 			const uint32_t lp = tid * A_IPT + j;
 			const uint32_t c = cj[j];
-			if (lp + 1 < cnt && s_key[2 + lp] != c) {
-				/* A cell with one agent (most of them): nothing to shuffle, a
-				 * lone wolf finds no sheep, a lone agent is never eaten. */
+			if (!mixed && lp + 1 < cnt && s_key[2 + lp] != c) {
+				/* A cell with one agent: nothing to shuffle, a lone wolf finds
+				 * no sheep, a lone agent is never eaten. (In a tile of several
+				 * replicas every cell takes the general path below.) */
+				const RA &ra = ra_t;
 				const uint32_t v = s_val[lp];
 				int32_t e = (int32_t)(v & V_ENERGY);
 				const uint32_t t = (v & V_WOLF) ? 1u : 0u;
-				uint32_t thr = P.wolves_thr;
+				uint32_t thr = ra.wolves_thr;
+				uint32_t ate = 0;
 				if (!t) {
-					thr = P.sheep_thr;
+					thr = ra.sheep_thr;
 					if (grass_alive_at<GT>(gj[j], alive_at, P.R)) {
-						e += (int32_t)P.sheep_gain;
-						grass_at[c] = eaten_until;
-						acc[4] += 1;
+						e += (int32_t)ra.sheep_gain;
+						grass_at[c] = (GT)(tick + 1 + ra.R);
+						ate = 1;
 					}
 				}
 				uint32_t nv = v & V_WOLF;
@@ -800,11 +962,19 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 				}
 				overflow |= e > (int32_t)V_ENERGY;
 				s_val[lp] = nv | ((uint32_t)e & V_ENERGY);
-				const uint32_t kids = (nv & V_CHILD) ? 1u : 0u;
-				acc[5] += 1;
-				acc[t] += 1 + kids;
-				acc[2 + t] += (uint32_t)e + ce;
-				nout += 1 + kids;
+				const uint32_t outs = (nv & V_CHILD) ? 2u : 1u;
+				const unsigned long long en = (uint32_t)e + ce;
+				if (t) {
+					acc[AC_WOLVES] += outs;
+					acc[AC_WOLVES_EN] += en;
+				} else {
+					acc[AC_SHEEP] += outs;
+					acc[AC_SHEEP_EN] += en;
+				}
+				acc[AC_EATEN] += ate;
+				acc[AC_OCC] += 1;
+				acc[AC_OUT] += outs;
+				nout += outs;
 				continue;
 			}
 			const uint32_t slot = atomicAdd(&s_nslow, 1u);
@@ -818,15 +988,16 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 			const uint32_t lp = s_slow_lp[si];
 			const uint32_t c = s_key[1 + lp];
 			const GT gslow = s_slow_g[si];
+			const uint32_t kk = mixed ? rep_index(T, nrep, c) : k_first;
+			const RA ra = mixed ? load_ra(T, kk, tick) : ra_t;
 			const uint32_t end = seg_end(s_key, keys, lp, cnt, base, L, c);
 			const uint32_t k = end - lp;
-			acc[5] += 1;
 			if (k > 1) {
 				uint4 r = make_uint4(0, 0, 0, 0);
 				for (uint32_t i = k - 1, m = 0; i > 0; i--, m++) {
 					if ((m & 3u) == 0)
-						r = pp_philox(c, step, PP_STREAM_SHUFFLE, m >> 2, P.key0,
-							P.key1);
+						r = pp_philox(c - ra.cb, ra.step, PP_STREAM_SHUFFLE, m >> 2,
+							ra.key0, ra.key1);
 					const uint32_t word = (m & 3u) == 0 ? r.x
 						: (m & 3u) == 1 ? r.y : (m & 3u) == 2 ? r.z : r.w;
 					const uint32_t jj = pp_bounded(word, i + 1);
@@ -848,19 +1019,19 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 					if (grass) {
 						grass = false;
 						ate = true;
-						e += (int32_t)P.sheep_gain;
+						e += (int32_t)ra.sheep_gain;
 					}
-					thr = P.sheep_thr;
+					thr = ra.sheep_thr;
 				} else {
 					for (uint32_t o = lp; o < end; o++) {
 						const uint32_t u = sv.get(o);
 						if (!(u & (V_WOLF | V_KILLED))) {
 							sv.set(o, u | V_KILLED);
-							e += (int32_t)P.wolves_gain;
+							e += (int32_t)ra.wolves_gain;
 							break;
 						}
 					}
-					thr = P.wolves_thr;
+					thr = ra.wolves_thr;
 				}
 				uint32_t nv = v & V_WOLF;
 				if (e > (int32_t)thr && (v & V_REPRO)) {
@@ -870,25 +1041,35 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 				overflow |= e > (int32_t)V_ENERGY;
 				sv.set(q, nv | ((uint32_t)e & V_ENERGY));
 			}
-			if (ate) {
-				grass_at[c] = eaten_until;
-				acc[4] += 1;
-			}
+			if (ate) grass_at[c] = (GT)(tick + 1 + ra.R);
+			unsigned long long sc[NACC] = {0, 0, 0, 0, 0, 0, 0};
+			sc[AC_EATEN] = ate ? 1 : 0;
+			sc[AC_OCC] = 1;
 			for (uint32_t q = lp; q < end; q++) {
 				const uint32_t v = sv.get(q);
 				const uint32_t e = v & V_ENERGY;
 				const uint32_t t = (v & V_WOLF) ? 1u : 0u;
 				const uint32_t outs = ((v & V_KILLED) == 0) + ((v & V_CHILD) != 0);
-				if (!(v & V_KILLED)) {
-					acc[t] += 1;
-					acc[2 + t] += e;
+				const uint32_t na = outs;
+				const uint32_t ea = ((v & V_KILLED) ? 0u : e)
+					+ ((v & V_CHILD) ? ((v & V_ODD) ? e - 1 : e) : 0u);
+				if (t) {
+					sc[AC_WOLVES] += na;
+					sc[AC_WOLVES_EN] += ea;
+				} else {
+					sc[AC_SHEEP] += na;
+					sc[AC_SHEEP_EN] += ea;
 				}
-				if (v & V_CHILD) {
-					acc[t] += 1;
-					acc[2 + t] += (v & V_ODD) ? e - 1 : e;
-				}
+				sc[AC_OUT] += outs;
 				if (q < (uint32_t)A_TILE) nout += outs;
 				else if (outs) atomicAdd(&act_cnt[tile + q / A_TILE], outs);
+			}
+			if (!mixed) {
+#pragma unroll
+				for (int i = 0; i < NACC; i++) acc[i] += sc[i];
+			} else {
+#pragma unroll
+				for (int i = 0; i < NACC; i++) mix_add(s_mix, T, kk, k_first, i, sc[i]);
 			}
 		}
 		__syncthreads();
@@ -899,33 +1080,43 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 		uint32_t total;
 		(void)wg_excl_scan<WG>(nout, s_tmp, total);
 		if (tid == 0 && total) atomicAdd(&act_cnt[tile], total);
+		if (mixed) {
+			const uint32_t nk = min(k_last - k_first + 1, (uint32_t)MIX_SLOTS);
+			for (uint32_t i = tid; i < nk * NACC; i += WG) {
+				const unsigned long long v = (&s_mix[0][0])[i];
+				if (v)
+					atomicAdd(&T.rctl[T.order[k_first + i / NACC]].acc[i % NACC], v);
+			}
+			__syncthreads();
+		}
 	}
-	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, step);
-	unsigned long long *dst[6] = {&ctl->acc[0], &ctl->acc[1], &ctl->acc[2],
-		&ctl->acc[3], &ctl->eaten, &ctl->occupied};
-	wg_add<6>(acc, s_red, dst);
+	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
+	if (acc_k != NO_REP) wg_add<NACC>(acc, s_red, &T.rctl[T.order[acc_k]].acc[0]);
 }
 
 /* One work-group: exclusive scan of the action tiles' output counts (the
- * output offsets), then the end of iteration s -> s+1. Grass statistics come
- * from the regrow-step encoding without touching the grid: with D dead cells
- * and countdown sum S at state s, every dead countdown drops by one (S -= D),
- * the cells whose regrow step is s+1 come alive (ring slot (s+1) mod (R+1)),
- * and the E cells eaten this iteration restart at R (slot s mod (R+1)):
- * exactly the grass kernel (pp_gpu.cl:448-476) plus the sums of
- * reduce_grass1/2 (pp_gpu.cl:485-583). Then the statistics row s+1 is
- * written, the capacity checked (pp_gpu.c:847-852) and the per-iteration
- * counters cleared. */
+ * output offsets), then the end of tick t for every replica of the batch
+ * (in table order, so that its agents' batch positions are the prefix of the
+ * outputs of the replicas before it). Grass statistics come from the
+ * regrow-tick encoding without touching the grid: with D dead cells and
+ * countdown sum S, every dead countdown drops by one (S -= D), the cells whose
+ * regrow tick is t+1 come alive (ring slot (t+1) mod (R+1)), and the E cells
+ * eaten at this tick restart at R (slot t mod (R+1)): exactly the grass
+ * kernel (pp_gpu.cl:448-476) plus the sums of reduce_grass1/2
+ * (pp_gpu.cl:485-583). Then the replica's statistics row s+1 is written, its
+ * capacity checked (pp_gpu.c:847-852) and its counters cleared; a replica
+ * that reached its iteration count is done, one above its capacity failed,
+ * and the emit drops the agents of both. The batch fails when its outputs
+ * exceed the agent buffers. */
 constexpr int SCAN_WG = 1024;
 constexpr int SCAN_UNROLL = 8;
 
 __global__ __launch_bounds__(SCAN_WG) void k_act_scan(
-	uint32_t *__restrict__ act_cnt, Ctl *ctl, unsigned long long *ring,
-	pp_stats64_t *stats, Params P) {
+	uint32_t *__restrict__ act_cnt, Ctl *ctl, Tables T, Params P) {
 	__shared__ uint32_t s_tmp[SCAN_WG / WAVE];
 	if (ctl->error) return;
-	const uint32_t step = ctl->step;
-	const uint32_t L = ctl->live[step & 1];
+	const uint32_t tick = ctl->tick;
+	const uint32_t L = ctl->live[tick & 1];
 	const uint32_t nt = (L + A_TILE - 1) / A_TILE;
 	const uint32_t per = (nt + SCAN_WG - 1) / SCAN_WG;
 	const uint32_t t0 = threadIdx.x * per, t1 = min(t0 + per, nt);
@@ -940,6 +1131,10 @@ __global__ __launch_bounds__(SCAN_WG) void k_act_scan(
 	}
 	uint32_t total;
 	uint32_t run = wg_excl_scan<SCAN_WG>(sum, s_tmp, total);
+	if (total > P.cap) {
+		if (threadIdx.x == 0) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
+		return;
+	}
 	for (uint32_t t = t0; t < t1; t += SCAN_UNROLL) {
 		uint32_t v[SCAN_UNROLL];
 #pragma unroll
@@ -951,69 +1146,121 @@ __global__ __launch_bounds__(SCAN_WG) void k_act_scan(
 			run += v[u];
 		}
 	}
-	if (threadIdx.x != 0) return;
-	const uint32_t s = step;
-	if (total > P.cap) {
-		set_error(ctl, PP_OUT_OF_RESOURCES, s);
-		return;
+	/* The replicas, in table order, by the first wave (no barriers). */
+	if (threadIdx.x >= WAVE) return;
+	const int lane = threadIdx.x;
+	const uint32_t nrep = ctl->nrep;
+	uint32_t carry = 0;
+	unsigned long long bs[3] = {0, 0, 0};  /* agent-steps, occupied, eaten */
+	for (uint32_t k0 = 0; k0 < nrep; k0 += WAVE) {
+		const uint32_t k = k0 + lane;
+		uint32_t slot = 0, out = 0;
+		if (k < nrep) {
+			slot = T.order[k];
+			out = (uint32_t)T.rctl[slot].acc[AC_OUT];
+		}
+		uint32_t x = out;
+#pragma unroll
+		for (int o = 1; o < WAVE; o <<= 1) {
+			const uint32_t t = __shfl_up(x, o);
+			if (lane >= o) x += t;
+		}
+		const uint32_t pre = x - out;
+		const uint32_t tot = __shfl(x, WAVE - 1);
+		if (k < nrep) {
+			RepCtl &rc = T.rctl[slot];
+			const Rep &rp = T.rep[slot];
+			rc.base = carry + pre;
+			if (rc.status == RS_ACTIVE) {
+				const uint32_t s = tick - rp.t0;
+				if (out > rp.cap) {
+					rc.status = RS_FAILED;
+					rc.error = PP_OUT_OF_RESOURCES;
+					rc.err_step = s;
+					rc.n = 0;
+				} else {
+					const uint32_t M = rp.R + 1;
+					unsigned long long D = rc.dead_cells, S = rc.cd_sum;
+					S -= D;
+					const uint32_t expire = (tick + 1) % M;
+					D -= rp.ring[expire];
+					rp.ring[expire] = 0;
+					const unsigned long long E = rc.acc[AC_EATEN];
+					rp.ring[tick % M] += E;
+					D += E;
+					S += E * (unsigned long long)rp.R;
+					rc.dead_cells = D;
+					rc.cd_sum = S;
+					write_row(rc, rp, &rp.stats[(s + 1) % rp.stats_rows]);
+					bs[0] += rc.n;
+					bs[1] += rc.acc[AC_OCC];
+					bs[2] += E;
+					rc.agent_steps += rc.n;
+					rc.step = s + 1;
+					rc.n = out;
+					if (s + 1 >= rp.iters) {
+						rc.status = RS_DONE;
+						rc.n = 0;
+					}
+				}
+			}
+#pragma unroll
+			for (int i = 0; i < NACC; i++) rc.acc[i] = 0;
+		}
+		carry += tot;
 	}
-	const uint32_t M = P.R + 1;
-	unsigned long long D = ctl->dead_cells, S = ctl->cd_sum;
-	S -= D;
-	const uint32_t expire = (s + 1) % M;
-	D -= ring[expire];
-	ring[expire] = 0;
-	const unsigned long long E = ctl->eaten;
-	ring[s % M] += E;
-	D += E;
-	S += E * (unsigned long long)P.R;
-	ctl->dead_cells = D;
-	ctl->cd_sum = S;
-	write_row(ctl, &stats[(s + 1) % STATS_RING], P);
-	ctl->agent_steps += ctl->n;
+#pragma unroll
+	for (int i = 0; i < 3; i++) bs[i] = wave_sum(bs[i]);
+	if (lane != 0) return;
+	ctl->agent_steps += bs[0];
 	ctl->sum_live += L;
 	ctl->sum_out += total;
-	ctl->sum_occ += ctl->occupied;
-	ctl->sum_eaten += E;
+	ctl->sum_occ += bs[1];
+	ctl->sum_eaten += bs[2];
 	ctl->n = total;
-	ctl->out_n = total;
-	for (int i = 0; i < 4; i++) ctl->acc[i] = 0;
-	ctl->eaten = 0;
-	ctl->occupied = 0;
-	ctl->live[(s + 1) & 1] = 0;
-	ctl->step = s + 1;
+	ctl->live[(tick + 1) & 1] = 0;
+	ctl->tick = tick + 1;
 }
 
 /* Emits the new state and moves it for the next iteration. Sorted position
- * p emits its agent unless eaten, then its newborn if any; the agents' ids in
- * the new state are these outputs in position order (tile offset from
- * k_act_scan plus a work-group scan). Each output is staged in LDS, then
+ * p emits its agent unless eaten, then its newborn if any; the agents' batch
+ * positions in the new state are these outputs in position order (tile
+ * offset from k_act_scan plus a work-group scan), and an agent's local id is
+ * its position minus its replica's base. Each output is staged in LDS, then
  * written with coalesced stores to the SoA state and immediately moved for
- * iteration step (= s+1): its (key, value) pair goes to slot id and its
- * pass-0 digit is counted for the sort tile holding id. The outputs of one
- * action tile (at most 2 x A_TILE) span at most two sort tiles. */
+ * tick t+1: its (key, value) pair goes to its position and its pass-0 digit
+ * is counted for the sort tile holding it. Agents of a replica that is done
+ * or failed get DEAD_KEY. The outputs of one action tile (at most
+ * 2 x A_TILE) span at most two sort tiles. */
 template <int RBITS>
 __global__ __launch_bounds__(WG) void k_emit_move(
 	const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
 	const uint32_t *__restrict__ act_off, uint32_t *__restrict__ ocell,
 	int32_t *__restrict__ oenergy, uint8_t *__restrict__ otype,
 	uint32_t *__restrict__ mkeys, uint32_t *__restrict__ mvals,
-	uint32_t *__restrict__ hist, Ctl *ctl, Params P) {
+	uint32_t *__restrict__ hist, Ctl *ctl, Tables T, Params P) {
 	constexpr uint32_t R = 1u << RBITS;
 	__shared__ uint32_t s_oc[A_STAGE];
 	__shared__ uint32_t s_ov[A_STAGE];
 	__shared__ uint32_t s_h[2 * R];
 	__shared__ uint32_t s_tmp[NWAVE];
-	__shared__ unsigned long long s_red[NWAVE][1];
+	__shared__ unsigned long long s_red[NWAVE];
 
 	if (ctl->error) return;
 	const int tid = threadIdx.x;
-	const uint32_t step = ctl->step;            /* iteration being moved */
-	const uint32_t L = ctl->live[(step - 1) & 1];
+	const uint32_t tick = ctl->tick;            /* tick being moved for */
+	const uint32_t L = ctl->live[(tick - 1) & 1];
+	const uint32_t nrep = ctl->nrep;
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
 	const uint32_t dmask = (1u << P.digit_bits) - 1u;
-	unsigned long long live[1] = {0};
+	unsigned long long live = 0;
 	bool overflow = false;
+	/* A batch of one replica (every plain simulation) needs no lookups. */
+	RM m_one{};
+	if (nrep == 1 && ntiles > blockIdx.x) {
+		const uint32_t s0 = T.order[0];
+		m_one = load_rm(T.rep[s0], T.rctl[s0], tick);
+	}
 
 	for (uint32_t i = tid; i < 2 * R; i += WG) s_h[i] = 0;
 	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -1056,6 +1303,19 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 			}
 		}
 		__syncthreads();
+		/* Replicas of this tile's positions (keys ascend); uniform over the
+		 * group. */
+		bool mixed = false;
+		RM m_t = m_one;
+		if (nrep != 1) {
+			const uint32_t k_first = rep_index(T, nrep,
+				__builtin_amdgcn_readfirstlane(keys[base]));
+			const uint32_t k_last = rep_index(T, nrep,
+				__builtin_amdgcn_readfirstlane(keys[min(base + A_TILE, L) - 1]));
+			mixed = k_first != k_last;
+			const uint32_t s0 = T.order[k_first];
+			m_t = load_rm(T.rep[s0], T.rctl[s0], tick);
+		}
 		const uint32_t to0 = goff / P.sort_tile;
 		for (uint32_t i0 = 0; i0 < total; i0 += WG) {
 			const uint32_t i = i0 + tid;
@@ -1067,13 +1327,19 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 				ocell[id] = cc;
 				oenergy[id] = (int32_t)en;
 				otype[id] = (uint8_t)t;
-				uint32_t mv;
-				const uint32_t mk = move_agent(id, step, cc, (int32_t)en, t, P,
-					&mv, overflow);
+				RM m = m_t;
+				if (mixed) {
+					const uint32_t sl = T.order[rep_index(T, nrep, cc)];
+					m = load_rm(T.rep[sl], T.rctl[sl], tick);
+				}
+				uint32_t mv = 0, mk = DEAD_KEY;
+				if (m.active)
+					mk = move_agent(id - m.base, cc - m.cb, (int32_t)en, t, m, &mv,
+						overflow);
 				mkeys[id] = mk;
 				mvals[id] = mv;
 				if (mk != DEAD_KEY) {
-					live[0] += 1;
+					live += 1;
 					d = (id / P.sort_tile - to0) * R + (mk & dmask);
 				}
 			}
@@ -1089,15 +1355,15 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		}
 		__syncthreads();
 	}
-	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, step);
+	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
 	const int lane = tid & (WAVE - 1), w = tid / WAVE;
-	const unsigned long long tsum = wave_sum(live[0]);
-	if (lane == 0) s_red[w][0] = tsum;
+	const unsigned long long tsum = wave_sum(live);
+	if (lane == 0) s_red[w] = tsum;
 	__syncthreads();
 	if (tid == 0) {
 		unsigned long long sum = 0;
-		for (int i = 0; i < NWAVE; i++) sum += s_red[i][0];
-		if (sum) atomicAdd(&ctl->live[step & 1], (uint32_t)sum);
+		for (int i = 0; i < NWAVE; i++) sum += s_red[i];
+		if (sum) atomicAdd(&ctl->live[tick & 1], (uint32_t)sum);
 	}
 }
 
@@ -1128,51 +1394,165 @@ struct PendingEvent {
 	hipEvent_t start, stop;
 };
 
+/* Launch shape of the sort: keys per thread of a sort tile (tile = 256 x
+ * kpt positions) and the widest digit (passes = ceil(key bits / max_bits),
+ * bits spread evenly over the passes). */
+struct SortShape {
+	int kpt;
+	int max_bits;
+};
+
+/* Host record of a replica. */
+struct Replica {
+	uint32_t id;
+	int slot;              /* device slot, -1 once released */
+	uint32_t cb, C;
+	pp_params_t prm;
+	uint64_t seed;
+	uint32_t iters, reserve, n0;
+	uint32_t status;       /* RS_* */
+	int error;
+	uint32_t err_step;
+	uint32_t rows_have;    /* rows 0 .. rows_have - 1 collected */
+	uint32_t n;            /* agents of its current state */
+	uint64_t agent_steps;
+	int keep;              /* keep every row on the host */
+	std::vector<pp_stats64_t> rows;
+	pp_stats64_t *sink;    /* else rows >= sink_from are written here */
+	uint32_t sink_from;
+};
+
+uint32_t ceil_div(uint64_t a, uint64_t b) {
+	return (uint32_t)((a + b - 1) / b);
+}
+
+uint32_t min_u32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+uint32_t grass_bytes_for(uint32_t R) {
+	return R + REFRESH + 1 <= 256 ? 1 : R + REFRESH + 1 <= 65536 ? 2 : 4;
+}
+
+uint32_t key_bits_for(uint64_t cells) {
+	uint32_t bits = 0;
+	while (bits < 32 && ((cells - 1) >> bits) != 0) bits++;
+	return bits ? bits : 1;
+}
+
+void shape_params(const SortShape &sh, uint32_t key_bits, uint32_t cap,
+	uint32_t *passes, uint32_t *digit_bits, uint32_t *hist_cols) {
+	*passes = (key_bits + sh.max_bits - 1) / sh.max_bits;
+	*digit_bits = (key_bits + *passes - 1) / *passes;
+	*hist_cols = ceil_div(cap, (uint64_t)WG * sh.kpt);
+}
+
+uint64_t hist_bytes_for(const SortShape &sh, uint32_t key_bits, uint32_t cap) {
+	uint32_t passes, bits, cols;
+	shape_params(sh, key_bits, cap, &passes, &bits, &cols);
+	(void)passes;
+	return (4ull << bits) * cols;
+}
+
+/* The default shape: tiles of 4096 keys; up to 22 key bits in passes of at
+ * most 11 bits (1600 x 1600 = 2 x 11), wider keys in passes of at most 8
+ * bits (12800 x 12800 = 4 x 7), where longer runs per bucket keep the
+ * scatter coalesced. */
+SortShape default_shape(uint32_t key_bits) {
+	SortShape sh;
+	sh.kpt = 16;
+	const char *kpt_env = getenv("PPHPC_SORT_KPT");
+	if (kpt_env && atoi(kpt_env) == 8) sh.kpt = 8;
+	sh.max_bits = key_bits <= 22 ? 11 : 8;
+	return sh;
+}
+
+/* Shapes the tuner may try: both tile sizes, both digit limits, as long as
+ * the histogram table stays within max(32 MiB, the default's). */
+std::vector<SortShape> tune_candidates(uint32_t key_bits, uint32_t cap) {
+	const SortShape def = default_shape(key_bits);
+	const uint64_t limit = std::max<uint64_t>(32ull << 20,
+		hist_bytes_for(def, key_bits, cap));
+	std::vector<SortShape> out;
+	out.push_back(def);
+	const int kpts[2] = {16, 8};
+	const int bits[2] = {11, 8};
+	for (int a = 0; a < 2; a++)
+		for (int b = 0; b < 2; b++) {
+			SortShape sh{kpts[a], bits[b]};
+			if (sh.kpt == def.kpt && sh.max_bits == def.max_bits) continue;
+			if (hist_bytes_for(sh, key_bits, cap) <= limit) out.push_back(sh);
+		}
+	return out;
+}
+
 } /* namespace */
 
-struct pp_sim {
-	pp_params_t prm;
-	Params P;
-	uint64_t seed;
-	uint32_t cap;
-	int device;
-	int num_cu;
-	hipStream_t stream;
+struct pp_batch {
+	int device = -1;
+	int num_cu = 256;
+	hipStream_t stream = nullptr;
 
-	uint32_t *cell;
-	int32_t *energy;
-	uint8_t *type;
-	uint32_t *keys[3];
-	uint32_t *vals[3];
-	void *grass_at;
-	uint32_t grass_bytes;     /* 1, 2 or 4 bytes per cell */
-	unsigned long long *ring;
-	uint32_t ring_n;
-	uint32_t *hist;
-	uint32_t hist_words;
-	uint32_t *totals;
-	uint32_t *act_cnt;
-	uint32_t act_words;
-	Ctl *ctl;
-	pp_stats64_t *stats_dev;
-	pp_stats64_t *stats_host; /* pinned, STATS_RING rows */
-	Ctl *ctl_host;            /* pinned mirror */
+	Params P{};
+	uint32_t key_bits = 1;
+	uint32_t max_rep = 0;
+	uint32_t stats_rows = 0;
+	uint32_t ring_stride = 0;
+	uint32_t grass_bytes = 4;
+	SortShape shape{16, 11};
+	int rbits = 11;
+	uint32_t grid_up = 0, grid_down = 0, grid_act = 0, grid_emit = 0;
 
-	int kpt;
-	int rbits;                /* template radix bits of the sort kernels */
-	uint32_t grid_move, grid_up, grid_down, grid_act, grid_emit, grid_cells;
-	uint32_t grid_agents;
+	uint32_t *cell = nullptr;
+	int32_t *energy = nullptr;
+	uint8_t *type = nullptr;
+	uint32_t *keys[3] = {nullptr, nullptr, nullptr};
+	uint32_t *vals[3] = {nullptr, nullptr, nullptr};
+	void *grass_at = nullptr;
+	unsigned long long *ring = nullptr;
+	uint32_t *hist = nullptr;
+	uint64_t hist_alloc_words = 0;
+	uint32_t hist_words = 0;
+	uint32_t *totals = nullptr;
+	uint32_t *act_cnt = nullptr;
+	uint32_t act_words = 0;
+	Ctl *ctl = nullptr;
+	Ctl *ctl_host = nullptr;        /* pinned mirror */
+	int ctl_fresh = 0;              /* mirror matches the device */
+	Rep *rep_dev = nullptr;
+	std::vector<Rep> rep_host;
+	RepCtl *rctl_dev = nullptr;
+	RepCtl *rctl_host = nullptr;    /* pinned mirror */
+	uint32_t *cbs_dev = nullptr;
+	uint32_t *order_dev = nullptr;
+	pp_stats64_t *stats_dev = nullptr;
+	pp_stats64_t *stats_host = nullptr; /* pinned, same layout */
 
-	hipGraphExec_t graph_exec;
-	int graph_steps;
+	hipGraphExec_t graph_exec = nullptr;
+	int graph_steps = 0;
 
-	int profiling;
+	int profiling = 0;
 	std::vector<PendingEvent> pending;
 	std::vector<hipEvent_t> free_events;
-	uint64_t prof_launches[KI_COUNT];
-	double prof_ns[KI_COUNT];
+	uint64_t prof_launches[KI_COUNT] = {};
+	double prof_ns[KI_COUNT] = {};
 
-	int initialised;
+	std::vector<int64_t> slot_rid;            /* replica id per slot, -1 free */
+	std::map<uint32_t, uint32_t> free_cells;  /* first cell -> length */
+	uint64_t reserved = 0;
+	std::map<uint32_t, Replica> reps;
+	uint32_t next_id = 0;
+
+	Tables tables() const {
+		return Tables{rep_dev, rctl_dev, cbs_dev, order_dev};
+	}
+};
+
+struct pp_sim {
+	pp_batch_t *b = nullptr;
+	pp_params_t prm{};
+	uint64_t seed = 0;
+	uint32_t cap = 0;
+	uint32_t rid = 0;
+	int initialised = 0;
 };
 
 #define HIP_OK(expr)                                                        \
@@ -1185,10 +1565,16 @@ struct pp_sim {
 		}                                                                   \
 	} while (0)
 
-static hipEvent_t event_get(pp_sim_t *s) {
-	if (!s->free_events.empty()) {
-		hipEvent_t e = s->free_events.back();
-		s->free_events.pop_back();
+#define PP_TRY(expr)                                                        \
+	do {                                                                    \
+		int rc_ = (expr);                                                   \
+		if (rc_ != PP_SUCCESS) return rc_;                                  \
+	} while (0)
+
+static hipEvent_t event_get(pp_batch_t *b) {
+	if (!b->free_events.empty()) {
+		hipEvent_t e = b->free_events.back();
+		b->free_events.pop_back();
 		return e;
 	}
 	hipEvent_t e = nullptr;
@@ -1196,33 +1582,33 @@ static hipEvent_t event_get(pp_sim_t *s) {
 	return e;
 }
 
-/* Profiling wrappers: bracket a launch with events on the sim's stream. */
-static void prof_begin(pp_sim_t *s, int kernel, PendingEvent &pe) {
+/* Profiling wrappers: bracket a launch with events on the batch's stream. */
+static void prof_begin(pp_batch_t *b, int kernel, PendingEvent &pe) {
 	pe.kernel = kernel;
 	pe.start = pe.stop = nullptr;
-	if (!s->profiling) return;
-	pe.start = event_get(s);
-	pe.stop = event_get(s);
-	if (pe.start) (void)hipEventRecord(pe.start, s->stream);
+	if (!b->profiling) return;
+	pe.start = event_get(b);
+	pe.stop = event_get(b);
+	if (pe.start) (void)hipEventRecord(pe.start, b->stream);
 }
 
-static void prof_end(pp_sim_t *s, PendingEvent &pe) {
-	if (!s->profiling || !pe.start || !pe.stop) return;
-	(void)hipEventRecord(pe.stop, s->stream);
-	s->pending.push_back(pe);
+static void prof_end(pp_batch_t *b, PendingEvent &pe) {
+	if (!b->profiling || !pe.start || !pe.stop) return;
+	(void)hipEventRecord(pe.stop, b->stream);
+	b->pending.push_back(pe);
 }
 
-static int prof_collect(pp_sim_t *s) {
-	for (auto &pe : s->pending) {
+static int prof_collect(pp_batch_t *b) {
+	for (auto &pe : b->pending) {
 		float ms = 0.f;
 		HIP_OK(hipEventSynchronize(pe.stop));
 		HIP_OK(hipEventElapsedTime(&ms, pe.start, pe.stop));
-		s->prof_ns[pe.kernel] += 1e6 * (double)ms;
-		s->prof_launches[pe.kernel] += 1;
-		s->free_events.push_back(pe.start);
-		s->free_events.push_back(pe.stop);
+		b->prof_ns[pe.kernel] += 1e6 * (double)ms;
+		b->prof_launches[pe.kernel] += 1;
+		b->free_events.push_back(pe.start);
+		b->free_events.push_back(pe.stop);
 	}
-	s->pending.clear();
+	b->pending.clear();
 	return PP_SUCCESS;
 }
 
@@ -1231,381 +1617,789 @@ static int prof_collect(pp_sim_t *s) {
 static int pass_src(uint32_t p) { return p == 0 ? 0 : 1 + (int)((p - 1) & 1); }
 static int pass_dst(uint32_t p) { return 1 + (int)(p & 1); }
 
+template <typename GT>
+static void launch_sort_scan(pp_batch_t *b, uint32_t pass) {
+	k_sort_scan<GT><<<1u << b->P.digit_bits, WG, 0, b->stream>>>(b->hist,
+		b->totals, b->ctl, pass, b->P, b->act_cnt, b->act_words,
+		(GT *)b->grass_at);
+}
+
 template <int KPT, int RBITS>
-static void launch_sort(pp_sim_t *s, uint32_t pass) {
+static void launch_sort(pp_batch_t *b, uint32_t pass) {
 	PendingEvent pe;
 	const int src = pass_src(pass), dst = pass_dst(pass);
 	if (pass > 0) {
-		prof_begin(s, KI_SORT_UP, pe);
-		k_sort_up<KPT><<<s->grid_up, WG, 0, s->stream>>>(s->keys[src], s->hist,
-			s->ctl, pass, s->P);
-		prof_end(s, pe);
+		prof_begin(b, KI_SORT_UP, pe);
+		k_sort_up<KPT><<<b->grid_up, WG, 0, b->stream>>>(b->keys[src], b->hist,
+			b->ctl, pass, b->P);
+		prof_end(b, pe);
 	}
-	prof_begin(s, KI_SORT_SCAN, pe);
-	const uint32_t nscan = 1u << s->P.digit_bits;
-	if (s->grass_bytes == 1)
-		k_sort_scan<uint8_t><<<nscan, WG, 0, s->stream>>>(s->hist, s->totals,
-			s->ctl, pass, s->P, s->act_cnt, s->act_words,
-			(uint8_t *)s->grass_at);
-	else if (s->grass_bytes == 2)
-		k_sort_scan<uint16_t><<<nscan, WG, 0, s->stream>>>(s->hist, s->totals,
-			s->ctl, pass, s->P, s->act_cnt, s->act_words,
-			(uint16_t *)s->grass_at);
-	else
-		k_sort_scan<uint32_t><<<nscan, WG, 0, s->stream>>>(s->hist, s->totals,
-			s->ctl, pass, s->P, s->act_cnt, s->act_words,
-			(uint32_t *)s->grass_at);
-	prof_end(s, pe);
-	prof_begin(s, KI_SORT_DOWN, pe);
-	k_sort_down<KPT, RBITS><<<s->grid_down, WG, 0, s->stream>>>(s->keys[src],
-		s->vals[src], s->keys[dst], s->vals[dst], s->hist, s->totals, s->ctl,
-		pass, s->P);
-	prof_end(s, pe);
+	prof_begin(b, KI_SORT_SCAN, pe);
+	if (b->grass_bytes == 1) launch_sort_scan<uint8_t>(b, pass);
+	else if (b->grass_bytes == 2) launch_sort_scan<uint16_t>(b, pass);
+	else launch_sort_scan<uint32_t>(b, pass);
+	prof_end(b, pe);
+	prof_begin(b, KI_SORT_DOWN, pe);
+	k_sort_down<KPT, RBITS><<<b->grid_down, WG, 0, b->stream>>>(b->keys[src],
+		b->vals[src], b->keys[dst], b->vals[dst], b->hist, b->totals, b->ctl,
+		pass, b->P);
+	prof_end(b, pe);
 }
 
-static void launch_sort_pass(pp_sim_t *s, uint32_t pass) {
-	if (s->kpt == 8) {
-		if (s->rbits == 11) launch_sort<8, 11>(s, pass);
-		else launch_sort<8, 8>(s, pass);
+static void launch_sort_pass(pp_batch_t *b, uint32_t pass) {
+	if (b->shape.kpt == 8) {
+		if (b->rbits == 11) launch_sort<8, 11>(b, pass);
+		else launch_sort<8, 8>(b, pass);
 	} else {
-		if (s->rbits == 11) launch_sort<16, 11>(s, pass);
-		else launch_sort<16, 8>(s, pass);
+		if (b->rbits == 11) launch_sort<16, 11>(b, pass);
+		else launch_sort<16, 8>(b, pass);
 	}
 }
 
-/* Enqueue one iteration (no synchronisation): the sort of the pairs the
- * previous iteration's emit (or the move after init) produced, the actions,
- * the scan + end of iteration, and the emit + move of the next iteration. */
-static int enqueue_step(pp_sim_t *s) {
+/* Pass-0 histograms of the pairs in buffer 0 for the current shape (after
+ * a change of shape; otherwise the emit builds them). */
+static void launch_hist0(pp_batch_t *b) {
 	PendingEvent pe;
-	for (uint32_t pass = 0; pass < s->P.passes; pass++) launch_sort_pass(s, pass);
-	const int sorted = pass_dst(s->P.passes - 1);
-	prof_begin(s, KI_ACT, pe);
-	if (s->grass_bytes == 1)
-		k_act<uint8_t><<<s->grid_act, WG, 0, s->stream>>>(s->keys[sorted],
-			s->vals[sorted], (uint8_t *)s->grass_at, s->act_cnt, s->hist,
-			s->hist_words, s->ctl, s->P);
-	else if (s->grass_bytes == 2)
-		k_act<uint16_t><<<s->grid_act, WG, 0, s->stream>>>(s->keys[sorted],
-			s->vals[sorted], (uint16_t *)s->grass_at, s->act_cnt, s->hist,
-			s->hist_words, s->ctl, s->P);
+	prof_begin(b, KI_SORT_UP, pe);
+	if (b->shape.kpt == 8)
+		k_sort_up<8><<<b->grid_up, WG, 0, b->stream>>>(b->keys[0], b->hist,
+			b->ctl, 0, b->P);
 	else
-		k_act<uint32_t><<<s->grid_act, WG, 0, s->stream>>>(s->keys[sorted],
-			s->vals[sorted], (uint32_t *)s->grass_at, s->act_cnt, s->hist,
-			s->hist_words, s->ctl, s->P);
-	prof_end(s, pe);
-	prof_begin(s, KI_ACT_SCAN, pe);
-	k_act_scan<<<1, SCAN_WG, 0, s->stream>>>(s->act_cnt, s->ctl, s->ring,
-		s->stats_dev, s->P);
-	prof_end(s, pe);
-	prof_begin(s, KI_EMIT_MOVE, pe);
-	if (s->rbits == 11)
-		k_emit_move<11><<<s->grid_emit, WG, 0, s->stream>>>(s->keys[sorted],
-			s->vals[sorted], s->act_cnt, s->cell, s->energy, s->type,
-			s->keys[0], s->vals[0], s->hist, s->ctl, s->P);
+		k_sort_up<16><<<b->grid_up, WG, 0, b->stream>>>(b->keys[0], b->hist,
+			b->ctl, 0, b->P);
+	prof_end(b, pe);
+}
+
+template <typename GT>
+static void launch_act(pp_batch_t *b, int sorted) {
+	k_act<GT><<<b->grid_act, WG, 0, b->stream>>>(b->keys[sorted],
+		b->vals[sorted], (GT *)b->grass_at, b->act_cnt, b->hist, b->hist_words,
+		b->ctl, b->tables(), b->P);
+}
+
+/* Enqueue one tick (no synchronisation): the sort of the pairs the previous
+ * tick's emit (or the moves at admission) produced, the actions, the scan +
+ * end of tick, and the emit + move of the next tick. */
+static int enqueue_step(pp_batch_t *b) {
+	PendingEvent pe;
+	for (uint32_t pass = 0; pass < b->P.passes; pass++) launch_sort_pass(b, pass);
+	const int sorted = pass_dst(b->P.passes - 1);
+	prof_begin(b, KI_ACT, pe);
+	if (b->grass_bytes == 1) launch_act<uint8_t>(b, sorted);
+	else if (b->grass_bytes == 2) launch_act<uint16_t>(b, sorted);
+	else launch_act<uint32_t>(b, sorted);
+	prof_end(b, pe);
+	prof_begin(b, KI_ACT_SCAN, pe);
+	k_act_scan<<<1, SCAN_WG, 0, b->stream>>>(b->act_cnt, b->ctl, b->tables(),
+		b->P);
+	prof_end(b, pe);
+	prof_begin(b, KI_EMIT_MOVE, pe);
+	if (b->rbits == 11)
+		k_emit_move<11><<<b->grid_emit, WG, 0, b->stream>>>(b->keys[sorted],
+			b->vals[sorted], b->act_cnt, b->cell, b->energy, b->type,
+			b->keys[0], b->vals[0], b->hist, b->ctl, b->tables(), b->P);
 	else
-		k_emit_move<8><<<s->grid_emit, WG, 0, s->stream>>>(s->keys[sorted],
-			s->vals[sorted], s->act_cnt, s->cell, s->energy, s->type,
-			s->keys[0], s->vals[0], s->hist, s->ctl, s->P);
-	prof_end(s, pe);
+		k_emit_move<8><<<b->grid_emit, WG, 0, b->stream>>>(b->keys[sorted],
+			b->vals[sorted], b->act_cnt, b->cell, b->energy, b->type,
+			b->keys[0], b->vals[0], b->hist, b->ctl, b->tables(), b->P);
+	prof_end(b, pe);
 	HIP_OK(hipGetLastError());
 	return PP_SUCCESS;
 }
 
-static int read_ctl(pp_sim_t *s) {
-	HIP_OK(hipMemcpyAsync(s->ctl_host, s->ctl, sizeof(Ctl),
-		hipMemcpyDeviceToHost, s->stream));
-	HIP_OK(hipStreamSynchronize(s->stream));
-	return prof_collect(s);
+static void drop_graph(pp_batch_t *b) {
+	if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+	b->graph_exec = nullptr;
+	b->graph_steps = 0;
 }
 
-static uint32_t ceil_div(uint64_t a, uint64_t b) {
-	return (uint32_t)((a + b - 1) / b);
+/* Sets the sort shape and the launch grids that depend on it. */
+static void apply_shape(pp_batch_t *b, const SortShape &sh) {
+	drop_graph(b);
+	b->shape = sh;
+	Params &P = b->P;
+	shape_params(sh, b->key_bits, P.cap, &P.passes, &P.digit_bits, &P.hist_cols);
+	P.sort_tile = WG * sh.kpt;
+	b->rbits = P.digit_bits > 8 ? 11 : 8;
+	b->hist_words = (1u << P.digit_bits) * P.hist_cols;
+	/* Enough groups to fill every CU several times, never more than the
+	 * tiles of a full-capacity population; all loops are grid-strided. */
+	const uint32_t cu = (uint32_t)b->num_cu;
+	b->grid_up = min_u32(P.hist_cols, 8 * cu);
+	b->grid_down = min_u32(P.hist_cols, (b->rbits == 11 ? 1 : 2) * cu);
+	b->grid_act = min_u32(ceil_div(P.cap, A_TILE), 8 * cu);
+	b->grid_emit = min_u32(ceil_div(P.cap, A_TILE), 8 * cu);
 }
 
-static uint32_t min_u32(uint32_t a, uint32_t b) { return a < b ? a : b; }
-
-static void sim_free(pp_sim_t *s) {
-	if (!s) return;
-	if (s->device >= 0) (void)hipSetDevice(s->device);
-	if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
-	for (auto &pe : s->pending) {
-		(void)hipEventDestroy(pe.start);
-		(void)hipEventDestroy(pe.stop);
+/* Capture `steps` ticks into one executable graph (kernels only; no memset
+ * or copy nodes). */
+static int build_graph(pp_batch_t *b, int steps) {
+	hipGraph_t g = nullptr;
+	HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+	int rc = PP_SUCCESS;
+	for (int i = 0; i < steps && rc == PP_SUCCESS; i++) rc = enqueue_step(b);
+	hipError_t e = hipStreamEndCapture(b->stream, &g);
+	if (rc != PP_SUCCESS) {
+		if (g) (void)hipGraphDestroy(g);
+		return rc;
 	}
-	for (auto e : s->free_events) (void)hipEventDestroy(e);
-	(void)hipFree(s->cell);
-	(void)hipFree(s->energy);
-	(void)hipFree(s->type);
-	for (int i = 0; i < 3; i++) {
-		(void)hipFree(s->keys[i]);
-		(void)hipFree(s->vals[i]);
-	}
-	(void)hipFree(s->grass_at);
-	(void)hipFree(s->ring);
-	(void)hipFree(s->hist);
-	(void)hipFree(s->totals);
-	(void)hipFree(s->act_cnt);
-	(void)hipFree(s->ctl);
-	(void)hipFree(s->stats_dev);
-	if (s->stats_host) (void)hipHostFree(s->stats_host);
-	if (s->ctl_host) (void)hipHostFree(s->ctl_host);
-	if (s->stream) (void)hipStreamDestroy(s->stream);
-	delete s;
-}
-
-static int sim_alloc(pp_sim_t *s) {
-	const uint64_t A = s->cap, C = s->P.C;
-	/* Vector accesses of the move read/write 16 bytes at multiples of 4. */
-	const uint64_t A4 = (A + 3) & ~3ull;
-	HIP_OK(hipMalloc(&s->cell, A4 * 4));
-	HIP_OK(hipMalloc(&s->energy, A4 * 4));
-	HIP_OK(hipMalloc(&s->type, A4));
-	for (int i = 0; i < 3; i++) {
-		HIP_OK(hipMalloc(&s->keys[i], A4 * 4));
-		HIP_OK(hipMalloc(&s->vals[i], A4 * 4));
-	}
-	HIP_OK(hipMalloc(&s->grass_at, C * s->grass_bytes));
-	s->ring_n = s->P.R + 1;
-	HIP_OK(hipMalloc(&s->ring, (size_t)s->ring_n * 8));
-	s->hist_words = (1u << s->P.digit_bits) * s->P.hist_cols;
-	HIP_OK(hipMalloc(&s->hist, (size_t)s->hist_words * 4));
-	HIP_OK(hipMalloc(&s->totals, RADIX_MAX * 4));
-	s->act_words = ceil_div(A, A_TILE) + 1;
-	HIP_OK(hipMalloc(&s->act_cnt, (size_t)s->act_words * 4));
-	HIP_OK(hipMalloc(&s->ctl, sizeof(Ctl)));
-	HIP_OK(hipMalloc(&s->stats_dev, STATS_RING * sizeof(pp_stats64_t)));
-	HIP_OK(hipHostMalloc(&s->stats_host, STATS_RING * sizeof(pp_stats64_t),
-		hipHostMallocDefault));
-	HIP_OK(hipHostMalloc(&s->ctl_host, sizeof(Ctl), hipHostMallocDefault));
-	HIP_OK(hipMemset(s->ctl, 0, sizeof(Ctl)));
-	HIP_OK(hipDeviceSynchronize());
-	memset(s->ctl_host, 0, sizeof(Ctl));
+	HIP_OK(e);
+	e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+	(void)hipGraphDestroy(g);
+	HIP_OK(e);
+	b->graph_steps = steps;
 	return PP_SUCCESS;
 }
 
-extern "C" {
+/* Enqueue `ticks` ticks: whole graphs of 8, then single launches. */
+static int enqueue_ticks(pp_batch_t *b, uint32_t ticks) {
+	b->ctl_fresh = 0;
+	const int use_graph = !b->profiling && getenv("PPHPC_NO_GRAPH") == nullptr;
+	uint32_t i = 0;
+	if (use_graph && ticks >= 8) {
+		if (!b->graph_exec) PP_TRY(build_graph(b, 8));
+		for (; i + (uint32_t)b->graph_steps <= ticks; i += b->graph_steps)
+			HIP_OK(hipGraphLaunch(b->graph_exec, b->stream));
+	}
+	for (; i < ticks; i++) PP_TRY(enqueue_step(b));
+	return PP_SUCCESS;
+}
 
-int pp_sim_create(const pp_params_t *p, uint64_t seed, uint64_t capacity,
-	int device, pp_sim_t **out) {
-	if (!p || !out) return PP_INVALID_ARGS;
+static int read_ctl(pp_batch_t *b) {
+	HIP_OK(hipMemcpyAsync(b->ctl_host, b->ctl, sizeof(Ctl),
+		hipMemcpyDeviceToHost, b->stream));
+	HIP_OK(hipMemcpyAsync(b->rctl_host, b->rctl_dev,
+		(size_t)b->max_rep * sizeof(RepCtl), hipMemcpyDeviceToHost, b->stream));
+	HIP_OK(hipStreamSynchronize(b->stream));
+	b->ctl_fresh = 1;
+	return prof_collect(b);
+}
+
+/* Replica table: the slots in use sorted by first cell, and its length. The
+ * stream must be idle. */
+static int upload_table(pp_batch_t *b) {
+	std::vector<std::pair<uint32_t, uint32_t>> t;
+	for (uint32_t s = 0; s < b->max_rep; s++)
+		if (b->slot_rid[s] >= 0) t.push_back({b->rep_host[s].cb, s});
+	std::sort(t.begin(), t.end());
+	std::vector<uint32_t> cbs(t.size()), order(t.size());
+	for (size_t i = 0; i < t.size(); i++) {
+		cbs[i] = t[i].first;
+		order[i] = t[i].second;
+	}
+	if (!t.empty()) {
+		HIP_OK(hipMemcpy(b->cbs_dev, cbs.data(), t.size() * 4,
+			hipMemcpyHostToDevice));
+		HIP_OK(hipMemcpy(b->order_dev, order.data(), t.size() * 4,
+			hipMemcpyHostToDevice));
+	}
+	const uint32_t nrep = (uint32_t)t.size();
+	HIP_OK(hipMemcpy(&b->ctl->nrep, &nrep, 4, hipMemcpyHostToDevice));
+	b->ctl_host->nrep = nrep;
+	return PP_SUCCESS;
+}
+
+static void release_cells(pp_batch_t *b, uint32_t cb, uint32_t C) {
+	auto it = b->free_cells.emplace(cb, C).first;
+	auto next = std::next(it);
+	if (next != b->free_cells.end() && it->first + it->second == next->first) {
+		it->second += next->second;
+		b->free_cells.erase(next);
+	}
+	if (it != b->free_cells.begin()) {
+		auto prev = std::prev(it);
+		if (prev->first + prev->second == it->first) {
+			prev->second += it->second;
+			b->free_cells.erase(it);
+		}
+	}
+}
+
+/* First fit in the free cell ranges; returns the first cell or -1. */
+static int64_t take_cells(pp_batch_t *b, uint32_t C) {
+	for (auto it = b->free_cells.begin(); it != b->free_cells.end(); ++it) {
+		if (it->second < C) continue;
+		const uint32_t cb = it->first, len = it->second;
+		b->free_cells.erase(it);
+		if (len > C) b->free_cells.emplace(cb + C, len - C);
+		return cb;
+	}
+	return -1;
+}
+
+static void release_slot(pp_batch_t *b, Replica &r) {
+	if (r.slot < 0) return;
+	b->slot_rid[r.slot] = -1;
+	release_cells(b, r.cb, r.C);
+	b->reserved -= r.reserve;
+	r.slot = -1;
+}
+
+/* After the stream is idle: new statistics rows of every replica (to its
+ * host rows or sink), replica states, and the release of the slots, cells
+ * and reservations of replicas that are done or failed. */
+static int collect(pp_batch_t *b) {
+	PP_TRY(read_ctl(b));
+	std::vector<Replica *> got;
+	for (auto &kv : b->reps) {
+		Replica &r = kv.second;
+		if (r.slot < 0) continue;
+		const RepCtl &rc = b->rctl_host[r.slot];
+		if (rc.status == RS_FREE) continue;
+		const uint32_t last = rc.step;  /* rows 0..last exist */
+		if (last + 1 > r.rows_have) {
+			const uint32_t first = r.rows_have, nrows = last + 1 - first;
+			const uint32_t R = b->stats_rows;
+			pp_stats64_t *dev = b->stats_dev + (size_t)r.slot * R;
+			pp_stats64_t *host = b->stats_host + (size_t)r.slot * R;
+			const uint32_t a = first % R;
+			const uint32_t n1 = min_u32(nrows, R - a);
+			HIP_OK(hipMemcpyAsync(host + a, dev + a, (size_t)n1 * sizeof(pp_stats64_t),
+				hipMemcpyDeviceToHost, b->stream));
+			if (nrows > n1)
+				HIP_OK(hipMemcpyAsync(host, dev, (size_t)(nrows - n1) *
+					sizeof(pp_stats64_t), hipMemcpyDeviceToHost, b->stream));
+			got.push_back(&r);
+		}
+	}
+	if (!got.empty()) HIP_OK(hipStreamSynchronize(b->stream));
+	int dirty = 0;
+	for (auto &kv : b->reps) {
+		Replica &r = kv.second;
+		if (r.slot < 0) continue;
+		const RepCtl &rc = b->rctl_host[r.slot];
+		if (rc.status == RS_FREE) continue;
+		const uint32_t R = b->stats_rows;
+		const pp_stats64_t *host = b->stats_host + (size_t)r.slot * R;
+		for (uint32_t i = r.rows_have; i <= rc.step; i++) {
+			const pp_stats64_t &row = host[i % R];
+			if (r.keep) {
+				if (r.rows.size() <= i) r.rows.resize(i + 1);
+				r.rows[i] = row;
+			} else if (r.sink && i >= r.sink_from) {
+				r.sink[i - r.sink_from] = row;
+			}
+		}
+		r.rows_have = rc.step + 1;
+		r.n = rc.n;
+		r.agent_steps = rc.agent_steps;
+		r.status = rc.status;
+		if (rc.status == RS_FAILED) {
+			r.error = rc.error;
+			r.err_step = rc.err_step;
+		}
+		if (rc.status == RS_DONE || rc.status == RS_FAILED) {
+			release_slot(b, r);
+			dirty = 1;
+		}
+	}
+	if (dirty) PP_TRY(upload_table(b));
+	return PP_SUCCESS;
+}
+
+static uint32_t active_count(const pp_batch_t *b) {
+	uint32_t n = 0;
+	for (auto &kv : b->reps)
+		if (kv.second.slot >= 0 && kv.second.status == RS_ACTIVE) n++;
+	return n;
+}
+
+/* Runs up to `ticks` ticks in chunks that the stats rings hold, collecting
+ * after each; stops early when no replica is active or on a batch error. */
+static int advance(pp_batch_t *b, uint32_t ticks) {
+	HIP_OK(hipSetDevice(b->device));
+	if (!b->ctl_fresh) PP_TRY(read_ctl(b));
+	if (b->ctl_host->error) return b->ctl_host->error;
+	const uint64_t last = (uint64_t)b->ctl_host->tick + ticks + b->P.R + 1;
+	if (last >= 0xFFFFFFFFull) return PP_INVALID_ARGS;
+	uint32_t done = 0;
+	const uint32_t chunk_max = b->stats_rows / 2;
+	while (done < ticks && active_count(b) > 0) {
+		const uint32_t chunk = min_u32(ticks - done, chunk_max);
+		PP_TRY(enqueue_ticks(b, chunk));
+		PP_TRY(collect(b));
+		done += chunk;
+		if (b->ctl_host->error) return b->ctl_host->error;
+	}
+	return PP_SUCCESS;
+}
+
+static void batch_free(pp_batch_t *b) {
+	if (!b) return;
+	if (b->device >= 0) (void)hipSetDevice(b->device);
+	drop_graph(b);
+	for (auto &pe : b->pending) {
+		(void)hipEventDestroy(pe.start);
+		(void)hipEventDestroy(pe.stop);
+	}
+	for (auto e : b->free_events) (void)hipEventDestroy(e);
+	(void)hipFree(b->cell);
+	(void)hipFree(b->energy);
+	(void)hipFree(b->type);
+	for (int i = 0; i < 3; i++) {
+		(void)hipFree(b->keys[i]);
+		(void)hipFree(b->vals[i]);
+	}
+	(void)hipFree(b->grass_at);
+	(void)hipFree(b->ring);
+	(void)hipFree(b->hist);
+	(void)hipFree(b->totals);
+	(void)hipFree(b->act_cnt);
+	(void)hipFree(b->ctl);
+	(void)hipFree(b->rep_dev);
+	(void)hipFree(b->rctl_dev);
+	(void)hipFree(b->cbs_dev);
+	(void)hipFree(b->order_dev);
+	(void)hipFree(b->stats_dev);
+	if (b->stats_host) (void)hipHostFree(b->stats_host);
+	if (b->ctl_host) (void)hipHostFree(b->ctl_host);
+	if (b->rctl_host) (void)hipHostFree(b->rctl_host);
+	if (b->stream) (void)hipStreamDestroy(b->stream);
+	delete b;
+}
+
+/* Device bytes of a batch (the allocations of batch_alloc). */
+static uint64_t batch_bytes(uint64_t A, uint64_t C, uint32_t max_rep,
+	uint32_t R, uint32_t stats_rows, uint64_t hist_words) {
+	const uint64_t A4 = (A + 3) & ~3ull;
+	uint64_t t = A4 * (4 + 4 + 1) + 6 * A4 * 4;
+	t += C * grass_bytes_for(R);
+	t += (uint64_t)max_rep * (R + 1) * 8;
+	t += hist_words * 4 + RADIX_MAX * 4;
+	t += ((uint64_t)ceil_div(A, A_TILE) + 1) * 4;
+	t += sizeof(Ctl) + (uint64_t)max_rep * (sizeof(Rep) + sizeof(RepCtl) + 8);
+	t += (uint64_t)max_rep * stats_rows * sizeof(pp_stats64_t);
+	return t;
+}
+
+static uint32_t stats_rows_for(uint32_t max_rep) {
+	return max_rep == 1 ? STATS_RING : 1024;
+}
+
+static uint64_t max_hist_words(uint32_t key_bits, uint32_t cap) {
+	uint64_t w = 0;
+	for (const SortShape &sh : tune_candidates(key_bits, cap))
+		w = std::max(w, hist_bytes_for(sh, key_bits, cap) / 4);
+	return w;
+}
+
+static int batch_alloc(pp_batch_t *b) {
+	const uint64_t A = b->P.cap, C = b->P.C;
+	/* Vector accesses read/write 16 bytes at multiples of 4 agents. */
+	const uint64_t A4 = (A + 3) & ~3ull;
+	HIP_OK(hipMalloc(&b->cell, A4 * 4));
+	HIP_OK(hipMalloc(&b->energy, A4 * 4));
+	HIP_OK(hipMalloc(&b->type, A4));
+	for (int i = 0; i < 3; i++) {
+		HIP_OK(hipMalloc(&b->keys[i], A4 * 4));
+		HIP_OK(hipMalloc(&b->vals[i], A4 * 4));
+	}
+	HIP_OK(hipMalloc(&b->grass_at, C * b->grass_bytes));
+	b->ring_stride = b->P.R + 1;
+	HIP_OK(hipMalloc(&b->ring, (size_t)b->max_rep * b->ring_stride * 8));
+	b->hist_alloc_words = max_hist_words(b->key_bits, b->P.cap);
+	HIP_OK(hipMalloc(&b->hist, (size_t)b->hist_alloc_words * 4));
+	HIP_OK(hipMalloc(&b->totals, RADIX_MAX * 4));
+	b->act_words = ceil_div(A, A_TILE) + 1;
+	HIP_OK(hipMalloc(&b->act_cnt, (size_t)b->act_words * 4));
+	HIP_OK(hipMalloc(&b->ctl, sizeof(Ctl)));
+	HIP_OK(hipMalloc(&b->rep_dev, (size_t)b->max_rep * sizeof(Rep)));
+	HIP_OK(hipMalloc(&b->rctl_dev, (size_t)b->max_rep * sizeof(RepCtl)));
+	HIP_OK(hipMalloc(&b->cbs_dev, (size_t)b->max_rep * 4));
+	HIP_OK(hipMalloc(&b->order_dev, (size_t)b->max_rep * 4));
+	const size_t srows = (size_t)b->max_rep * b->stats_rows;
+	HIP_OK(hipMalloc(&b->stats_dev, srows * sizeof(pp_stats64_t)));
+	HIP_OK(hipHostMalloc(&b->stats_host, srows * sizeof(pp_stats64_t),
+		hipHostMallocDefault));
+	HIP_OK(hipHostMalloc(&b->ctl_host, sizeof(Ctl), hipHostMallocDefault));
+	HIP_OK(hipHostMalloc(&b->rctl_host, (size_t)b->max_rep * sizeof(RepCtl),
+		hipHostMallocDefault));
+	HIP_OK(hipMemset(b->ctl, 0, sizeof(Ctl)));
+	HIP_OK(hipMemset(b->rctl_dev, 0, (size_t)b->max_rep * sizeof(RepCtl)));
+	HIP_OK(hipDeviceSynchronize());
+	memset(b->ctl_host, 0, sizeof(Ctl));
+	memset(b->rctl_host, 0, (size_t)b->max_rep * sizeof(RepCtl));
+	b->rep_host.assign(b->max_rep, Rep{});
+	b->slot_rid.assign(b->max_rep, -1);
+	return PP_SUCCESS;
+}
+
+static int batch_reset(pp_batch_t *b);
+
+static int batch_create(uint64_t agents, uint64_t cells, uint32_t max_rep,
+	uint32_t max_R, int device, pp_batch_t **out) {
 	*out = nullptr;
+	/* Output offsets and ids are 32-bit, and an iteration can at most double
+	 * the survivors. */
+	if (agents == 0 || agents >= (1ull << 29)) return PP_INVALID_ARGS;
+	if (cells == 0 || cells > 0xFFFFFFFEull) return PP_INVALID_ARGS;
+	if (max_rep == 0 || max_rep > 65536) return PP_INVALID_ARGS;
+	if (max_R == 0 || max_R > (1u << 24)) return PP_INVALID_ARGS;
+	pp_batch_t *b = new pp_batch_t();
+	b->device = device;
+	if (device < 0 && hipGetDevice(&b->device) != hipSuccess) {
+		delete b;
+		return PP_LIBRARY_ERROR;
+	}
+	if (hipSetDevice(b->device) != hipSuccess) {
+		b->device = -1;
+		batch_free(b);
+		return PP_LIBRARY_ERROR;
+	}
+	if (hipDeviceGetAttribute(&b->num_cu, hipDeviceAttributeMultiprocessorCount,
+			b->device) != hipSuccess || b->num_cu <= 0)
+		b->num_cu = 256;
+	if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+		batch_free(b);
+		return PP_LIBRARY_ERROR;
+	}
+	b->P.cap = (uint32_t)agents;
+	b->P.C = (uint32_t)cells;
+	b->P.R = max_R;
+	b->max_rep = max_rep;
+	b->stats_rows = stats_rows_for(max_rep);
+	b->key_bits = key_bits_for(cells);
+	/* Narrowest grass storage whose modular window holds R + REFRESH + 1. */
+	b->grass_bytes = grass_bytes_for(max_R);
+	const char *gb_env = getenv("PPHPC_GRASS_BYTES");
+	if (gb_env) {
+		const int v = atoi(gb_env);
+		if ((v == 2 && max_R + REFRESH + 1 <= 65536) || v == 4) b->grass_bytes = v;
+	}
+	apply_shape(b, default_shape(b->key_bits));
+	int rc = batch_alloc(b);
+	if (rc != PP_SUCCESS) {
+		batch_free(b);
+		return rc;
+	}
+	rc = batch_reset(b);
+	if (rc != PP_SUCCESS) {
+		batch_free(b);
+		return rc;
+	}
+	*out = b;
+	return PP_SUCCESS;
+}
+
+/* Empties the batch (tick 0, no replicas). */
+static int batch_reset(pp_batch_t *b) {
+	HIP_OK(hipSetDevice(b->device));
+	HIP_OK(hipStreamSynchronize(b->stream));
+	PendingEvent pe;
+	prof_begin(b, KI_INIT_RESET, pe);
+	k_batch_reset<<<min_u32(ceil_div(b->hist_alloc_words, WG), 2048), WG, 0,
+		b->stream>>>(b->ctl, b->hist, (uint32_t)b->hist_alloc_words);
+	prof_end(b, pe);
+	HIP_OK(hipGetLastError());
+	HIP_OK(hipMemsetAsync(b->rctl_dev, 0, (size_t)b->max_rep * sizeof(RepCtl),
+		b->stream));
+	HIP_OK(hipStreamSynchronize(b->stream));
+	memset(b->ctl_host, 0, sizeof(Ctl));
+	memset(b->rctl_host, 0, (size_t)b->max_rep * sizeof(RepCtl));
+	b->ctl_fresh = 1;
+	b->reps.clear();
+	b->slot_rid.assign(b->max_rep, -1);
+	b->free_cells.clear();
+	b->free_cells[0] = b->P.C;
+	b->reserved = 0;
+	return PP_SUCCESS;
+}
+
+static int check_params(const pp_params_t *p) {
+	if (!p) return PP_INVALID_ARGS;
 	const uint64_t C = (uint64_t)p->grid_x * p->grid_y;
-	const uint64_t n0 = (uint64_t)p->init_sheep + p->init_wolves;
 	if (p->grid_x == 0 || p->grid_y == 0 || C > 0xFFFFFFFEull)
 		return PP_INVALID_ARGS;
 	if (p->grass_restart == 0 || p->grass_restart > (1u << 24))
 		return PP_INVALID_ARGS;
 	if (p->sheep_gain_from_food > (1u << 20) || p->wolves_gain_from_food > (1u << 20))
 		return PP_INVALID_ARGS;
+	if ((uint64_t)p->init_sheep + p->init_wolves >= (1ull << 29))
+		return PP_INVALID_ARGS;
+	return PP_SUCCESS;
+}
+
+/* Admits one replica: its slot, cells and agent reservation, its initial
+ * state appended to the batch state and moved for its first iteration. */
+static int admit(pp_batch_t *b, const pp_params_t *p, uint64_t seed,
+	uint64_t reserve, uint32_t iters, int keep, uint32_t *id_out) {
+	PP_TRY(check_params(p));
+	HIP_OK(hipSetDevice(b->device));
+	const uint32_t C = p->grid_x * p->grid_y;
+	const uint32_t n0 = p->init_sheep + p->init_wolves;
+	if (p->grass_restart > b->P.R || C > b->P.C) return PP_INVALID_ARGS;
+	if (reserve == 0) reserve = std::max<uint64_t>(4ull * n0, 1);
+	if (reserve < n0 || reserve > b->P.cap) return PP_INVALID_ARGS;
+	if (!b->ctl_fresh) PP_TRY(read_ctl(b));
+	if (b->ctl_host->error) return b->ctl_host->error;
+	const uint32_t tick = b->ctl_host->tick, pos = b->ctl_host->n;
+	/* (a plain simulation runs without a limit; pp_sim_run checks its runs) */
+	if (iters != ITERS_UNLIMITED &&
+		(uint64_t)tick + iters + b->P.R + 1 >= 0xFFFFFFFFull)
+		return PP_INVALID_ARGS;
+	if (b->reserved + reserve > b->P.cap || (uint64_t)pos + n0 > b->P.cap)
+		return PP_OUT_OF_RESOURCES;
+	int slot = -1;
+	for (uint32_t s = 0; s < b->max_rep; s++)
+		if (b->slot_rid[s] < 0) {
+			slot = (int)s;
+			break;
+		}
+	if (slot < 0) return PP_OUT_OF_RESOURCES;
+	const int64_t cb = take_cells(b, C);
+	if (cb < 0) return PP_OUT_OF_RESOURCES;
+
+	Rep rp{};
+	rp.cb = (uint32_t)cb;
+	rp.C = C;
+	rp.gx = p->grid_x;
+	rp.gy = p->grid_y;
+	rp.R = p->grass_restart;
+	rp.sheep_gain = p->sheep_gain_from_food;
+	rp.wolves_gain = p->wolves_gain_from_food;
+	rp.sheep_thr = p->sheep_reproduce_threshold;
+	rp.wolves_thr = p->wolves_reproduce_threshold;
+	rp.sheep_prob = p->sheep_reproduce_prob;
+	rp.wolves_prob = p->wolves_reproduce_prob;
+	rp.init_sheep = p->init_sheep;
+	rp.n0 = n0;
+	rp.key0 = (uint32_t)seed;
+	rp.key1 = (uint32_t)(seed >> 32);
+	rp.t0 = tick;
+	rp.iters = iters;
+	rp.cap = (uint32_t)reserve;
+	rp.stats_rows = b->stats_rows;
+	rp.ring = b->ring + (size_t)slot * b->ring_stride;
+	rp.stats = b->stats_dev + (size_t)slot * b->stats_rows;
+	int rc = PP_SUCCESS;
+	if (hipMemcpy(&b->rep_dev[slot], &rp, sizeof(Rep), hipMemcpyHostToDevice)
+			!= hipSuccess)
+		rc = PP_LIBRARY_ERROR;
+	if (rc) {
+		release_cells(b, (uint32_t)cb, C);
+		return rc;
+	}
+	b->rep_host[slot] = rp;
+	const uint32_t id = b->next_id++;
+	b->slot_rid[slot] = id;
+	b->reserved += reserve;
+	Replica &r = b->reps[id];
+	r.id = id;
+	r.slot = slot;
+	r.cb = (uint32_t)cb;
+	r.C = C;
+	r.prm = *p;
+	r.prm.grid_xy = C;
+	r.seed = seed;
+	r.iters = iters;
+	r.reserve = (uint32_t)reserve;
+	r.n0 = n0;
+	r.status = RS_ACTIVE;
+	r.error = 0;
+	r.err_step = 0xFFFFFFFFu;
+	r.rows_have = 0;
+	r.n = n0;
+	r.agent_steps = 0;
+	r.keep = keep;
+	r.sink = nullptr;
+	r.sink_from = 0;
+	if (keep) r.rows.reserve((size_t)iters + 1);
+	PP_TRY(upload_table(b));
+
+	const Tables T = b->tables();
+	const uint32_t cu = (uint32_t)b->num_cu;
+	PendingEvent pe;
+	prof_begin(b, KI_INIT_RESET, pe);
+	k_rep_reset<<<1, WG, 0, b->stream>>>(T, (uint32_t)slot);
+	prof_end(b, pe);
+	prof_begin(b, KI_INIT_CELLS, pe);
+	const uint32_t gc = min_u32(ceil_div(C, WG), 8 * cu);
+	if (b->grass_bytes == 1)
+		k_init_cells<uint8_t><<<gc, WG, 0, b->stream>>>((uint8_t *)b->grass_at, T,
+			(uint32_t)slot);
+	else if (b->grass_bytes == 2)
+		k_init_cells<uint16_t><<<gc, WG, 0, b->stream>>>((uint16_t *)b->grass_at,
+			T, (uint32_t)slot);
+	else
+		k_init_cells<uint32_t><<<gc, WG, 0, b->stream>>>((uint32_t *)b->grass_at,
+			T, (uint32_t)slot);
+	prof_end(b, pe);
+	prof_begin(b, KI_INIT_AGENTS, pe);
+	k_init_agents<<<min_u32(ceil_div(n0 ? n0 : 1, WG), 8 * cu), WG, 0,
+		b->stream>>>(b->cell, b->energy, b->type, T, (uint32_t)slot, pos);
+	prof_end(b, pe);
+	prof_begin(b, KI_INIT_DONE, pe);
+	k_rep_init_done<<<1, 64, 0, b->stream>>>(b->ctl, T, (uint32_t)slot, pos);
+	prof_end(b, pe);
+	prof_begin(b, KI_MOVE0, pe);
+	k_move0<<<min_u32(ceil_div(n0 ? n0 : 1, WG * M_IPT), 8 * cu), WG, 0,
+		b->stream>>>(b->cell, b->energy, b->type, b->keys[0], b->vals[0], b->hist,
+		b->ctl, T, (uint32_t)slot, pos, b->P);
+	prof_end(b, pe);
+	HIP_OK(hipGetLastError());
+	b->ctl_fresh = 0;
+	if (id_out) *id_out = id;
+	return PP_SUCCESS;
+}
+
+/* Batch positions of a replica's current agents: (base, n); the stream must
+ * be idle and the mirrors fresh. */
+static int state_of(pp_batch_t *b, const Replica &r, uint32_t *cell,
+	int32_t *energy, uint8_t *type, uint32_t *grass) {
+	if (r.slot < 0) return PP_INVALID_ARGS;
+	const RepCtl &rc = b->rctl_host[r.slot];
+	const size_t base = rc.base, n = rc.n;
+	if (cell && n) {
+		HIP_OK(hipMemcpy(cell, b->cell + base, n * 4, hipMemcpyDeviceToHost));
+		for (size_t i = 0; i < n; i++) cell[i] -= r.cb;
+	}
+	if (energy && n)
+		HIP_OK(hipMemcpy(energy, b->energy + base, n * 4, hipMemcpyDeviceToHost));
+	if (type && n) HIP_OK(hipMemcpy(type, b->type + base, n, hipMemcpyDeviceToHost));
+	if (grass) {
+		/* Countdown at tick t from the regrow tick a stored modulo 2^w:
+		 * d = (a - t) mod 2^w is the countdown if 1 <= d <= R, else 0. */
+		const uint32_t t = b->ctl_host->tick, R = r.prm.grass_restart, C = r.C;
+		const size_t gb = b->grass_bytes;
+		std::vector<uint8_t> raw((size_t)C * gb);
+		HIP_OK(hipMemcpy(raw.data(), (const uint8_t *)b->grass_at + (size_t)r.cb * gb,
+			raw.size(), hipMemcpyDeviceToHost));
+		const uint32_t mask = gb == 4 ? 0xFFFFFFFFu : ((1u << (8 * gb)) - 1u);
+		for (uint32_t c = 0; c < C; c++) {
+			uint32_t a;
+			if (gb == 1) a = raw[c];
+			else if (gb == 2) a = reinterpret_cast<const uint16_t *>(raw.data())[c];
+			else a = reinterpret_cast<const uint32_t *>(raw.data())[c];
+			const uint32_t d = (a - t) & mask;
+			grass[c] = (d >= 1 && d <= R) ? d : 0;
+		}
+	}
+	return PP_SUCCESS;
+}
+
+static int prof_get(pp_batch_t *b, int i, const char **name,
+	uint64_t *launches, double *total_ns) {
+	if (i < 0 || i >= KI_COUNT) return PP_INVALID_ARGS;
+	PP_TRY(prof_collect(b));
+	if (name) *name = kernel_names[i];
+	if (launches) *launches = b->prof_launches[i];
+	if (total_ns) *total_ns = b->prof_ns[i];
+	return PP_SUCCESS;
+}
+
+static int prof_export(pp_batch_t *b, const char *path) {
+	PP_TRY(prof_collect(b));
+	FILE *fp = fopen(path, "w");
+	if (!fp) return PP_UNABLE_SAVE_STATS;
+	double total = 0;
+	for (int i = 0; i < KI_COUNT; i++) total += b->prof_ns[i];
+	for (int i = 0; i < KI_COUNT; i++)
+		fprintf(fp, "\"%s\"\t%llu\t%lg\n", kernel_names[i],
+			(unsigned long long)b->prof_ns[i],
+			total > 0 ? b->prof_ns[i] / total : 0.0);
+	fclose(fp);
+	return PP_SUCCESS;
+}
+
+static double now_s() {
+	struct timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* Times `ticks` ticks of the current shape (graph replay, then collect). */
+static int time_ticks(pp_batch_t *b, uint32_t ticks, double *sec) {
+	HIP_OK(hipStreamSynchronize(b->stream));
+	const double t0 = now_s();
+	PP_TRY(enqueue_ticks(b, ticks));
+	HIP_OK(hipStreamSynchronize(b->stream));
+	*sec = now_s() - t0;
+	return collect(b);
+}
+
+extern "C" {
+
+/* ------------------------------------------------------ single simulation */
+
+int pp_sim_create(const pp_params_t *p, uint64_t seed, uint64_t capacity,
+	int device, pp_sim_t **out) {
+	if (!p || !out) return PP_INVALID_ARGS;
+	*out = nullptr;
+	PP_TRY(check_params(p));
+	const uint64_t C = (uint64_t)p->grid_x * p->grid_y;
+	const uint64_t n0 = (uint64_t)p->init_sheep + p->init_wolves;
 	if (capacity == 0) {
 		capacity = 4 * n0;
 		if (capacity < PP_DEFAULT_MAX_AGENTS) capacity = PP_DEFAULT_MAX_AGENTS;
 	}
-	/* Output offsets and ids are 32-bit, and an iteration can at most double
-	 * the survivors. */
 	if (capacity < n0 || capacity >= (1ull << 29)) return PP_INVALID_ARGS;
-
-	pp_sim_t *s = new pp_sim();
-	memset(s->prof_launches, 0, sizeof(s->prof_launches));
-	memset(s->prof_ns, 0, sizeof(s->prof_ns));
+	pp_sim_t *s = new pp_sim_t();
+	int rc = batch_create(capacity, C, 1, p->grass_restart, device, &s->b);
+	if (rc) {
+		delete s;
+		return rc;
+	}
 	s->prm = *p;
 	s->prm.grid_xy = (uint32_t)C;
 	s->seed = seed;
 	s->cap = (uint32_t)capacity;
-	s->device = device;
-	if (device < 0) {
-		if (hipGetDevice(&s->device) != hipSuccess) {
-			delete s;
-			return PP_LIBRARY_ERROR;
-		}
-	}
-	if (hipSetDevice(s->device) != hipSuccess) {
-		s->device = -1;
-		sim_free(s);
-		return PP_LIBRARY_ERROR;
-	}
-	if (hipDeviceGetAttribute(&s->num_cu, hipDeviceAttributeMultiprocessorCount,
-			s->device) != hipSuccess || s->num_cu <= 0)
-		s->num_cu = 256;
-	if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
-		sim_free(s);
-		return PP_LIBRARY_ERROR;
-	}
-
-	Params &P = s->P;
-	P.key0 = (uint32_t)seed;
-	P.key1 = (uint32_t)(seed >> 32);
-	P.gx = p->grid_x;
-	P.gy = p->grid_y;
-	P.C = (uint32_t)C;
-	P.R = p->grass_restart;
-	P.sheep_gain = p->sheep_gain_from_food;
-	P.wolves_gain = p->wolves_gain_from_food;
-	P.sheep_thr = p->sheep_reproduce_threshold;
-	P.wolves_thr = p->wolves_reproduce_threshold;
-	P.sheep_prob = p->sheep_reproduce_prob;
-	P.wolves_prob = p->wolves_reproduce_prob;
-	P.init_sheep = p->init_sheep;
-	P.n0 = (uint32_t)n0;
-	P.cap = s->cap;
-	/* Radix passes over the significant bits of the largest cell index, the
-	 * bits spread evenly: up to 22 bits in passes of at most 11 bits (2048
-	 * buckets; 1600 x 1600 = 2 x 11), wider indices in passes of at most 8
-	 * bits (12800 x 12800 = 4 x 7), where longer runs per bucket keep the
-	 * scatter coalesced. */
-	uint32_t bits = 0;
-	while (bits < 32 && ((C - 1) >> bits) != 0) bits++;
-	if (bits == 0) bits = 1;
-	const uint32_t max_bits = bits <= 22 ? 11 : 8;
-	P.passes = (bits + max_bits - 1) / max_bits;
-	P.digit_bits = (bits + P.passes - 1) / P.passes;
-	s->rbits = P.digit_bits > 8 ? 11 : 8;
-
-	/* Narrowest grass storage whose modular window holds R + REFRESH + 1. */
-	s->grass_bytes = P.R + REFRESH + 1 <= 256 ? 1
-		: P.R + REFRESH + 1 <= 65536 ? 2 : 4;
-	const char *gb_env = getenv("PPHPC_GRASS_BYTES");
-	if (gb_env) {
-		const int v = atoi(gb_env);
-		if ((v == 2 && P.R + REFRESH + 1 <= 65536) || v == 4) s->grass_bytes = v;
-	}
-
-	s->kpt = 16;
-	const char *kpt_env = getenv("PPHPC_SORT_KPT");
-	if (kpt_env && atoi(kpt_env) == 8) s->kpt = 8;
-	P.sort_tile = WG * s->kpt;
-	P.hist_cols = ceil_div(s->cap, P.sort_tile);
-	/* Grids: enough groups to fill every CU several times, never more than the
-	 * tiles of a full-capacity population; all loops are grid-strided. */
-	const uint32_t cu = (uint32_t)s->num_cu;
-	s->grid_move = min_u32(ceil_div(s->cap, WG * M_IPT), 8 * cu);
-	s->grid_up = min_u32(P.hist_cols, 8 * cu);
-	s->grid_down = min_u32(P.hist_cols, (s->rbits == 11 ? 1 : 2) * cu);
-	s->grid_act = min_u32(ceil_div(s->cap, A_TILE), 8 * cu);
-	s->grid_emit = min_u32(ceil_div(s->cap, A_TILE), 8 * cu);
-	s->grid_cells = min_u32(ceil_div(C, WG), 8 * cu);
-	s->grid_agents = min_u32(ceil_div(n0 ? n0 : 1, WG), 8 * cu);
-
-	int rc = sim_alloc(s);
-	if (rc != PP_SUCCESS) {
-		sim_free(s);
-		return rc;
-	}
 	*out = s;
 	return PP_SUCCESS;
 }
 
-void pp_sim_destroy(pp_sim_t *s) { sim_free(s); }
+void pp_sim_destroy(pp_sim_t *s) {
+	if (!s) return;
+	batch_free(s->b);
+	delete s;
+}
+
+static Replica *sim_rep(const pp_sim_t *s) {
+	if (!s || !s->initialised) return nullptr;
+	auto it = s->b->reps.find(s->rid);
+	return it == s->b->reps.end() ? nullptr : &it->second;
+}
 
 int pp_sim_init(pp_sim_t *s, pp_stats64_t *stats0) {
 	if (!s) return PP_INVALID_ARGS;
-	HIP_OK(hipSetDevice(s->device));
-	PendingEvent pe;
-	prof_begin(s, KI_INIT_RESET, pe);
-	k_init_reset<<<min_u32(ceil_div(s->ring_n + s->hist_words, WG), 2048), WG, 0,
-		s->stream>>>(s->ctl, s->ring, s->ring_n, s->hist, s->hist_words);
-	prof_end(s, pe);
-	prof_begin(s, KI_INIT_CELLS, pe);
-	if (s->grass_bytes == 1)
-		k_init_cells<uint8_t><<<s->grid_cells, WG, 0, s->stream>>>(
-			(uint8_t *)s->grass_at, s->ring, s->ctl, s->P);
-	else if (s->grass_bytes == 2)
-		k_init_cells<uint16_t><<<s->grid_cells, WG, 0, s->stream>>>(
-			(uint16_t *)s->grass_at, s->ring, s->ctl, s->P);
-	else
-		k_init_cells<uint32_t><<<s->grid_cells, WG, 0, s->stream>>>(
-			(uint32_t *)s->grass_at, s->ring, s->ctl, s->P);
-	prof_end(s, pe);
-	prof_begin(s, KI_INIT_AGENTS, pe);
-	k_init_agents<<<s->grid_agents, WG, 0, s->stream>>>(s->cell, s->energy,
-		s->type, s->ctl, s->P);
-	prof_end(s, pe);
-	prof_begin(s, KI_INIT_DONE, pe);
-	k_init_done<<<1, 64, 0, s->stream>>>(s->ctl, s->stats_dev, s->P);
-	prof_end(s, pe);
-	prof_begin(s, KI_MOVE0, pe);
-	k_move0<<<s->grid_move, WG, 0, s->stream>>>(s->cell, s->energy, s->type,
-		s->keys[0], s->vals[0], s->hist, s->ctl, s->P);
-	prof_end(s, pe);
-	HIP_OK(hipGetLastError());
-	HIP_OK(hipMemcpyAsync(s->stats_host, s->stats_dev, sizeof(pp_stats64_t),
-		hipMemcpyDeviceToHost, s->stream));
-	int rc = read_ctl(s);
-	if (rc) return rc;
-	if (s->ctl_host->error) return s->ctl_host->error;
-	if (stats0) *stats0 = s->stats_host[0];
+	s->initialised = 0;
+	PP_TRY(batch_reset(s->b));
+	uint32_t id = 0;
+	PP_TRY(admit(s->b, &s->prm, s->seed, s->cap, ITERS_UNLIMITED, 0, &id));
+	s->rid = id;
 	s->initialised = 1;
-	return PP_SUCCESS;
-}
-
-/* Capture `steps` iterations into one executable graph (kernels only; no
- * memset or copy nodes). */
-static int build_graph(pp_sim_t *s, int steps) {
-	hipGraph_t g = nullptr;
-	HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-	int rc = PP_SUCCESS;
-	for (int i = 0; i < steps && rc == PP_SUCCESS; i++) rc = enqueue_step(s);
-	hipError_t e = hipStreamEndCapture(s->stream, &g);
-	if (rc != PP_SUCCESS) {
-		if (g) (void)hipGraphDestroy(g);
-		return rc;
-	}
-	HIP_OK(e);
-	e = hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0);
-	(void)hipGraphDestroy(g);
-	HIP_OK(e);
-	s->graph_steps = steps;
-	return PP_SUCCESS;
-}
-
-static int copy_rows(pp_sim_t *s, uint32_t first_row, uint32_t nrows) {
-	/* rows first_row .. first_row + nrows - 1 of the ring (may wrap) */
-	uint32_t a = first_row % STATS_RING;
-	uint32_t n1 = min_u32(nrows, STATS_RING - a);
-	HIP_OK(hipMemcpyAsync(s->stats_host + a, s->stats_dev + a,
-		(size_t)n1 * sizeof(pp_stats64_t), hipMemcpyDeviceToHost, s->stream));
-	if (nrows > n1)
-		HIP_OK(hipMemcpyAsync(s->stats_host, s->stats_dev,
-			(size_t)(nrows - n1) * sizeof(pp_stats64_t), hipMemcpyDeviceToHost,
-			s->stream));
+	Replica *r = sim_rep(s);
+	r->sink = stats0;
+	r->sink_from = 0;
+	const int rc = collect(s->b);
+	r->sink = nullptr;
+	if (rc) return rc;
+	if (s->b->ctl_host->error) return s->b->ctl_host->error;
 	return PP_SUCCESS;
 }
 
 int pp_sim_run(pp_sim_t *s, uint32_t iters, pp_stats64_t *out) {
-	if (!s || !s->initialised) return PP_INVALID_ARGS;
-	HIP_OK(hipSetDevice(s->device));
-	if (s->ctl_host->error) return s->ctl_host->error;
-	const uint64_t last = (uint64_t)s->ctl_host->step + iters + s->P.R + 1;
+	Replica *r = sim_rep(s);
+	if (!r) return PP_INVALID_ARGS;
+	pp_batch_t *b = s->b;
+	HIP_OK(hipSetDevice(b->device));
+	if (b->ctl_host->error) return b->ctl_host->error;
+	if (r->error) return r->error;
+	const uint64_t last = (uint64_t)b->ctl_host->tick + iters + b->P.R + 1;
 	if (last >= 0xFFFFFFFFull) return PP_INVALID_ARGS;
-	const int use_graph = !s->profiling && getenv("PPHPC_NO_GRAPH") == nullptr;
 	uint32_t done = 0;
 	while (done < iters) {
-		const uint32_t chunk = min_u32(iters - done, STATS_RING / 2);
-		const uint32_t step0 = s->ctl_host->step;
-		uint32_t i = 0;
-		if (use_graph && chunk >= 8) {
-			if (!s->graph_exec) {
-				int rc = build_graph(s, 8);
-				if (rc) return rc;
-			}
-			for (; i + (uint32_t)s->graph_steps <= chunk; i += s->graph_steps)
-				HIP_OK(hipGraphLaunch(s->graph_exec, s->stream));
-		}
-		for (; i < chunk; i++) {
-			int rc = enqueue_step(s);
-			if (rc) return rc;
-		}
-		int rc = copy_rows(s, step0 + 1, chunk);
+		const uint32_t chunk = min_u32(iters - done, b->stats_rows / 2);
+		const uint32_t step0 = r->rows_have - 1;
+		r->sink = out ? out + done : nullptr;
+		r->sink_from = step0 + 1;
+		int rc = enqueue_ticks(b, chunk);
+		if (!rc) rc = collect(b);
+		r->sink = nullptr;
 		if (rc) return rc;
-		rc = read_ctl(s);
-		if (rc) return rc;
-		const uint32_t got = s->ctl_host->step - step0;
-		if (out)
-			for (uint32_t r = 0; r < got; r++)
-				out[done + r] = s->stats_host[(step0 + 1 + r) % STATS_RING];
+		const uint32_t got = r->rows_have - 1 - step0;
 		done += got;
-		if (s->ctl_host->error) return s->ctl_host->error;
+		if (b->ctl_host->error) return b->ctl_host->error;
+		if (r->error) return r->error;
 		if (got != chunk) return PP_LIBRARY_ERROR;
 	}
 	return PP_SUCCESS;
@@ -1617,74 +2411,56 @@ int pp_sim_step(pp_sim_t *s, pp_stats64_t *out) {
 
 int pp_sim_simulate(pp_sim_t *s, pp_stats64_t *out) {
 	if (!s) return PP_INVALID_ARGS;
-	int rc = pp_sim_init(s, out);
-	if (rc) return rc;
+	PP_TRY(pp_sim_init(s, out));
 	return pp_sim_run(s, s->prm.iters, out ? out + 1 : nullptr);
 }
 
 uint32_t pp_sim_step_count(const pp_sim_t *s) {
-	return s ? s->ctl_host->step : 0;
+	const Replica *r = sim_rep(s);
+	return r ? r->rows_have - 1 : 0;
 }
 
 uint64_t pp_sim_num_agents(const pp_sim_t *s) {
-	return s ? s->ctl_host->n : 0;
+	const Replica *r = sim_rep(s);
+	return r ? r->n : 0;
 }
 
 uint32_t pp_sim_error_step(const pp_sim_t *s) {
-	return (s && s->ctl_host->error) ? s->ctl_host->err_step : 0xFFFFFFFFu;
+	if (!s) return 0xFFFFFFFFu;
+	if (s->b->ctl_host->error) return s->b->ctl_host->err_step;
+	const Replica *r = sim_rep(s);
+	return (r && r->error) ? r->err_step : 0xFFFFFFFFu;
 }
 
 uint64_t pp_sim_agent_steps(const pp_sim_t *s) {
-	return s ? s->ctl_host->agent_steps : 0;
+	return s ? s->b->ctl_host->agent_steps : 0;
 }
 
 uint64_t pp_sim_capacity(const pp_sim_t *s) { return s ? s->cap : 0; }
 
 int pp_sim_counters(const pp_sim_t *s, uint64_t out[5]) {
 	if (!s || !out) return PP_INVALID_ARGS;
-	out[0] = s->ctl_host->agent_steps;
-	out[1] = s->ctl_host->sum_live;
-	out[2] = s->ctl_host->sum_out;
-	out[3] = s->ctl_host->sum_occ;
-	out[4] = s->ctl_host->sum_eaten;
+	const Ctl *c = s->b->ctl_host;
+	out[0] = c->agent_steps;
+	out[1] = c->sum_live;
+	out[2] = c->sum_out;
+	out[3] = c->sum_occ;
+	out[4] = c->sum_eaten;
 	return PP_SUCCESS;
 }
 
 int pp_sim_get_state(pp_sim_t *s, uint32_t *cell, int32_t *energy,
 	uint8_t *type, uint32_t *grass) {
-	if (!s || !s->initialised) return PP_INVALID_ARGS;
-	HIP_OK(hipSetDevice(s->device));
-	int rc = read_ctl(s);
-	if (rc) return rc;
-	const size_t n = s->ctl_host->n;
-	if (cell && n) HIP_OK(hipMemcpy(cell, s->cell, n * 4, hipMemcpyDeviceToHost));
-	if (energy && n)
-		HIP_OK(hipMemcpy(energy, s->energy, n * 4, hipMemcpyDeviceToHost));
-	if (type && n) HIP_OK(hipMemcpy(type, s->type, n, hipMemcpyDeviceToHost));
-	if (grass) {
-		/* Countdown at state st from the regrow step a stored modulo 2^w:
-		 * d = (a - st) mod 2^w is the countdown if 1 <= d <= R, else 0. */
-		const uint32_t st = s->ctl_host->step, R = s->P.R, C = s->P.C;
-		const size_t gb = s->grass_bytes;
-		std::vector<uint8_t> raw((size_t)C * gb);
-		HIP_OK(hipMemcpy(raw.data(), s->grass_at, raw.size(),
-			hipMemcpyDeviceToHost));
-		const uint32_t mask = gb == 4 ? 0xFFFFFFFFu : ((1u << (8 * gb)) - 1u);
-		for (uint32_t c = 0; c < C; c++) {
-			uint32_t a;
-			if (gb == 1) a = raw[c];
-			else if (gb == 2) a = reinterpret_cast<const uint16_t *>(raw.data())[c];
-			else a = reinterpret_cast<const uint32_t *>(raw.data())[c];
-			const uint32_t d = (a - st) & mask;
-			grass[c] = (d >= 1 && d <= R) ? d : 0;
-		}
-	}
-	return PP_SUCCESS;
+	Replica *r = sim_rep(s);
+	if (!r) return PP_INVALID_ARGS;
+	HIP_OK(hipSetDevice(s->b->device));
+	PP_TRY(read_ctl(s->b));
+	return state_of(s->b, *r, cell, energy, type, grass);
 }
 
 int pp_sim_set_profiling(pp_sim_t *s, int enable) {
 	if (!s) return PP_INVALID_ARGS;
-	s->profiling = enable ? 1 : 0;
+	s->b->profiling = enable ? 1 : 0;
 	return PP_SUCCESS;
 }
 
@@ -1692,53 +2468,226 @@ int pp_sim_prof_count(const pp_sim_t *s) { return s ? KI_COUNT : 0; }
 
 int pp_sim_prof_get(pp_sim_t *s, int i, const char **name, uint64_t *launches,
 	double *total_ns) {
-	if (!s || i < 0 || i >= KI_COUNT) return PP_INVALID_ARGS;
-	int rc = prof_collect(s);
-	if (rc) return rc;
-	if (name) *name = kernel_names[i];
-	if (launches) *launches = s->prof_launches[i];
-	if (total_ns) *total_ns = s->prof_ns[i];
-	return PP_SUCCESS;
+	if (!s) return PP_INVALID_ARGS;
+	return prof_get(s->b, i, name, launches, total_ns);
 }
 
 int pp_sim_prof_reset(pp_sim_t *s) {
 	if (!s) return PP_INVALID_ARGS;
-	int rc = prof_collect(s);
-	memset(s->prof_launches, 0, sizeof(s->prof_launches));
-	memset(s->prof_ns, 0, sizeof(s->prof_ns));
-	return rc;
+	return pp_batch_prof_reset(s->b);
 }
 
 int pp_sim_prof_export(pp_sim_t *s, const char *path) {
 	if (!s || !path) return PP_INVALID_ARGS;
-	int rc = prof_collect(s);
-	if (rc) return rc;
-	FILE *fp = fopen(path, "w");
-	if (!fp) return PP_UNABLE_SAVE_STATS;
-	double total = 0;
-	for (int i = 0; i < KI_COUNT; i++) total += s->prof_ns[i];
-	for (int i = 0; i < KI_COUNT; i++)
-		fprintf(fp, "\"%s\"\t%llu\t%lg\n", kernel_names[i],
-			(unsigned long long)s->prof_ns[i],
-			total > 0 ? s->prof_ns[i] / total : 0.0);
-	fclose(fp);
-	return PP_SUCCESS;
+	return prof_export(s->b, path);
 }
 
 int pp_sim_info(const pp_sim_t *s, uint32_t *sort_passes, uint32_t *sort_tile,
 	uint32_t *action_tile, uint32_t *grid_move) {
 	if (!s) return PP_INVALID_ARGS;
-	if (sort_passes) *sort_passes = s->P.passes;
-	if (sort_tile) *sort_tile = s->P.sort_tile;
-	if (action_tile) *action_tile = A_TILE;
-	if (grid_move) *grid_move = s->grid_move;
-	return PP_SUCCESS;
+	return pp_batch_info(s->b, sort_passes, sort_tile, action_tile, grid_move);
 }
 
 int pp_sim_sync(pp_sim_t *s) {
 	if (!s) return PP_INVALID_ARGS;
-	HIP_OK(hipSetDevice(s->device));
-	HIP_OK(hipStreamSynchronize(s->stream));
+	return pp_batch_sync(s->b);
+}
+
+/* ---------------------------------------------------------------- batches */
+
+int pp_batch_create(uint64_t agent_capacity, uint64_t cell_capacity,
+	uint32_t max_replicas, uint32_t max_grass_restart, int device,
+	pp_batch_t **out) {
+	if (!out) return PP_INVALID_ARGS;
+	return batch_create(agent_capacity, cell_capacity, max_replicas,
+		max_grass_restart, device, out);
+}
+
+void pp_batch_destroy(pp_batch_t *b) { batch_free(b); }
+
+uint64_t pp_batch_footprint(uint64_t agent_capacity, uint64_t cell_capacity,
+	uint32_t max_replicas, uint32_t max_grass_restart) {
+	if (agent_capacity == 0 || cell_capacity == 0 || max_replicas == 0 ||
+		max_grass_restart == 0 || agent_capacity >= (1ull << 29) ||
+		cell_capacity > 0xFFFFFFFEull)
+		return 0;
+	return batch_bytes(agent_capacity, cell_capacity, max_replicas,
+		max_grass_restart, stats_rows_for(max_replicas),
+		max_hist_words(key_bits_for(cell_capacity), (uint32_t)agent_capacity));
+}
+
+int pp_batch_add(pp_batch_t *b, const pp_params_t *p, uint64_t seed,
+	uint64_t agent_reserve, uint32_t *id) {
+	if (!b || !p) return PP_INVALID_ARGS;
+	return admit(b, p, seed, agent_reserve, p->iters, 1, id);
+}
+
+int pp_batch_run(pp_batch_t *b, uint32_t ticks) {
+	if (!b) return PP_INVALID_ARGS;
+	return advance(b, ticks);
+}
+
+int pp_batch_replica(pp_batch_t *b, uint32_t id, int *state, uint32_t *steps,
+	int *error, uint64_t *agent_steps) {
+	if (!b) return PP_INVALID_ARGS;
+	auto it = b->reps.find(id);
+	if (it == b->reps.end()) return PP_INVALID_ARGS;
+	const Replica &r = it->second;
+	if (state) *state = (int)r.status;
+	if (steps) *steps = r.rows_have ? r.rows_have - 1 : 0;
+	if (error) *error = r.error;
+	if (agent_steps) *agent_steps = r.agent_steps;
+	return PP_SUCCESS;
+}
+
+int pp_batch_rows(pp_batch_t *b, uint32_t id, uint32_t first, uint32_t n,
+	pp_stats64_t *out) {
+	if (!b || (n && !out)) return PP_INVALID_ARGS;
+	auto it = b->reps.find(id);
+	if (it == b->reps.end()) return PP_INVALID_ARGS;
+	const Replica &r = it->second;
+	if ((uint64_t)first + n > r.rows_have) return PP_INVALID_ARGS;
+	for (uint32_t i = 0; i < n; i++) out[i] = r.rows[first + i];
+	return PP_SUCCESS;
+}
+
+int pp_batch_get_state(pp_batch_t *b, uint32_t id, uint32_t *n_agents,
+	uint32_t *cell, int32_t *energy, uint8_t *type, uint32_t *grass) {
+	if (!b) return PP_INVALID_ARGS;
+	auto it = b->reps.find(id);
+	if (it == b->reps.end()) return PP_INVALID_ARGS;
+	HIP_OK(hipSetDevice(b->device));
+	PP_TRY(read_ctl(b));
+	const Replica &r = it->second;
+	if (r.slot < 0) return PP_INVALID_ARGS;
+	if (n_agents) *n_agents = b->rctl_host[r.slot].n;
+	return state_of(b, r, cell, energy, type, grass);
+}
+
+int pp_batch_release(pp_batch_t *b, uint32_t id) {
+	if (!b) return PP_INVALID_ARGS;
+	auto it = b->reps.find(id);
+	if (it == b->reps.end() || it->second.slot >= 0) return PP_INVALID_ARGS;
+	b->reps.erase(it);
+	return PP_SUCCESS;
+}
+
+int pp_batch_resources(const pp_batch_t *b, uint64_t *agents_free,
+	uint32_t *cells_free_run, uint32_t *slots_free, uint32_t *active) {
+	if (!b) return PP_INVALID_ARGS;
+	if (agents_free) *agents_free = b->P.cap - b->reserved;
+	if (cells_free_run) {
+		uint32_t m = 0;
+		for (auto &kv : b->free_cells) m = std::max(m, kv.second);
+		*cells_free_run = m;
+	}
+	if (slots_free) {
+		uint32_t n = 0;
+		for (int64_t v : b->slot_rid) n += v < 0;
+		*slots_free = n;
+	}
+	if (active) *active = active_count(b);
+	return PP_SUCCESS;
+}
+
+uint32_t pp_batch_tick(const pp_batch_t *b) {
+	return b ? b->ctl_host->tick : 0;
+}
+
+int pp_batch_counters(const pp_batch_t *b, uint64_t out[5]) {
+	if (!b || !out) return PP_INVALID_ARGS;
+	const Ctl *c = b->ctl_host;
+	out[0] = c->agent_steps;
+	out[1] = c->sum_live;
+	out[2] = c->sum_out;
+	out[3] = c->sum_occ;
+	out[4] = c->sum_eaten;
+	return PP_SUCCESS;
+}
+
+int pp_batch_info(const pp_batch_t *b, uint32_t *sort_passes,
+	uint32_t *sort_tile, uint32_t *action_tile, uint32_t *grid_move) {
+	if (!b) return PP_INVALID_ARGS;
+	if (sort_passes) *sort_passes = b->P.passes;
+	if (sort_tile) *sort_tile = b->P.sort_tile;
+	if (action_tile) *action_tile = A_TILE;
+	if (grid_move) *grid_move = b->grid_emit;
+	return PP_SUCCESS;
+}
+
+int pp_batch_set_shape(pp_batch_t *b, uint32_t keys_per_thread,
+	uint32_t max_digit_bits) {
+	if (!b) return PP_INVALID_ARGS;
+	if (keys_per_thread != 8 && keys_per_thread != 16) return PP_INVALID_ARGS;
+	if (max_digit_bits != 8 && max_digit_bits != 11) return PP_INVALID_ARGS;
+	const SortShape sh{(int)keys_per_thread, (int)max_digit_bits};
+	if (hist_bytes_for(sh, b->key_bits, b->P.cap) > b->hist_alloc_words * 4)
+		return PP_INVALID_ARGS;
+	HIP_OK(hipSetDevice(b->device));
+	if (sh.kpt == b->shape.kpt && sh.max_bits == b->shape.max_bits)
+		return PP_SUCCESS;
+	apply_shape(b, sh);
+	/* The emit built the pass-0 histograms for the old shape. */
+	launch_hist0(b);
+	HIP_OK(hipGetLastError());
+	b->ctl_fresh = 0;
+	return PP_SUCCESS;
+}
+
+int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks, uint32_t *keys_per_thread,
+	uint32_t *max_digit_bits, double *sec_per_tick) {
+	if (!b || trial_ticks == 0 || trial_ticks > b->stats_rows / 2)
+		return PP_INVALID_ARGS;
+	HIP_OK(hipSetDevice(b->device));
+	const std::vector<SortShape> cands = tune_candidates(b->key_bits, b->P.cap);
+	SortShape best = b->shape;
+	double best_t = -1.0;
+	for (const SortShape &sh : cands) {
+		if (active_count(b) == 0) break;
+		PP_TRY(pp_batch_set_shape(b, (uint32_t)sh.kpt, (uint32_t)sh.max_bits));
+		/* one untimed tick builds the graph and warms the caches */
+		PP_TRY(enqueue_ticks(b, 8));
+		PP_TRY(collect(b));
+		double t = 0;
+		PP_TRY(time_ticks(b, trial_ticks, &t));
+		if (best_t < 0 || t < best_t) {
+			best_t = t;
+			best = sh;
+		}
+	}
+	PP_TRY(pp_batch_set_shape(b, (uint32_t)best.kpt, (uint32_t)best.max_bits));
+	if (keys_per_thread) *keys_per_thread = (uint32_t)best.kpt;
+	if (max_digit_bits) *max_digit_bits = (uint32_t)best.max_bits;
+	if (sec_per_tick) *sec_per_tick = best_t < 0 ? 0.0 : best_t / trial_ticks;
+	return b->ctl_host->error;
+}
+
+int pp_batch_set_profiling(pp_batch_t *b, int enable) {
+	if (!b) return PP_INVALID_ARGS;
+	b->profiling = enable ? 1 : 0;
+	return PP_SUCCESS;
+}
+
+int pp_batch_prof_count(const pp_batch_t *b) { return b ? KI_COUNT : 0; }
+
+int pp_batch_prof_get(pp_batch_t *b, int i, const char **name,
+	uint64_t *launches, double *total_ns) {
+	if (!b) return PP_INVALID_ARGS;
+	return prof_get(b, i, name, launches, total_ns);
+}
+
+int pp_batch_prof_reset(pp_batch_t *b) {
+	if (!b) return PP_INVALID_ARGS;
+	const int rc = prof_collect(b);
+	memset(b->prof_launches, 0, sizeof(b->prof_launches));
+	memset(b->prof_ns, 0, sizeof(b->prof_ns));
+	return rc;
+}
+
+int pp_batch_sync(pp_batch_t *b) {
+	if (!b) return PP_INVALID_ARGS;
+	HIP_OK(hipSetDevice(b->device));
+	HIP_OK(hipStreamSynchronize(b->stream));
 	return PP_SUCCESS;
 }
 

@@ -122,6 +122,27 @@ def _lib():
         "pp_sim_info": (i32, [vp, P(u32), P(u32), P(u32), P(u32)]),
         "pp_sim_destroy": (None, [vp]),
         "pp_sim_sync": (i32, [vp]),
+        "pp_batch_footprint": (u64, [u64, u64, u32, u32]),
+        "pp_batch_create": (i32, [u64, u64, u32, u32, i32, P(vp)]),
+        "pp_batch_add": (i32, [vp, P(Params), u64, u64, P(u32)]),
+        "pp_batch_run": (i32, [vp, u32]),
+        "pp_batch_replica": (i32, [vp, u32, P(i32), P(u32), P(i32), P(u64)]),
+        "pp_batch_rows": (i32, [vp, u32, u32, u32, P(Stats)]),
+        "pp_batch_get_state": (i32, [vp, u32, P(u32), vp, vp, vp, vp]),
+        "pp_batch_release": (i32, [vp, u32]),
+        "pp_batch_resources": (i32, [vp, P(u64), P(u32), P(u32), P(u32)]),
+        "pp_batch_tick": (u32, [vp]),
+        "pp_batch_counters": (i32, [vp, P(u64)]),
+        "pp_batch_info": (i32, [vp, P(u32), P(u32), P(u32), P(u32)]),
+        "pp_batch_set_shape": (i32, [vp, u32, u32]),
+        "pp_batch_tune": (i32, [vp, u32, P(u32), P(u32), P(ctypes.c_double)]),
+        "pp_batch_set_profiling": (i32, [vp, i32]),
+        "pp_batch_prof_get": (i32, [vp, i32, P(ctypes.c_char_p), P(u64),
+                                    P(ctypes.c_double)]),
+        "pp_batch_prof_reset": (i32, [vp]),
+        "pp_batch_prof_count": (i32, [vp]),
+        "pp_batch_sync": (i32, [vp]),
+        "pp_batch_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -313,3 +334,167 @@ class Simulation:
     def profile_export(self, path: str):
         self._check(_lib().pp_sim_prof_export(self._h, path.encode()),
                     "prof_export")
+
+
+# Replica states (PP_REPLICA_* of pphpc.h).
+RUNNING, DONE, FAILED = 1, 2, 3
+
+
+class Batch:
+    """Replicas of different grids, parameters, seeds and iteration counts
+    advancing together on one GPU (a ``pp_batch_t`` handle). A replica's
+    statistics rows are those of the same simulation run alone."""
+
+    def __init__(self, agent_capacity: int, cell_capacity: int,
+                 max_replicas: int, max_grass_restart: int, device: int = -1):
+        h = ctypes.c_void_p()
+        rc = _lib().pp_batch_create(agent_capacity, cell_capacity,
+                                    max_replicas, max_grass_restart, device,
+                                    ctypes.byref(h))
+        if rc:
+            raise PPError(rc, "pp_batch_create")
+        self._h = h
+        self.params = {}
+
+    @staticmethod
+    def footprint(agent_capacity: int, cell_capacity: int, max_replicas: int,
+                  max_grass_restart: int) -> int:
+        """Device bytes a batch with these limits allocates."""
+        return _lib().pp_batch_footprint(agent_capacity, cell_capacity,
+                                         max_replicas, max_grass_restart)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().pp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc:
+            raise PPError(rc, what)
+
+    def add(self, params: Params, seed: int, reserve: int = 0):
+        """Admit a replica; returns its id, or None when the batch has no
+        room for it now (PP_OUT_OF_RESOURCES)."""
+        rid = ctypes.c_uint32()
+        rc = _lib().pp_batch_add(self._h, ctypes.byref(params), seed, reserve,
+                                 ctypes.byref(rid))
+        if rc == -8:
+            return None
+        self._check(rc, "add")
+        self.params[rid.value] = params
+        return rid.value
+
+    def run(self, ticks: int):
+        self._check(_lib().pp_batch_run(self._h, ticks), f"run({ticks})")
+
+    def replica(self, rid: int) -> dict:
+        st, steps, err, ast = (ctypes.c_int(), ctypes.c_uint32(),
+                               ctypes.c_int(), ctypes.c_uint64())
+        self._check(_lib().pp_batch_replica(
+            self._h, rid, ctypes.byref(st), ctypes.byref(steps),
+            ctypes.byref(err), ctypes.byref(ast)), "replica")
+        return {"state": st.value, "steps": steps.value, "error": err.value,
+                "agent_steps": ast.value}
+
+    def rows(self, rid: int, first: int = 0, n: Optional[int] = None):
+        if n is None:
+            n = self.replica(rid)["steps"] + 1 - first
+        out = (Stats * n)()
+        self._check(_lib().pp_batch_rows(self._h, rid, first, n, out), "rows")
+        return list(out)
+
+    def rows_array(self, rid: int) -> np.ndarray:
+        return stats_rows_to_array(self.rows(rid))
+
+    def state(self, rid: int):
+        """(cell, energy, type, grass) of a running replica, cells local to
+        its grid."""
+        p = self.params[rid]
+        n = ctypes.c_uint32()
+        self._check(_lib().pp_batch_get_state(self._h, rid, ctypes.byref(n),
+                                              None, None, None, None), "state")
+        cell = np.empty(n.value, np.uint32)
+        energy = np.empty(n.value, np.int32)
+        typ = np.empty(n.value, np.uint8)
+        grass = np.empty(p.grid_x * p.grid_y, np.uint32)
+        self._check(_lib().pp_batch_get_state(
+            self._h, rid, ctypes.byref(n),
+            cell.ctypes.data_as(ctypes.c_void_p),
+            energy.ctypes.data_as(ctypes.c_void_p),
+            typ.ctypes.data_as(ctypes.c_void_p),
+            grass.ctypes.data_as(ctypes.c_void_p)), "state")
+        return cell, energy, typ, grass
+
+    def release(self, rid: int):
+        self._check(_lib().pp_batch_release(self._h, rid), "release")
+        self.params.pop(rid, None)
+
+    def resources(self) -> dict:
+        a, c, s, n = (ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32(),
+                      ctypes.c_uint32())
+        self._check(_lib().pp_batch_resources(
+            self._h, ctypes.byref(a), ctypes.byref(c), ctypes.byref(s),
+            ctypes.byref(n)), "resources")
+        return {"agents_free": a.value, "cells_free_run": c.value,
+                "slots_free": s.value, "active": n.value}
+
+    @property
+    def tick(self) -> int:
+        return _lib().pp_batch_tick(self._h)
+
+    def counters(self) -> dict:
+        out = (ctypes.c_uint64 * 5)()
+        self._check(_lib().pp_batch_counters(self._h, out), "counters")
+        return dict(zip(("agent_steps", "live", "out", "occupied", "eaten"),
+                        tuple(out)))
+
+    def info(self) -> dict:
+        a, b, c, d = (ctypes.c_uint32() for _ in range(4))
+        self._check(_lib().pp_batch_info(self._h, ctypes.byref(a),
+                                         ctypes.byref(b), ctypes.byref(c),
+                                         ctypes.byref(d)), "info")
+        return {"sort_passes": a.value, "sort_tile": b.value,
+                "action_tile": c.value, "grid_move": d.value}
+
+    def set_shape(self, keys_per_thread: int, max_digit_bits: int):
+        self._check(_lib().pp_batch_set_shape(self._h, keys_per_thread,
+                                              max_digit_bits), "set_shape")
+
+    def tune(self, trial_ticks: int = 32) -> dict:
+        k, b, t = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+        self._check(_lib().pp_batch_tune(self._h, trial_ticks, ctypes.byref(k),
+                                         ctypes.byref(b), ctypes.byref(t)),
+                    "tune")
+        return {"keys_per_thread": k.value, "max_digit_bits": b.value,
+                "sec_per_tick": t.value}
+
+    def set_profiling(self, enable: bool):
+        self._check(_lib().pp_batch_set_profiling(self._h, int(enable)),
+                    "set_profiling")
+
+    def profile(self) -> dict:
+        out = {}
+        for i in range(_lib().pp_batch_prof_count(self._h)):
+            name = ctypes.c_char_p()
+            launches = ctypes.c_uint64()
+            ns = ctypes.c_double()
+            self._check(_lib().pp_batch_prof_get(
+                self._h, i, ctypes.byref(name), ctypes.byref(launches),
+                ctypes.byref(ns)), "prof_get")
+            out[name.value.decode()] = (launches.value, ns.value)
+        return out
+
+    def sync(self):
+        self._check(_lib().pp_batch_sync(self._h), "sync")

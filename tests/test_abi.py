@@ -116,3 +116,24 @@ def test_cli_invalid_params_file(tmp_path):
     bad.write_text("INIT_SHEEP = 3\nINIT_SHEEP = 4\n")
     r = _pp(cwd=str(tmp_path))          # default -p config.txt
     assert r.returncode == pp.PP_INVALID_PARAMS_FILE & 0xFF
+
+
+@pytest.mark.parametrize("args", [
+    (0, 100, 4, 10), (1 << 29, 100, 4, 10), (100, 0, 4, 10),
+    (100, 1 << 32, 4, 10), (100, 100, 0, 10), (100, 100, 4, 0),
+])
+def test_batch_create_rejects_invalid_limits(args):
+    """Checked on the host before any HIP call."""
+    with pytest.raises(pp.PPError) as e:
+        pp.Batch(*args)
+    assert e.value.code == pp.PP_INVALID_ARGS
+    assert pp.Batch.footprint(*args) == 0
+
+
+def test_batch_footprint_counts_every_buffer():
+    """33 bytes per agent (SoA 9 + three pair buffers 24), the grass (1 byte
+    per cell for a restart <= 191), rings, histograms and stats rings."""
+    A, C = 1_000_000, 4_000_000
+    f = pp.Batch.footprint(A, C, 8, 10)
+    assert 33 * A + C < f < 33 * A + C + 8 * 1024 * 56 + (40 << 20)
+    assert pp.Batch.footprint(A, C, 8, 300) - f == C + 8 * 290 * 8
