@@ -19,8 +19,19 @@ Also reported (one JSON line on stdout, progress on stderr):
   roofline_12800  the same for the 12800x12800 run (HBM-resident working set)
   cpu_baseline  the single-threaded C oracle on a bounded sample of the same
                 workload, on this host's cores
+  batched       64 replicas of grid sides drawn from {100 .. 3200}, v1 or v2s
+                parameters 50/50, seeds 1..64, 4000 iterations each: one
+                batch (one set of launches per iteration, sort shape from the
+                tuner) against the same 64 run one at a time as plain
+                simulations; replicas/hour both ways, and every statistics row
+                compared between the two
+  serving       a Poisson stream of requests (grid side log-uniform over
+                {100 .. 6400}, v1 or v2s, 1000-4000 iterations, random seeds)
+                served by continuous batching with admission by device
+                footprint: p50/p99 completion latency and agent-steps/s
 """
 import argparse
+import collections
 import json
 import os
 import subprocess
@@ -173,6 +184,81 @@ def cpu_baseline(params, capacity, sample_steps):
                       f"{float(kv['seconds']):.1f} s"}
 
 
+def batched_leg(n, iters, draw_key=4, tune_ticks=16):
+    """The batched-replica workload, batched and then plain, same GPU."""
+    import numpy as np
+    from pphpc_amd import Batch, Params, Simulation, stats_rows_to_array
+    from pphpc_amd.workloads import batch_replicas, reserve_for
+    reps = batch_replicas(n, draw_key, iters)
+    cells = sum(L * L for _, L, _, _ in reps)
+    agents = sum(reserve_for(d) for _, _, d, _ in reps)
+    log(f"batched: {n} replicas, {cells} cells, {agents} reserved agents")
+    t0 = time.perf_counter()
+    b = Batch(agents, cells, n, 10, device=0)
+    ids = [b.add(Params.from_values(**d), seed, reserve_for(d))
+           for _, _, d, seed in reps]
+    tuned = b.tune(tune_ticks) if tune_ticks else None
+    while b.resources()["active"]:
+        b.run(512)
+    b.sync()
+    t1 = time.perf_counter()
+    batched_s = t1 - t0
+    agent_steps = b.counters()["agent_steps"]
+    info = b.info()
+    rows_b = [b.rows_array(r) for r in ids]
+    states = [b.replica(r)["state"] for r in ids]
+    b.close()
+    log(f"batched: {batched_s:.2f} s ({agent_steps / batched_s:.3e} "
+        f"agent-steps/s), shape {tuned}")
+    plain_s = 0.0
+    equal = True
+    for (kind, L, d, seed), rb in zip(reps, rows_b):
+        t2 = time.perf_counter()
+        sim = Simulation(Params.from_values(**d), seed=seed,
+                         capacity=reserve_for(d), device=0)
+        rows = sim.simulate()
+        sim.close()
+        plain_s += time.perf_counter() - t2
+        equal &= bool(np.array_equal(stats_rows_to_array(rows), rb))
+    log(f"plain: {plain_s:.2f} s, rows equal: {equal}")
+    return {
+        "replicas": n, "iters": iters, "cells": cells,
+        "sides": sorted(collections.Counter(L for _, L, _, _ in reps).items()),
+        "kinds": dict(collections.Counter(k for k, _, _, _ in reps)),
+        "seeds": "1..%d" % n,
+        "batched_s": round(batched_s, 3), "plain_s": round(plain_s, 3),
+        "replicas_per_hour_batched": round(n * 3600 / batched_s, 1),
+        "replicas_per_hour_plain": round(n * 3600 / plain_s, 1),
+        "speedup": round(plain_s / batched_s, 3),
+        "agent_steps": int(agent_steps),
+        "agent_steps_per_s_batched": agent_steps / batched_s,
+        "agent_steps_per_s_plain": agent_steps / plain_s,
+        "all_done": all(st == 2 for st in states),
+        "rows_equal_plain": equal,
+        "sort_shape": tuned, "sort_passes": info["sort_passes"],
+        "note": "wall time from batch creation to the last replica's "
+                "completion vs the sum of create+simulate+destroy of the "
+                "plain runs; the tuner's trial iterations count as progress",
+    }
+
+
+def serving_leg(seconds, rate, hbm_gb, chunk, draw_key=5):
+    from pphpc_amd.serve import Server
+    from pphpc_amd.workloads import serve_requests
+    stream = serve_requests(rate, seconds, draw_key)
+    log(f"serving: {len(stream)} requests over {seconds} s at {rate}/s")
+    with Server(int(hbm_gb * (1 << 30)), chunk=chunk) as srv:
+        reqs, summary = srv.run(stream, log=log)
+    summary.update({"arrival_rate_per_s": rate, "window_s": seconds,
+                    "chunk_ticks": chunk, "hbm_budget_gib": hbm_gb,
+                    "sides": "log-uniform over 100..6400",
+                    "iters": "uniform 1000..4000"})
+    log(f"serving: p50 {summary['p50_latency_s']} s, p99 "
+        f"{summary['p99_latency_s']} s, {summary['agent_steps_per_s']:.3e} "
+        "agent-steps/s")
+    return summary
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +271,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-big", action="store_true")
+    ap.add_argument("--batch-replicas", type=int, default=64)
+    ap.add_argument("--batch-iters", type=int, default=4000)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-rate", type=float, default=3.0)
+    ap.add_argument("--serve-hbm-gb", type=float, default=48.0)
+    ap.add_argument("--serve-chunk", type=int, default=32)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -236,6 +330,15 @@ def main():
         log(f"12800^2 dominant kernel {roof_big['kernel']}: "
             f"{roof_big['achieved']} GB/s")
 
+    batched = None
+    if not args.no_batch:
+        batched = batched_leg(args.batch_replicas, args.batch_iters)
+
+    serving = None
+    if not args.no_serve:
+        serving = serving_leg(args.serve_seconds, args.serve_rate,
+                              args.serve_hbm_gb, args.serve_chunk)
+
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: oracle, {args.cpu_steps} iterations")
@@ -270,6 +373,8 @@ def main():
         },
         "roofline": roof,
         "roofline_12800": roof_big,
+        "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
