@@ -275,7 +275,7 @@ def main():
     ap.add_argument("--batch-iters", type=int, default=4000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
-    ap.add_argument("--serve-rate", type=float, default=3.0)
+    ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-hbm-gb", type=float, default=48.0)
     ap.add_argument("--serve-chunk", type=int, default=32)
     ap.add_argument("--no-serve", action="store_true")

@@ -216,6 +216,10 @@ def statref_lib():
         L.ppr_run.restype = ctypes.c_int
         L.ppr_run.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
                               ctypes.c_uint32, ctypes.c_void_p]
+        L.ppr_run_cells.restype = ctypes.c_int
+        L.ppr_run_cells.argtypes = [ctypes.POINTER(ctypes.c_uint32),
+                                    ctypes.c_uint32, ctypes.c_uint32,
+                                    ctypes.c_void_p]
         L.ppr_mt_outputs.restype = None
         L.ppr_mt_outputs.argtypes = [ctypes.c_uint32, ctypes.c_uint32,
                                      ctypes.c_void_p]
@@ -223,8 +227,12 @@ def statref_lib():
     return _SREF
 
 
-def statref_run(params, seed, iters=None):
-    """(iters+1, 7) uint64 rows of one NetLogo-rules replica (MT19937)."""
+def statref_run(params, seed, iters=None, rules="cells"):
+    """(iters+1, 7) uint64 rows of one replica of the independent simulator
+    (MT19937): rules "cells" = the Java single-thread semantics (per-cell
+    shuffled acting order, a wolf takes the first live sheep of the list),
+    "netlogo" = the NetLogo rules (global random order, a wolf takes a
+    uniformly random sheep)."""
     p = params_from(params)
     it = p.iters if iters is None else iters
     arr = (ctypes.c_uint32 * 11)(p.init_sheep, p.sheep_gain_from_food,
@@ -235,8 +243,9 @@ def statref_run(params, seed, iters=None):
                                  p.wolves_reproduce_prob, p.grass_restart,
                                  p.grid_x, p.grid_y)
     rows = np.zeros((it + 1, 7), np.uint64)
-    rc = statref_lib().ppr_run(arr, seed, it,
-                               rows.ctypes.data_as(ctypes.c_void_p))
+    fn = {"cells": statref_lib().ppr_run_cells,
+          "netlogo": statref_lib().ppr_run}[rules]
+    rc = fn(arr, seed, it, rows.ctypes.data_as(ctypes.c_void_p))
     if rc:
         raise ValueError("statref rejected the parameters")
     return rows

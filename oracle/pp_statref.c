@@ -1,28 +1,46 @@
 /*
- * pp_statref.c -- independent CPU simulation of the PPHPC model used as the
+ * pp_statref.c -- independent CPU simulations of the PPHPC model used as the
  * "reference side" of the statistical-equivalence tests.
  *
- * TEST INFRASTRUCTURE ONLY. It shares nothing with the oracle or the GPU
- * implementation except the model rules: it follows the canonical NetLogo
- * specification (netlogo/pphpc.nlogo:18-159) step by step, with its own
- * random number generator and its own orderings:
- *   - RNG: MT19937 (Matsumoto & Nishimura 1998, init_genrand seeding), bounded
- *     draws by rejection (uniform, unlike the multiply-high map of the step
- *     under test);
- *   - go (pphpc.nlogo:63-98): all turtles move (random 5: directions 0..3 are
- *     headings 0/90/180/270 and fd 1, 4 stays; pphpc.nlogo:116-122), lose one
- *     energy and die below 1; all patches grow grass (pphpc.nlogo:151-159);
- *     then ALL turtles act in one global random order (ask turtles): sheep eat
- *     green grass (pphpc.nlogo:124-131), wolves catch ONE RANDOM sheep of the
- *     patch that is not newborn (who <= maxwho, pphpc.nlogo:144-149); both try
- *     to reproduce (pphpc.nlogo:133-142); newborns are not asked this tick;
- *   - setup (pphpc.nlogo:18-60): patches alive with probability 1/2, else
- *     countdown 1 + random R; sheep then wolves at random patches with energy
- *     1 + random (2 * gain).
- * The step under test orders actions per cell (Java Cell.agentActions,
- * Cell.java:236-259) and lets a wolf take the first live sheep of the
- * shuffled cell list (Wolf.java:66-92); both are equal in distribution to the
- * NetLogo choices restated here, which is what the statistical tests check.
+ * TEST INFRASTRUCTURE ONLY. They share nothing with the oracle or the GPU
+ * implementation except the model rules, and use their own random number
+ * generator and their own orderings: MT19937 (Matsumoto & Nishimura 1998,
+ * init_genrand seeding), bounded draws by rejection (uniform, unlike the
+ * multiply-high map of the step under test). Two rule sets:
+ *
+ * ppr_run_cells -- the Java single-thread semantics (SimWorker.java:183-232,
+ *   Cell.java:175-308, AbstractAgent.java:91-167, Sheep.java:66-83,
+ *   Wolf.java:66-92), the semantics the step under test follows:
+ *   - move: cells in index order, each cell's agents in list order lose one
+ *     energy, those still above 0 draw one of 5 moves (stay, up, right,
+ *     down, left) and are appended to the destination cell's list;
+ *     then every positive grass countdown drops by 1;
+ *   - act: cells in index order; the cell's list is shuffled (Fisher-Yates
+ *     from the last position, as Collections.shuffle); each live agent in
+ *     that order eats (a sheep the live grass; a wolf the FIRST live sheep
+ *     of the shuffled list, possibly one that has already acted) and, if its
+ *     energy exceeds the threshold, draws U[0,100) against the reproduction
+ *     probability; a child gets floor(e/2) and joins the cell's newborns;
+ *   - next state per cell: the live agents in shuffled order, then the
+ *     newborns. The grid is a torus in both directions (the Java space wraps
+ *     its right/left neighbours across rows, VonNeumann2DTorusSpace.java:
+ *     73-76; NetLogo and the OpenCL kernels use a torus, as here).
+ *
+ * ppr_run -- the NetLogo rules (netlogo/pphpc.nlogo:18-159): all turtles
+ *   move (random 5: headings 0/90/180/270 and fd 1, 4 stays;
+ *   pphpc.nlogo:116-122), lose one energy and die below 1; all patches grow
+ *   grass (pphpc.nlogo:151-159); then ALL turtles act in one global random
+ *   order (ask turtles): sheep eat green grass (pphpc.nlogo:124-131), wolves
+ *   catch ONE RANDOM sheep of the patch that is not newborn (who <= maxwho,
+ *   pphpc.nlogo:144-149); both try to reproduce (pphpc.nlogo:133-142);
+ *   newborns are not asked this tick.
+ *
+ * The two differ in the wolf's choice of prey: the Java wolf takes the first
+ * live sheep of the shuffled list, so a sheep that acted before the wolf is
+ * a likelier victim than one still to act; the NetLogo wolf draws uniformly.
+ * Both share setup (pphpc.nlogo:18-60, CellGrassInitCoinRandCounter.java:
+ * 49-67): patches alive with probability 1/2, else countdown 1 + random R;
+ * sheep then wolves at random patches with energy 1 + random (2 * gain).
  */
 #include <stdint.h>
 #include <stdio.h>
@@ -273,6 +291,179 @@ int ppr_run(const uint32_t params[11], uint32_t seed, uint32_t iters,
 		gather(&w, rows + 7 * (size_t)t);
 	}
 	destroy(&w);
+	return 0;
+}
+
+/* ------------------------------------------------- Java single-thread rules */
+
+typedef struct {
+	int32_t energy;
+	uint8_t wolf;
+} jagent;
+
+static void *xalloc(size_t n, size_t sz) { return xrealloc(NULL, n * sz); }
+
+/* Runs setup + iters steps of the Java single-thread semantics; rows as
+ * ppr_run. The state is kept in cell order: agents of cell 0, then cell 1,
+ * ..., each cell's agents in its list order (cnt[c] of them). */
+int ppr_run_cells(const uint32_t params[11], uint32_t seed, uint32_t iters,
+	uint64_t *rows) {
+	ppr_params p;
+	memcpy(&p, params, sizeof(p));
+	if (p.gx == 0 || p.gy == 0 || p.grass_restart == 0) return -2;
+	const size_t C = (size_t)p.gx * p.gy;
+	mt19937 rng;
+	mt_seed(&rng, seed);
+	int32_t *grass = xalloc(C, sizeof(int32_t));
+	for (size_t c = 0; c < C; c++)
+		grass[c] = mt_below(&rng, 2) == 1
+			? 1 + (int32_t)mt_below(&rng, p.grass_restart) : 0;
+	/* initial agents in creation order per cell */
+	size_t n = (size_t)p.init_sheep + p.init_wolves, cap = n ? 2 * n : 16;
+	jagent *ag = xalloc(cap, sizeof(jagent));
+	uint32_t *acell = xalloc(cap, sizeof(uint32_t));
+	for (size_t i = 0; i < n; i++) {
+		const int wolf = i >= p.init_sheep;
+		const uint32_t gain = wolf ? p.wolves_gain : p.sheep_gain;
+		ag[i].energy = 1 + (int32_t)mt_below(&rng, 2 * gain);
+		ag[i].wolf = (uint8_t)wolf;
+		const uint32_t x = mt_below(&rng, p.gx), y = mt_below(&rng, p.gy);
+		acell[i] = y * p.gx + x;
+	}
+	uint32_t *cnt = xalloc(C + 1, sizeof(uint32_t));
+	uint32_t *start = xalloc(C + 1, sizeof(uint32_t));
+	/* order the initial agents by cell, keeping creation order (the cell
+	 * lists of putInitAgent) */
+	jagent *cur = xalloc(cap, sizeof(jagent));
+	memset(cnt, 0, (C + 1) * sizeof(uint32_t));
+	for (size_t i = 0; i < n; i++) cnt[acell[i]]++;
+	start[0] = 0;
+	for (size_t c = 0; c < C; c++) start[c + 1] = start[c] + cnt[c];
+	for (size_t i = 0; i < n; i++) cur[start[acell[i]]++] = ag[i];
+	start[0] = 0;
+	for (size_t c = 0; c < C; c++) start[c + 1] = start[c] + cnt[c];
+	jagent *mov = NULL, *kids = NULL, *nxt = NULL;
+	uint32_t *mcell = NULL, *mcnt = xalloc(C + 1, sizeof(uint32_t));
+	size_t mcap = 0, kcap = 0, ncap = 0;
+
+	for (uint32_t t = 0; t <= iters; t++) {
+		/* statistics of the state (cell order) */
+		uint64_t *row = rows + 7 * (size_t)t;
+		memset(row, 0, 7 * sizeof(uint64_t));
+		for (size_t i = 0; i < n; i++) {
+			if (cur[i].wolf) { row[1]++; row[4] += (uint64_t)cur[i].energy; }
+			else { row[0]++; row[3] += (uint64_t)cur[i].energy; }
+		}
+		for (size_t c = 0; c < C; c++) {
+			if (grass[c] <= 0) row[2]++;
+			else row[5] += (uint64_t)grass[c];
+		}
+		if (t == iters) break;
+		/* 1. move, in cell order and list order */
+		if (mcap < n) {
+			mcap = n;
+			mov = xrealloc(mov, mcap * sizeof(jagent));
+			mcell = xrealloc(mcell, mcap * sizeof(uint32_t));
+		}
+		size_t m = 0;
+		for (size_t c = 0; c < C; c++) {
+			const uint32_t x = (uint32_t)(c % p.gx), y = (uint32_t)(c / p.gx);
+			for (uint32_t i = start[c]; i < start[c + 1]; i++) {
+				jagent a = cur[i];
+				a.energy -= 1;
+				if (a.energy <= 0) continue;
+				uint32_t nx = x, ny = y;
+				switch (mt_below(&rng, 5)) {
+				case 1: ny = (y + p.gy - 1) % p.gy; break;   /* up */
+				case 2: nx = (x + 1) % p.gx; break;          /* right */
+				case 3: ny = (y + 1) % p.gy; break;          /* down */
+				case 4: nx = (x + p.gx - 1) % p.gx; break;   /* left */
+				default: break;                               /* stay */
+				}
+				mov[m] = a;
+				mcell[m] = ny * p.gx + nx;
+				m++;
+			}
+		}
+		/* 2. grass */
+		for (size_t c = 0; c < C; c++)
+			if (grass[c] > 0) grass[c]--;
+		/* destination lists in append order (stable by cell) */
+		memset(mcnt, 0, (C + 1) * sizeof(uint32_t));
+		for (size_t i = 0; i < m; i++) mcnt[mcell[i]]++;
+		uint32_t run = 0;
+		for (size_t c = 0; c < C; c++) {
+			const uint32_t k = mcnt[c];
+			mcnt[c] = run;
+			run += k;
+		}
+		mcnt[C] = run;
+		if (ncap < m) {
+			ncap = m;
+			nxt = xrealloc(nxt, ncap * sizeof(jagent));
+		}
+		for (size_t i = 0; i < m; i++) nxt[mcnt[mcell[i]]++] = mov[i];
+		for (size_t c = C; c > 0; c--) mcnt[c] = mcnt[c - 1];
+		mcnt[0] = 0;
+		/* 3. actions per cell; the next state is written in cell order */
+		size_t out = 0;
+		if (cap < 2 * m + 1) {
+			cap = 2 * m + 1;
+			cur = xrealloc(cur, cap * sizeof(jagent));
+		}
+		for (size_t c = 0; c < C; c++) {
+			jagent *L = nxt + mcnt[c];
+			const uint32_t k = mcnt[c + 1] - mcnt[c];
+			for (uint32_t i = k; i > 1; i--) {
+				const uint32_t j = mt_below(&rng, i);
+				const jagent tmp = L[i - 1];
+				L[i - 1] = L[j];
+				L[j] = tmp;
+			}
+			if (kcap < k) {
+				kcap = k;
+				kids = xrealloc(kids, kcap * sizeof(jagent));
+			}
+			uint32_t nk = 0;
+			for (uint32_t i = 0; i < k; i++) {
+				jagent *a = &L[i];
+				if (a->energy <= 0) continue;
+				uint32_t thr, prob;
+				if (!a->wolf) {
+					if (grass[c] <= 0) {
+						grass[c] = (int32_t)p.grass_restart;
+						a->energy += (int32_t)p.sheep_gain;
+					}
+					thr = p.sheep_thr;
+					prob = p.sheep_prob;
+				} else {
+					for (uint32_t o = 0; o < k; o++) {
+						if (!L[o].wolf && L[o].energy > 0) {
+							L[o].energy = 0;
+							a->energy += (int32_t)p.wolves_gain;
+							break;
+						}
+					}
+					thr = p.wolves_thr;
+					prob = p.wolves_prob;
+				}
+				if (a->energy > (int32_t)thr && mt_below(&rng, 100) < prob) {
+					jagent child = *a;
+					child.energy = a->energy / 2;
+					a->energy -= child.energy;
+					kids[nk++] = child;
+				}
+			}
+			start[c] = (uint32_t)out;
+			for (uint32_t i = 0; i < k; i++)
+				if (L[i].energy > 0) cur[out++] = L[i];
+			for (uint32_t i = 0; i < nk; i++) cur[out++] = kids[i];
+		}
+		start[C] = (uint32_t)out;
+		n = out;
+	}
+	free(grass); free(ag); free(acell); free(cnt); free(start); free(cur);
+	free(mov); free(mcell); free(mcnt); free(kids); free(nxt);
 	return 0;
 }
 

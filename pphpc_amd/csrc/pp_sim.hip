@@ -156,6 +156,8 @@ struct Ctl {
 	uint32_t err_step; /* tick at which error was set */
 	int32_t err_rep;   /* slot of the replica involved, -1 for the batch */
 	uint32_t nrep;     /* replicas in the lookup table */
+	uint32_t nmixed;   /* action tiles listed for k_act_mixed this tick */
+	uint32_t pad;
 	/* sums over completed ticks of: agents at the start, alive after the
 	 * move, after actions, occupied cells, grass eaten */
 	unsigned long long agent_steps, sum_live, sum_out, sum_occ, sum_eaten;
@@ -838,6 +840,234 @@ __device__ __forceinline__ void mix_add(unsigned long long (*s_mix)[NACC],
 	else atomicAdd(&T.rctl[T.order[k]].acc[idx], v);
 }
 
+/* Adds the per-thread counters of one wave to dst (wave sums, one atomic
+ * per counter from lane 0). Kept out of line: it runs when an action
+ * work-group moves on to another replica's tiles, and inlined it would hold
+ * registers over the whole tile loop. */
+__device__ __noinline__ void wave_flush(unsigned long long a0,
+	unsigned long long a1, unsigned long long a2, unsigned long long a3,
+	unsigned long long a4, unsigned long long a5, unsigned long long a6,
+	unsigned long long *dst) {
+	unsigned long long v[NACC] = {a0, a1, a2, a3, a4, a5, a6};
+	const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+	for (int k = 0; k < NACC; k++) {
+		const unsigned long long t = wave_sum(v[k]);
+		if (lane == 0 && t) atomicAdd(&dst[k], t);
+	}
+}
+
+/* LDS of one action work-group. */
+template <typename GT>
+struct ActLds {
+	uint32_t key[A_TILE + 1];
+	uint32_t val[A_TILE];
+	uint32_t slow_lp[A_TILE];
+	GT slow_g[A_TILE];
+	uint32_t nslow;
+	uint32_t tmp[NWAVE];
+	unsigned long long red[NWAVE][NACC];
+	unsigned long long mix[MIX_SLOTS][NACC];
+};
+
+/* Agent actions of one tile of A_TILE sorted positions (loaded in S.key /
+ * S.val, cnt valid), whose agents belong to the replicas k_first..k_last of
+ * the table. MIXED = false: one replica (ra), counters summed into acc;
+ * MIXED = true: several, each cell looks up its replica and counters go
+ * through LDS slots. See k_act for the rules. Returns the overflow flag. */
+template <typename GT, bool MIXED>
+__device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
+	uint32_t cnt, const uint32_t *__restrict__ keys, uint32_t *vals,
+	GT *grass_at, uint32_t *act_cnt, const Tables &T, uint32_t nrep,
+	uint32_t tick, uint32_t L, uint32_t Rmax, uint32_t k_first,
+	uint32_t k_last, const RA &ra_u, unsigned long long (&acc)[NACC]) {
+	const int tid = threadIdx.x;
+	const uint32_t base = tile * A_TILE;
+	const uint32_t alive_at = tick + 1;
+	const SegVals sv{S.val, vals + base};
+	bool overflow = false;
+	/* Heads of this thread's positions; their grass loads are issued
+	 * together before any segment is processed. */
+	uint32_t cj[A_IPT];
+	GT gj[A_IPT];
+	bool hj[A_IPT];
+#pragma unroll
+	for (int j = 0; j < A_IPT; j++) {
+		const uint32_t lp = tid * A_IPT + j;
+		cj[j] = S.key[1 + lp];
+		hj[j] = lp < cnt && S.key[lp] != cj[j];
+		gj[j] = hj[j] ? grass_at[cj[j]] : (GT)0;
+	}
+	/* Cells with one agent (most of them) are handled in place; heads of
+	 * longer segments are listed and then processed by consecutive lanes, so
+	 * the long serial path runs once per wave, not once per position of every
+	 * lane. Segments are independent: the order of the list does not change
+	 * any result. In a tile of several replicas every cell takes the general
+	 * path. */
+	if (tid == 0) S.nslow = 0;
+	if (MIXED)
+		for (int i = tid; i < MIX_SLOTS * NACC; i += WG) (&S.mix[0][0])[i] = 0;
+	__syncthreads();
+	uint32_t nout = 0;
+#pragma unroll
+	for (int j = 0; j < A_IPT; j++) {
+		if (!hj[j]) continue;
+		const uint32_t lp = tid * A_IPT + j;
+		const uint32_t c = cj[j];
+		if (!MIXED && lp + 1 < cnt && S.key[2 + lp] != c) {
+			/* A cell with one agent: nothing to shuffle, a lone wolf finds no
+			 * sheep, a lone agent is never eaten. */
+			const uint32_t v = S.val[lp];
+			int32_t e = (int32_t)(v & V_ENERGY);
+			const uint32_t t = (v & V_WOLF) ? 1u : 0u;
+			uint32_t thr = ra_u.wolves_thr;
+			uint32_t ate = 0;
+			if (!t) {
+				thr = ra_u.sheep_thr;
+				if (grass_alive_at<GT>(gj[j], alive_at, Rmax)) {
+					e += (int32_t)ra_u.sheep_gain;
+					grass_at[c] = (GT)(tick + 1 + ra_u.R);
+					ate = 1;
+				}
+			}
+			uint32_t nv = v & V_WOLF;
+			uint32_t ce = 0;
+			if (e > (int32_t)thr && (v & V_REPRO)) {
+				nv |= V_CHILD | ((e & 1) ? V_ODD : 0u);
+				ce = (uint32_t)(e / 2);
+				e -= e / 2;
+			}
+			overflow |= e > (int32_t)V_ENERGY;
+			S.val[lp] = nv | ((uint32_t)e & V_ENERGY);
+			const uint32_t outs = (nv & V_CHILD) ? 2u : 1u;
+			const unsigned long long en = (uint32_t)e + ce;
+			if (t) {
+				acc[AC_WOLVES] += outs;
+				acc[AC_WOLVES_EN] += en;
+			} else {
+				acc[AC_SHEEP] += outs;
+				acc[AC_SHEEP_EN] += en;
+			}
+			acc[AC_EATEN] += ate;
+			acc[AC_OCC] += 1;
+			acc[AC_OUT] += outs;
+			nout += outs;
+			continue;
+		}
+		const uint32_t slot = atomicAdd(&S.nslow, 1u);
+		S.slow_lp[slot] = lp;
+		S.slow_g[slot] = gj[j];
+	}
+	__syncthreads();
+	const uint32_t nslow = S.nslow;
+#pragma unroll 1
+	for (uint32_t si = tid; si < nslow; si += WG) {
+		const uint32_t lp = S.slow_lp[si];
+		const uint32_t c = S.key[1 + lp];
+		const GT gslow = S.slow_g[si];
+		const uint32_t kk = MIXED ? rep_index(T, nrep, c) : k_first;
+		const RA ra = MIXED ? load_ra(T, kk, tick) : ra_u;
+		const uint32_t end = seg_end(S.key, keys, lp, cnt, base, L, c);
+		const uint32_t k = end - lp;
+		if (k > 1) {
+			uint4 r = make_uint4(0, 0, 0, 0);
+			for (uint32_t i = k - 1, m = 0; i > 0; i--, m++) {
+				if ((m & 3u) == 0)
+					r = pp_philox(c - ra.cb, ra.step, PP_STREAM_SHUFFLE, m >> 2,
+						ra.key0, ra.key1);
+				const uint32_t word = (m & 3u) == 0 ? r.x
+					: (m & 3u) == 1 ? r.y : (m & 3u) == 2 ? r.z : r.w;
+				const uint32_t jj = pp_bounded(word, i + 1);
+				if (jj != i) {
+					const uint32_t a = sv.get(lp + i);
+					sv.set(lp + i, sv.get(lp + jj));
+					sv.set(lp + jj, a);
+				}
+			}
+		}
+		bool grass = grass_alive_at<GT>(gslow, alive_at, Rmax);
+		bool ate = false;
+		for (uint32_t q = lp; q < end; q++) {
+			const uint32_t v = sv.get(q);
+			if (v & V_KILLED) continue;
+			int32_t e = (int32_t)(v & V_ENERGY);
+			uint32_t thr;
+			if (!(v & V_WOLF)) {
+				if (grass) {
+					grass = false;
+					ate = true;
+					e += (int32_t)ra.sheep_gain;
+				}
+				thr = ra.sheep_thr;
+			} else {
+				for (uint32_t o = lp; o < end; o++) {
+					const uint32_t u = sv.get(o);
+					if (!(u & (V_WOLF | V_KILLED))) {
+						sv.set(o, u | V_KILLED);
+						e += (int32_t)ra.wolves_gain;
+						break;
+					}
+				}
+				thr = ra.wolves_thr;
+			}
+			uint32_t nv = v & V_WOLF;
+			if (e > (int32_t)thr && (v & V_REPRO)) {
+				nv |= V_CHILD | ((e & 1) ? V_ODD : 0u);
+				e -= e / 2;
+			}
+			overflow |= e > (int32_t)V_ENERGY;
+			sv.set(q, nv | ((uint32_t)e & V_ENERGY));
+		}
+		if (ate) grass_at[c] = (GT)(tick + 1 + ra.R);
+		unsigned long long sc[NACC] = {0, 0, 0, 0, 0, 0, 0};
+		sc[AC_EATEN] = ate ? 1 : 0;
+		sc[AC_OCC] = 1;
+		for (uint32_t q = lp; q < end; q++) {
+			const uint32_t v = sv.get(q);
+			const uint32_t e = v & V_ENERGY;
+			const uint32_t t = (v & V_WOLF) ? 1u : 0u;
+			const uint32_t outs = ((v & V_KILLED) == 0) + ((v & V_CHILD) != 0);
+			const uint32_t ea = ((v & V_KILLED) ? 0u : e)
+				+ ((v & V_CHILD) ? ((v & V_ODD) ? e - 1 : e) : 0u);
+			if (t) {
+				sc[AC_WOLVES] += outs;
+				sc[AC_WOLVES_EN] += ea;
+			} else {
+				sc[AC_SHEEP] += outs;
+				sc[AC_SHEEP_EN] += ea;
+			}
+			sc[AC_OUT] += outs;
+			if (q < (uint32_t)A_TILE) nout += outs;
+			else if (outs) atomicAdd(&act_cnt[tile + q / A_TILE], outs);
+		}
+		if (!MIXED) {
+#pragma unroll
+			for (int i = 0; i < NACC; i++) acc[i] += sc[i];
+		} else {
+#pragma unroll
+			for (int i = 0; i < NACC; i++) mix_add(S.mix, T, kk, k_first, i, sc[i]);
+		}
+	}
+	__syncthreads();
+	/* Write back the positions this tile owns (not those of the previous
+	 * tile's last segment, which its lane wrote). */
+	for (uint32_t i = tid; i < cnt; i += WG)
+		if (S.key[1 + i] != S.key[0]) vals[base + i] = S.val[i];
+	uint32_t total;
+	(void)wg_excl_scan<WG>(nout, S.tmp, total);
+	if (tid == 0 && total) atomicAdd(&act_cnt[tile], total);
+	if (MIXED) {
+		const uint32_t nk = min(k_last - k_first + 1, (uint32_t)MIX_SLOTS);
+		for (uint32_t i = tid; i < nk * NACC; i += WG) {
+			const unsigned long long v = (&S.mix[0][0])[i];
+			if (v)
+				atomicAdd(&T.rctl[T.order[k_first + i / NACC]].acc[i % NACC], v);
+		}
+		__syncthreads();
+	}
+	return overflow;
+}
+
 /* Agent actions for the agents sorted by cell (ascending agent id within a
  * cell). A work-group stages A_TILE sorted positions in LDS; the thread owning
  * a position that starts a run of equal cells (the segment that find_cell_idx,
@@ -858,24 +1088,17 @@ __device__ __forceinline__ void mix_add(unsigned long long (*s_mix)[NACC],
  * newborn; act_cnt[t] (zeroed by the pass-0 scan) receives the number of
  * agents the positions of tile t emit. Per replica, the counts and energy
  * sums of the new state, the grass eaten, the occupied cells and the outputs
- * are summed into its control words: in registers over the tiles of one
- * replica, through LDS slots in the few tiles shared by several. This kernel
- * also clears the pass-0 histogram table that k_emit_move fills for the next
- * iteration. */
+ * are summed into its control words, in registers over the tiles of one
+ * replica. Tiles holding agents of several replicas are listed for
+ * k_act_mixed (segments never span replicas, so the two kernels touch
+ * disjoint cells and values). This kernel also clears the pass-0 histogram
+ * table that k_emit_move fills for the next iteration. */
 template <typename GT>
 __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
-	uint32_t *vals, GT *grass_at, uint32_t *act_cnt,
+	uint32_t *vals, GT *grass_at, uint32_t *act_cnt, uint32_t *mixed_list,
 	uint32_t *__restrict__ hist, uint32_t hist_words, Ctl *ctl, Tables T,
 	Params P) {
-	__shared__ uint32_t s_key[A_TILE + 1];
-	__shared__ uint32_t s_val[A_TILE];
-	__shared__ uint32_t s_slow_lp[A_TILE];
-	__shared__ GT s_slow_g[A_TILE];
-	__shared__ uint32_t s_nslow;
-	__shared__ uint32_t s_tmp[NWAVE];
-	__shared__ unsigned long long s_red[NWAVE][NACC];
-	__shared__ unsigned long long s_mix[MIX_SLOTS][NACC];
-
+	__shared__ ActLds<GT> S;
 	if (ctl->error) return;
 	const int tid = threadIdx.x;
 	for (uint32_t i = blockIdx.x * WG + tid; i < hist_words; i += gridDim.x * WG)
@@ -883,215 +1106,71 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 	const uint32_t tick = ctl->tick, nrep = ctl->nrep;
 	const uint32_t L = ctl->live[tick & 1];
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
-	const uint32_t alive_at = tick + 1;
 	unsigned long long acc[NACC] = {0, 0, 0, 0, 0, 0, 0};
 	uint32_t acc_k = NO_REP;   /* replica the registers sum for */
+	RA ra{};
 	bool overflow = false;
-	/* A batch of one replica (every plain simulation) needs no lookups. */
-	RA ra_one{};
-	if (nrep == 1 && ntiles > blockIdx.x) ra_one = load_ra(T, 0, tick);
+	/* Each group takes a contiguous run of tiles, so it crosses few replica
+	 * boundaries. */
+	const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+	const uint32_t t_end = min(ntiles, (blockIdx.x + 1) * per);
 
-	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-		const uint32_t base = tile * A_TILE;
-		const uint32_t cnt = load_tile(keys, vals, s_key, s_val, base, L);
-		const SegVals sv{s_val, vals + base};
+	for (uint32_t tile = blockIdx.x * per; tile < t_end; tile++) {
+		const uint32_t cnt = load_tile(keys, vals, S.key, S.val, tile * A_TILE, L);
 		/* Replicas of this tile: keys ascend, so they are the table entries
-		 * k_first..k_last. The branch below is uniform over the group. */
+		 * k_first..k_last. Every branch below is uniform over the group. */
 		const uint32_t k_first = rep_index(T, nrep,
-			__builtin_amdgcn_readfirstlane(s_key[1]));
+			__builtin_amdgcn_readfirstlane(S.key[1]));
 		const uint32_t k_last = rep_index(T, nrep,
-			__builtin_amdgcn_readfirstlane(s_key[cnt]));
-		const bool mixed = k_first != k_last;
-		if (!mixed && k_first != acc_k) {
-			if (acc_k != NO_REP)
-				wg_add<NACC>(acc, s_red, &T.rctl[T.order[acc_k]].acc[0]);
-			acc_k = k_first;
-		}
-		const RA ra_t = nrep == 1 ? ra_one : load_ra(T, k_first, tick);
-		/* Heads of this thread's positions; their grass loads are issued
-		 * together before any segment is processed. */
-		uint32_t cj[A_IPT];
-		GT gj[A_IPT];
-		bool hj[A_IPT];
-#pragma unroll
-		for (int j = 0; j < A_IPT; j++) {
-			const uint32_t lp = tid * A_IPT + j;
-			cj[j] = s_key[1 + lp];
-			hj[j] = lp < cnt && s_key[lp] != cj[j];
-			gj[j] = hj[j] ? grass_at[cj[j]] : (GT)0;
-		}
-		/* Cells with one agent (most of them) are handled in place; heads of
-		 * longer segments are listed and then processed by consecutive
-		 * lanes, so the long serial path runs once per wave, not once per
-		 * position of every lane. Segments are independent: the order of the
-		 * list does not change any result. */
-		if (tid == 0) s_nslow = 0;
-		if (mixed)
-			for (int i = tid; i < MIX_SLOTS * NACC; i += WG) (&s_mix[0][0])[i] = 0;
-		__syncthreads();
-		uint32_t nout = 0;
-#pragma unroll
-		for (int j = 0; j < A_IPT; j++) {
-			if (!hj[j]) continue;
-			const uint32_t lp = tid * A_IPT + j;
-			const uint32_t c = cj[j];
-			if (!mixed && lp + 1 < cnt && s_key[2 + lp] != c) {
-				/* A cell with one agent: nothing to shuffle, a lone wolf finds
-				 * no sheep, a lone agent is never eaten. (In a tile of several
-				 * replicas every cell takes the general path below.) */
-				const RA &ra = ra_t;
-				const uint32_t v = s_val[lp];
-				int32_t e = (int32_t)(v & V_ENERGY);
-				const uint32_t t = (v & V_WOLF) ? 1u : 0u;
-				uint32_t thr = ra.wolves_thr;
-				uint32_t ate = 0;
-				if (!t) {
-					thr = ra.sheep_thr;
-					if (grass_alive_at<GT>(gj[j], alive_at, P.R)) {
-						e += (int32_t)ra.sheep_gain;
-						grass_at[c] = (GT)(tick + 1 + ra.R);
-						ate = 1;
-					}
-				}
-				uint32_t nv = v & V_WOLF;
-				uint32_t ce = 0;
-				if (e > (int32_t)thr && (v & V_REPRO)) {
-					nv |= V_CHILD | ((e & 1) ? V_ODD : 0u);
-					ce = (uint32_t)(e / 2);
-					e -= e / 2;
-				}
-				overflow |= e > (int32_t)V_ENERGY;
-				s_val[lp] = nv | ((uint32_t)e & V_ENERGY);
-				const uint32_t outs = (nv & V_CHILD) ? 2u : 1u;
-				const unsigned long long en = (uint32_t)e + ce;
-				if (t) {
-					acc[AC_WOLVES] += outs;
-					acc[AC_WOLVES_EN] += en;
-				} else {
-					acc[AC_SHEEP] += outs;
-					acc[AC_SHEEP_EN] += en;
-				}
-				acc[AC_EATEN] += ate;
-				acc[AC_OCC] += 1;
-				acc[AC_OUT] += outs;
-				nout += outs;
-				continue;
-			}
-			const uint32_t slot = atomicAdd(&s_nslow, 1u);
-			s_slow_lp[slot] = lp;
-			s_slow_g[slot] = gj[j];
-		}
-		__syncthreads();
-		const uint32_t nslow = s_nslow;
-#pragma unroll 1
-		for (uint32_t si = tid; si < nslow; si += WG) {
-			const uint32_t lp = s_slow_lp[si];
-			const uint32_t c = s_key[1 + lp];
-			const GT gslow = s_slow_g[si];
-			const uint32_t kk = mixed ? rep_index(T, nrep, c) : k_first;
-			const RA ra = mixed ? load_ra(T, kk, tick) : ra_t;
-			const uint32_t end = seg_end(s_key, keys, lp, cnt, base, L, c);
-			const uint32_t k = end - lp;
-			if (k > 1) {
-				uint4 r = make_uint4(0, 0, 0, 0);
-				for (uint32_t i = k - 1, m = 0; i > 0; i--, m++) {
-					if ((m & 3u) == 0)
-						r = pp_philox(c - ra.cb, ra.step, PP_STREAM_SHUFFLE, m >> 2,
-							ra.key0, ra.key1);
-					const uint32_t word = (m & 3u) == 0 ? r.x
-						: (m & 3u) == 1 ? r.y : (m & 3u) == 2 ? r.z : r.w;
-					const uint32_t jj = pp_bounded(word, i + 1);
-					if (jj != i) {
-						const uint32_t a = sv.get(lp + i);
-						sv.set(lp + i, sv.get(lp + jj));
-						sv.set(lp + jj, a);
-					}
-				}
-			}
-			bool grass = grass_alive_at<GT>(gslow, alive_at, P.R);
-			bool ate = false;
-			for (uint32_t q = lp; q < end; q++) {
-				const uint32_t v = sv.get(q);
-				if (v & V_KILLED) continue;
-				int32_t e = (int32_t)(v & V_ENERGY);
-				uint32_t thr;
-				if (!(v & V_WOLF)) {
-					if (grass) {
-						grass = false;
-						ate = true;
-						e += (int32_t)ra.sheep_gain;
-					}
-					thr = ra.sheep_thr;
-				} else {
-					for (uint32_t o = lp; o < end; o++) {
-						const uint32_t u = sv.get(o);
-						if (!(u & (V_WOLF | V_KILLED))) {
-							sv.set(o, u | V_KILLED);
-							e += (int32_t)ra.wolves_gain;
-							break;
-						}
-					}
-					thr = ra.wolves_thr;
-				}
-				uint32_t nv = v & V_WOLF;
-				if (e > (int32_t)thr && (v & V_REPRO)) {
-					nv |= V_CHILD | ((e & 1) ? V_ODD : 0u);
-					e -= e / 2;
-				}
-				overflow |= e > (int32_t)V_ENERGY;
-				sv.set(q, nv | ((uint32_t)e & V_ENERGY));
-			}
-			if (ate) grass_at[c] = (GT)(tick + 1 + ra.R);
-			unsigned long long sc[NACC] = {0, 0, 0, 0, 0, 0, 0};
-			sc[AC_EATEN] = ate ? 1 : 0;
-			sc[AC_OCC] = 1;
-			for (uint32_t q = lp; q < end; q++) {
-				const uint32_t v = sv.get(q);
-				const uint32_t e = v & V_ENERGY;
-				const uint32_t t = (v & V_WOLF) ? 1u : 0u;
-				const uint32_t outs = ((v & V_KILLED) == 0) + ((v & V_CHILD) != 0);
-				const uint32_t na = outs;
-				const uint32_t ea = ((v & V_KILLED) ? 0u : e)
-					+ ((v & V_CHILD) ? ((v & V_ODD) ? e - 1 : e) : 0u);
-				if (t) {
-					sc[AC_WOLVES] += na;
-					sc[AC_WOLVES_EN] += ea;
-				} else {
-					sc[AC_SHEEP] += na;
-					sc[AC_SHEEP_EN] += ea;
-				}
-				sc[AC_OUT] += outs;
-				if (q < (uint32_t)A_TILE) nout += outs;
-				else if (outs) atomicAdd(&act_cnt[tile + q / A_TILE], outs);
-			}
-			if (!mixed) {
-#pragma unroll
-				for (int i = 0; i < NACC; i++) acc[i] += sc[i];
-			} else {
-#pragma unroll
-				for (int i = 0; i < NACC; i++) mix_add(s_mix, T, kk, k_first, i, sc[i]);
-			}
-		}
-		__syncthreads();
-		/* Write back the positions this tile owns (not those of the previous
-		 * tile's last segment, which its lane wrote). */
-		for (uint32_t i = tid; i < cnt; i += WG)
-			if (s_key[1 + i] != s_key[0]) vals[base + i] = s_val[i];
-		uint32_t total;
-		(void)wg_excl_scan<WG>(nout, s_tmp, total);
-		if (tid == 0 && total) atomicAdd(&act_cnt[tile], total);
-		if (mixed) {
-			const uint32_t nk = min(k_last - k_first + 1, (uint32_t)MIX_SLOTS);
-			for (uint32_t i = tid; i < nk * NACC; i += WG) {
-				const unsigned long long v = (&s_mix[0][0])[i];
-				if (v)
-					atomicAdd(&T.rctl[T.order[k_first + i / NACC]].acc[i % NACC], v);
-			}
+			__builtin_amdgcn_readfirstlane(S.key[cnt]));
+		if (k_first != k_last) {
+			if (tid == 0) mixed_list[atomicAdd(&ctl->nmixed, 1u)] = tile;
 			__syncthreads();
+			continue;
 		}
+		if (k_first != acc_k) {
+			if (acc_k != NO_REP) {
+				wave_flush(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6],
+					&T.rctl[T.order[acc_k]].acc[0]);
+#pragma unroll
+				for (int i = 0; i < NACC; i++) acc[i] = 0;
+			}
+			acc_k = k_first;
+			ra = load_ra(T, k_first, tick);
+		}
+		overflow |= act_tile<GT, false>(S, tile, cnt, keys, vals, grass_at,
+			act_cnt, T, nrep, tick, L, P.R, k_first, k_last, ra, acc);
 	}
 	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
-	if (acc_k != NO_REP) wg_add<NACC>(acc, s_red, &T.rctl[T.order[acc_k]].acc[0]);
+	if (acc_k != NO_REP) wg_add<NACC>(acc, S.red, &T.rctl[T.order[acc_k]].acc[0]);
+}
+
+/* The tiles k_act listed as holding agents of several replicas (at most one
+ * per replica boundary): the same actions with a table lookup per cell. */
+template <typename GT>
+__global__ __launch_bounds__(WG) void k_act_mixed(
+	const uint32_t *__restrict__ keys, uint32_t *vals, GT *grass_at,
+	uint32_t *act_cnt, const uint32_t *mixed_list, Ctl *ctl, Tables T,
+	Params P) {
+	__shared__ ActLds<GT> S;
+	if (ctl->error) return;
+	const uint32_t tick = ctl->tick, nrep = ctl->nrep;
+	const uint32_t L = ctl->live[tick & 1];
+	const uint32_t nm = ctl->nmixed;
+	unsigned long long acc[NACC] = {0, 0, 0, 0, 0, 0, 0};
+	const RA none{};
+	bool overflow = false;
+	for (uint32_t i = blockIdx.x; i < nm; i += gridDim.x) {
+		const uint32_t tile = mixed_list[i];
+		const uint32_t cnt = load_tile(keys, vals, S.key, S.val, tile * A_TILE, L);
+		const uint32_t k_first = rep_index(T, nrep,
+			__builtin_amdgcn_readfirstlane(S.key[1]));
+		const uint32_t k_last = rep_index(T, nrep,
+			__builtin_amdgcn_readfirstlane(S.key[cnt]));
+		overflow |= act_tile<GT, true>(S, tile, cnt, keys, vals, grass_at,
+			act_cnt, T, nrep, tick, L, P.R, k_first, k_last, none, acc);
+	}
+	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
 }
 
 /* One work-group: exclusive scan of the action tiles' output counts (the
@@ -1218,8 +1297,56 @@ __global__ __launch_bounds__(SCAN_WG) void k_act_scan(
 	ctl->sum_occ += bs[1];
 	ctl->sum_eaten += bs[2];
 	ctl->n = total;
+	ctl->nmixed = 0;
 	ctl->live[(tick + 1) & 1] = 0;
 	ctl->tick = tick + 1;
+}
+
+/* Writes the staged outputs [0, total) of an action tile (batch positions
+ * goff + i) to the SoA state, moves each for the next iteration and counts
+ * its pass-0 digit. MIXED = false: every output belongs to replica m (its
+ * constants stay uniform across the group); true: each looks up its own.
+ * Returns the number of survivors of the move. */
+template <uint32_t R, bool MIXED>
+__device__ __forceinline__ unsigned long long emit_outputs(
+	const uint32_t *s_oc, const uint32_t *s_ov, uint32_t *s_h, uint32_t total,
+	uint32_t goff, uint32_t to0, uint32_t dmask, uint32_t *__restrict__ ocell,
+	int32_t *__restrict__ oenergy, uint8_t *__restrict__ otype,
+	uint32_t *__restrict__ mkeys, uint32_t *__restrict__ mvals,
+	const Tables &T, uint32_t nrep, uint32_t tick, const RM &m_u,
+	const Params &P, bool &overflow) {
+	unsigned long long live = 0;
+	for (uint32_t i0 = 0; i0 < total; i0 += WG) {
+		const uint32_t i = i0 + threadIdx.x;
+		uint32_t d = NO_DIGIT;
+		if (i < total) {
+			const uint32_t id = goff + i;
+			const uint32_t cc = s_oc[i], ov = s_ov[i];
+			const uint32_t en = ov & V_ENERGY, t = ov >> 31;
+			ocell[id] = cc;
+			oenergy[id] = (int32_t)en;
+			otype[id] = (uint8_t)t;
+			uint32_t mv = 0, mk = DEAD_KEY;
+			if (MIXED) {
+				const uint32_t sl = T.order[rep_index(T, nrep, cc)];
+				const RM m = load_rm(T.rep[sl], T.rctl[sl], tick);
+				if (m.active)
+					mk = move_agent(id - m.base, cc - m.cb, (int32_t)en, t, m, &mv,
+						overflow);
+			} else if (m_u.active) {
+				mk = move_agent(id - m_u.base, cc - m_u.cb, (int32_t)en, t, m_u,
+					&mv, overflow);
+			}
+			mkeys[id] = mk;
+			mvals[id] = mv;
+			if (mk != DEAD_KEY) {
+				live += 1;
+				d = (id / P.sort_tile - to0) * R + (mk & dmask);
+			}
+		}
+		wave_hist_add(s_h, d);
+	}
+	return live;
 }
 
 /* Emits the new state and moves it for the next iteration. Sorted position
@@ -1317,34 +1444,14 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 			m_t = load_rm(T.rep[s0], T.rctl[s0], tick);
 		}
 		const uint32_t to0 = goff / P.sort_tile;
-		for (uint32_t i0 = 0; i0 < total; i0 += WG) {
-			const uint32_t i = i0 + tid;
-			uint32_t d = NO_DIGIT;
-			if (i < total) {
-				const uint32_t id = goff + i;
-				const uint32_t cc = s_oc[i], ov = s_ov[i];
-				const uint32_t en = ov & V_ENERGY, t = ov >> 31;
-				ocell[id] = cc;
-				oenergy[id] = (int32_t)en;
-				otype[id] = (uint8_t)t;
-				RM m = m_t;
-				if (mixed) {
-					const uint32_t sl = T.order[rep_index(T, nrep, cc)];
-					m = load_rm(T.rep[sl], T.rctl[sl], tick);
-				}
-				uint32_t mv = 0, mk = DEAD_KEY;
-				if (m.active)
-					mk = move_agent(id - m.base, cc - m.cb, (int32_t)en, t, m, &mv,
-						overflow);
-				mkeys[id] = mk;
-				mvals[id] = mv;
-				if (mk != DEAD_KEY) {
-					live += 1;
-					d = (id / P.sort_tile - to0) * R + (mk & dmask);
-				}
-			}
-			wave_hist_add(s_h, d);
-		}
+		if (!mixed)
+			live += emit_outputs<R, false>(s_oc, s_ov, s_h, total, goff, to0, dmask,
+				ocell, oenergy, otype, mkeys, mvals, T, nrep, tick, m_t, P,
+				overflow);
+		else
+			live += emit_outputs<R, true>(s_oc, s_ov, s_h, total, goff, to0, dmask,
+				ocell, oenergy, otype, mkeys, mvals, T, nrep, tick, m_t, P,
+				overflow);
 		__syncthreads();
 		for (uint32_t i = tid; i < 2 * R; i += WG) {
 			const uint32_t h = s_h[i];
@@ -1513,6 +1620,7 @@ struct pp_batch {
 	uint32_t hist_words = 0;
 	uint32_t *totals = nullptr;
 	uint32_t *act_cnt = nullptr;
+	uint32_t *mixed_list = nullptr;  /* action tiles of several replicas */
 	uint32_t act_words = 0;
 	Ctl *ctl = nullptr;
 	Ctl *ctl_host = nullptr;        /* pinned mirror */
@@ -1673,8 +1781,13 @@ static void launch_hist0(pp_batch_t *b) {
 template <typename GT>
 static void launch_act(pp_batch_t *b, int sorted) {
 	k_act<GT><<<b->grid_act, WG, 0, b->stream>>>(b->keys[sorted],
-		b->vals[sorted], (GT *)b->grass_at, b->act_cnt, b->hist, b->hist_words,
-		b->ctl, b->tables(), b->P);
+		b->vals[sorted], (GT *)b->grass_at, b->act_cnt, b->mixed_list, b->hist,
+		b->hist_words, b->ctl, b->tables(), b->P);
+	/* A batch of one replica never has a tile of several. */
+	if (b->max_rep > 1)
+		k_act_mixed<GT><<<min_u32(b->max_rep, 256), WG, 0, b->stream>>>(
+			b->keys[sorted], b->vals[sorted], (GT *)b->grass_at, b->act_cnt,
+			b->mixed_list, b->ctl, b->tables(), b->P);
 }
 
 /* Enqueue one tick (no synchronisation): the sort of the pairs the previous
@@ -1945,6 +2058,7 @@ static void batch_free(pp_batch_t *b) {
 	(void)hipFree(b->hist);
 	(void)hipFree(b->totals);
 	(void)hipFree(b->act_cnt);
+	(void)hipFree(b->mixed_list);
 	(void)hipFree(b->ctl);
 	(void)hipFree(b->rep_dev);
 	(void)hipFree(b->rctl_dev);
@@ -1966,7 +2080,7 @@ static uint64_t batch_bytes(uint64_t A, uint64_t C, uint32_t max_rep,
 	t += C * grass_bytes_for(R);
 	t += (uint64_t)max_rep * (R + 1) * 8;
 	t += hist_words * 4 + RADIX_MAX * 4;
-	t += ((uint64_t)ceil_div(A, A_TILE) + 1) * 4;
+	t += ((uint64_t)ceil_div(A, A_TILE) + 1) * 8;
 	t += sizeof(Ctl) + (uint64_t)max_rep * (sizeof(Rep) + sizeof(RepCtl) + 8);
 	t += (uint64_t)max_rep * stats_rows * sizeof(pp_stats64_t);
 	return t;
@@ -2002,6 +2116,7 @@ static int batch_alloc(pp_batch_t *b) {
 	HIP_OK(hipMalloc(&b->totals, RADIX_MAX * 4));
 	b->act_words = ceil_div(A, A_TILE) + 1;
 	HIP_OK(hipMalloc(&b->act_cnt, (size_t)b->act_words * 4));
+	HIP_OK(hipMalloc(&b->mixed_list, (size_t)b->act_words * 4));
 	HIP_OK(hipMalloc(&b->ctl, sizeof(Ctl)));
 	HIP_OK(hipMalloc(&b->rep_dev, (size_t)b->max_rep * sizeof(Rep)));
 	HIP_OK(hipMalloc(&b->rctl_dev, (size_t)b->max_rep * sizeof(RepCtl)));

@@ -1,6 +1,8 @@
 """Replica statistics (focal measures, Welch t, Mann-Whitney, Holm) and the
 statistical equivalence of the model under test with an independent
-NetLogo-rules simulator (oracle/pp_statref.c, MT19937)."""
+simulator of the Java single-thread rules (oracle/pp_statref.c, MT19937),
+plus the NetLogo rules at the size where the two rule sets are not yet
+distinguishable."""
 import math
 from multiprocessing import Pool
 
@@ -85,25 +87,30 @@ def _oracle_focal(args):
 
 
 def _statref_focal(args):
-    d, seed = args
+    d, seed, rules = args
     from oracle.pp_oracle import statref_run
-    return rp.focal_measures(statref_run(d, seed), d["grid_x"] * d["grid_y"])
+    return rp.focal_measures(statref_run(d, seed, rules=rules),
+                             d["grid_x"] * d["grid_y"])
 
 
-def _samples(d, n=30, fault=0):
+def _samples(d, n=30, fault=0, rules="cells"):
     with Pool(6) as pool:
         ours = pool.map(_oracle_focal, [(d, s, fault) for s in range(1, n + 1)])
-        ref = pool.map(_statref_focal, [(d, s) for s in range(1001, 1001 + n)])
+        ref = pool.map(_statref_focal,
+                       [(d, s, rules) for s in range(1001, 1001 + n)])
     return ours, ref
 
 
-@pytest.mark.parametrize("name,d", [("v1 100x100", v1(100)),
-                                    ("reference v2 100x100", v2ref(100))])
-def test_model_equivalent_to_netlogo_rules(name, d):
+@pytest.mark.parametrize("name,d,rules", [
+    ("v1 100x100", v1(100), "cells"),
+    ("reference v2 100x100", v2ref(100), "cells"),
+    ("v1 100x100 NetLogo rules", v1(100), "netlogo"),
+])
+def test_model_equivalent_to_reference_rules(name, d, rules):
     """30 replicas of the model under test (the oracle, which the GPU
-    matches bit for bit) against 30 replicas of the independent NetLogo-rules
-    simulator: no focal measure differs after Holm correction."""
-    ours, ref = _samples(d)
+    matches bit for bit) against 30 replicas of the independent simulator:
+    no focal measure differs after Holm correction."""
+    ours, ref = _samples(d, rules=rules)
     rep = rp.compare(ours, ref)
     bad = [(r["measure"], r["test"], r["p"]) for r in rep["measures"]
            if r["reject"]]
