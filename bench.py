@@ -45,8 +45,8 @@ sys.path.insert(0, ROOT)
 METRIC = ("agent-steps/s @1600² v2 plain vs batched; replicas/hr; "
           "p99 run latency")
 HBM_PEAK_GBS = 8000.0
-STEP_KERNELS = ("sort_up", "sort_scan", "sort_down", "act", "act_scan",
-                "emit_move")
+STEP_KERNELS = ("bucket_scan", "bucket_sort", "sort_up", "sort_scan",
+                "sort_down", "act", "act_scan", "emit_move")
 
 
 def log(msg):
@@ -81,25 +81,24 @@ WORKLOADS = {
 
 
 def kernel_bytes(name, c, passes):
-    """Algorithmic (compulsory) HBM bytes of all launches of one kernel over a
-    run, from the run's exact event sums c (agent-steps N, live after move L,
-    agents after actions N', occupied cells O, grass eaten E):
-      sort_down  pass 0 reads the N moved pairs (8 B) and writes the L
-                 survivors; every later pass reads and writes L pairs:
-                 8N + 8L + 16L(P-1)
-      sort_up    later passes read the L keys once more: 4L(P-1)
-      act        read L sorted pairs, write the L updated values back, read
-                 the grass of each occupied cell, write each eaten one:
-                 12L + 4O + 4E
-      emit_move  read L sorted pairs, write the new state (9 B) and its moved
-                 pair (8 B) per agent: 8L + 17N'
+    """Algorithmic (compulsory) HBM bytes of all launches of one kernel over
+    iterations that all sort by row buckets, from the exact event sums c of
+    those iterations (agent-steps N, live after move L, agents after actions
+    N', occupied cells O, grass eaten E):
+      bucket_sort  read the N moved pairs (8 B), write the L survivors in
+                   cell order: 8N + 8L
+      act          read L sorted pairs, write the L updated values back, read
+                   the grass of each occupied cell, write each eaten one:
+                   12L + 4O + 4E
+      emit_move    read L sorted pairs, write the new state (9 B) and its
+                   moved pair (8 B) per agent: 8L + 17N'
+    The radix-sort kernels run only in the first iteration after a change of
+    the replicas and get no byte count here.
     """
     N, L, Np, O, E = (c["agent_steps"], c["live"], c["out"], c["occupied"],
                       c["eaten"])
-    if name == "sort_down":
-        return 8 * N + 8 * L + 16 * L * (passes - 1)
-    if name == "sort_up":
-        return 4 * L * (passes - 1) if passes > 1 else None
+    if name == "bucket_sort":
+        return 8 * N + 8 * L
     if name == "act":
         return 12 * L + 4 * O + 4 * E
     if name == "emit_move":
@@ -116,12 +115,15 @@ def measure_roofline(params, capacity, steps, seed=0):
     sim.init()
     sim.run(min(steps, 4), collect=False)      # warm the code paths
     sim.init()
+    sim.run(1, collect=False)                  # the radix-sorted iteration
+    c0 = sim.counters()
     sim.set_profiling(True)
     sim.profile_reset()
     sim.run(steps, collect=False)
     prof = sim.profile()
-    c = sim.counters()
+    c1 = sim.counters()
     sim.close()
+    c = {k: c1[k] - c0[k] for k in c1}
     step_kernels = {k: v for k, v in prof.items()
                     if k in STEP_KERNELS and v[0] > 0}
     dom = max((k for k in step_kernels if kernel_bytes(k, c, passes)),

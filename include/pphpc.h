@@ -160,6 +160,9 @@ int pp_sim_prof_export(pp_sim_t *s, const char *path);
 int pp_sim_info(const pp_sim_t *s, uint32_t *sort_passes, uint32_t *sort_tile,
 	uint32_t *action_tile, uint32_t *grid_move);
 
+/* Sort of the agents by cell, as pp_batch_set_sort. */
+int pp_sim_set_sort(pp_sim_t *s, uint32_t mode);
+
 /* Releases all device memory of the handle. NULL is accepted. */
 void pp_sim_destroy(pp_sim_t *s);
 
@@ -246,15 +249,23 @@ int pp_batch_counters(const pp_batch_t *b, uint64_t out[5]);
 int pp_batch_info(const pp_batch_t *b, uint32_t *sort_passes,
 	uint32_t *sort_tile, uint32_t *action_tile, uint32_t *grid_move);
 
-/* Sort launch shape: keys per thread (8 or 16; tile = 256 x that) and the
- * widest radix digit (8 or 11 bits). Results do not depend on it. */
+/* Sort of the agents by cell: mode 0 = the radix sort every iteration;
+ * 8..11 = from the second iteration after a change of the replicas, row
+ * buckets (blocks of 2^mode cells of a grid row); 1 (the default) = row
+ * buckets of 2^11 cells while the batch has at most 16384 of them, else the
+ * radix sort. Results do not depend on it. */
+int pp_batch_set_sort(pp_batch_t *b, uint32_t mode);
+
+/* Radix sort launch shape: keys per thread (8 or 16; tile = 256 x that) and
+ * the widest radix digit (8 or 11 bits). Results do not depend on it. */
 int pp_batch_set_shape(pp_batch_t *b, uint32_t keys_per_thread,
 	uint32_t max_digit_bits);
 
 /* Autotuner: runs trial_ticks iterations of the batch under each candidate
- * sort shape (the iterations count as normal progress), keeps the fastest
- * and reports it with its seconds per tick. */
-int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks,
+ * sort (bucket widths 2^8..2^11, radix shapes; the iterations count as
+ * normal progress), keeps the fastest and reports it (bucket_shift 0 = radix
+ * sort) with its seconds per tick. */
+int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks, uint32_t *bucket_shift,
 	uint32_t *keys_per_thread, uint32_t *max_digit_bits, double *sec_per_tick);
 
 /* Per-kernel timing of the batch, as pp_sim_set_profiling / prof_get. */

@@ -87,11 +87,12 @@ constexpr uint32_t NO_DIGIT = 0xFFFFFFFFu;
 constexpr uint32_t NO_REP = 0xFFFFFFFFu;
 constexpr uint32_t STATS_RING = 4096; /* stats rows kept per plain simulation */
 constexpr uint32_t ITERS_UNLIMITED = 0xFFFFFFFFu; /* iterations of a pp_sim */
-constexpr int M_IPT = 4;            /* agents per thread (move) */
 constexpr int A_IPT = 4;            /* sorted positions per thread (action) */
 constexpr int A_TILE = WG * A_IPT;
 constexpr int A_STAGE = 2 * A_TILE; /* staged outputs per action tile */
 constexpr int MIX_SLOTS = 16;       /* replicas per action tile summed in LDS */
+constexpr int SCAN_WG = 1024;       /* threads of the one-group scans */
+constexpr int SCAN_UNROLL = 8;
 /* Grass regrow ticks are stored modulo 2^(8 sizeof(GT)); alive cells are
  * re-stamped once every REFRESH ticks (1/REFRESH of the cells per tick), so
  * every stored value is within R + REFRESH of the tick. */
@@ -157,7 +158,7 @@ struct Ctl {
 	int32_t err_rep;   /* slot of the replica involved, -1 for the batch */
 	uint32_t nrep;     /* replicas in the lookup table */
 	uint32_t nmixed;   /* action tiles listed for k_act_mixed this tick */
-	uint32_t pad;
+	uint32_t nbuckets; /* buckets of the replicas in the table */
 	/* sums over completed ticks of: agents at the start, alive after the
 	 * move, after actions, occupied cells, grass eaten */
 	unsigned long long agent_steps, sum_live, sum_out, sum_occ, sum_eaten;
@@ -172,6 +173,10 @@ struct Params {
 	uint32_t hist_cols;  /* tile capacity of the histogram table */
 	uint32_t C;          /* cells of the global cell space */
 	uint32_t R;          /* largest grass restart allowed */
+	uint32_t bshift;     /* bucket blocks are 2^bshift cells of a grid row */
+	uint32_t bucket_on;  /* sort by row buckets; the emit counts for them */
+	uint32_t nb_cap;     /* bucket capacity of the count tables */
+	uint32_t bscan_one;  /* the buckets fit one scan chunk */
 };
 
 /* Replica tables in device memory: constants and control words by slot, and
@@ -182,6 +187,7 @@ struct Tables {
 	RepCtl *rctl;
 	const uint32_t *cbs;
 	const uint32_t *order;
+	const uint32_t *bbase;  /* first bucket of the k-th replica; [nrep] = all */
 };
 
 /* ---------------------------------------------------------------- helpers */
@@ -292,18 +298,32 @@ __device__ __forceinline__ void wave_hist_add(uint32_t *s_hist, uint32_t d) {
 
 /* What the move needs of a replica. */
 struct RM {
-	uint32_t cb, C, gx, sheep_prob, wolves_prob, key0, key1;
+	uint32_t cb, C, gx, gy, sheep_prob, wolves_prob, key0, key1;
 	uint32_t step;   /* local iteration the moved agents act in */
 	uint32_t base;   /* batch position of its local agent id 0 */
+	uint32_t bbase;  /* its first bucket */
+	uint32_t nb;     /* bucket blocks per grid row */
 	bool active;
 };
 
-__device__ __forceinline__ RM load_rm(const Rep &rp, const RepCtl &rc,
-	uint32_t tick) {
+/* Blocks of 2^S cells per grid row of width gx. */
+__device__ __forceinline__ uint32_t row_blocks(uint32_t gx, uint32_t S) {
+	return (gx + (1u << S) - 1u) >> S;
+}
+
+/* The move constants of the k-th replica of the table at tick `tick`. */
+__device__ __forceinline__ RM load_rm(const Tables &T, uint32_t k,
+	uint32_t tick, uint32_t S) {
+	const uint32_t slot = T.order[k];
+	const Rep &rp = T.rep[slot];
+	const RepCtl &rc = T.rctl[slot];
 	RM m;
 	m.cb = rp.cb;
 	m.C = rp.C;
 	m.gx = rp.gx;
+	m.gy = rp.gy;
+	m.bbase = T.bbase[k];
+	m.nb = row_blocks(rp.gx, S);
 	m.sheep_prob = rp.sheep_prob;
 	m.wolves_prob = rp.wolves_prob;
 	m.key0 = rp.key0;
@@ -314,35 +334,58 @@ __device__ __forceinline__ RM load_rm(const Rep &rp, const RepCtl &rc,
 	return m;
 }
 
-/* One move of a replica's agent with local id `id` at local cell c: Philox
- * block (id, step, MOVE, 0) under the replica's key; direction U[0,5) from
- * word 0 (0 stay, 1 +x, 2 -x, 3 +y, 4 -y, torus wrap: the table of
+/* One move of a replica's agent with local id `id` at local cell (x, y):
+ * Philox block (id, step, MOVE, 0) under the replica's key; direction U[0,5)
+ * from word 0 (0 stay, 1 +x, 2 -x, 3 +y, 4 -y, torus wrap: the table of
  * move_agent, pp_gpu.cl:780-803); reproduction draw U[0,100) from word 1,
  * kept as one bit (draw < probability of the agent's type) for the action
  * phase. Energy -1; the agent dies below 1 (pphpc.nlogo:66-70,
- * pp_gpu.cl:806-815): key DEAD_KEY, dropped by the first sort pass. Returns
- * the global key (cb + new local cell); *val gets the packed value. */
-__device__ __forceinline__ uint32_t move_agent(uint32_t id, uint32_t c,
-	int32_t energy, uint32_t wolf, const RM &m, uint32_t *val,
-	bool &overflow) {
+ * pp_gpu.cl:806-815): key DEAD_KEY, dropped by the sort. Returns the global
+ * key (cb + new local cell); *val gets the packed value and *bkt the bucket
+ * of the new cell (blocks of 2^S cells per row), NO_DIGIT for the dead. */
+__device__ __forceinline__ uint32_t move_agent(uint32_t id, uint32_t x,
+	uint32_t y, int32_t energy, uint32_t wolf, const RM &m, uint32_t S,
+	uint32_t *val, uint32_t *bkt, bool &overflow) {
 	const uint4 r = pp_philox(id, m.step, PP_STREAM_MOVE, 0, m.key0, m.key1);
 	const uint32_t dir = pp_bounded(r.x, 5);
 	const uint32_t dice = pp_bounded(r.y, 100);
 	const int32_t e = energy - 1;
 	*val = 0;
+	*bkt = NO_DIGIT;
 	if (e < 1) return DEAD_KEY;
-	const uint32_t x = c % m.gx;
-	const uint32_t wrap_y = m.C - m.gx;
-	uint32_t nc = c;
-	if (dir == 1) nc = (x == m.gx - 1) ? c - (m.gx - 1) : c + 1;
-	else if (dir == 2) nc = (x == 0) ? c + (m.gx - 1) : c - 1;
-	else if (dir == 3) nc = (c >= wrap_y) ? c - wrap_y : c + m.gx;
-	else if (dir == 4) nc = (c < m.gx) ? c + wrap_y : c - m.gx;
+	uint32_t nx = x, ny = y;
+	if (dir == 1) nx = (x == m.gx - 1) ? 0u : x + 1;
+	else if (dir == 2) nx = (x == 0) ? m.gx - 1 : x - 1;
+	else if (dir == 3) ny = (y == m.gy - 1) ? 0u : y + 1;
+	else if (dir == 4) ny = (y == 0) ? m.gy - 1 : y - 1;
 	const uint32_t prob = wolf ? m.wolves_prob : m.sheep_prob;
 	overflow |= e > (int32_t)V_ENERGY;
 	*val = ((uint32_t)e & V_ENERGY) | (wolf ? V_WOLF : 0u)
 		| (dice < prob ? V_REPRO : 0u);
-	return m.cb + nc;
+	*bkt = m.bbase + ny * m.nb + (nx >> S);
+	return m.cb + ny * m.gx + nx;
+}
+
+/* Adds 1 to the count of bucket `bkt` for every lane of the wave (NO_DIGIT
+ * lanes count nothing), merging runs of equal buckets in consecutive lanes
+ * first: into the LDS window s_win of WIN buckets from w0 when it falls
+ * there, else straight into the global table g_cnt. */
+template <uint32_t WIN>
+__device__ __forceinline__ void wave_bucket_add(uint32_t *s_win,
+	uint32_t *g_cnt, uint32_t w0, uint32_t bkt) {
+	const int lane = threadIdx.x & (WAVE - 1);
+	const uint32_t prev = __shfl_up(bkt, 1);
+	const bool head = lane == 0 || bkt != prev;
+	const uint64_t heads = __ballot(head);
+	if (head && bkt != NO_DIGIT) {
+		const uint64_t after = heads & ~((2ull << lane) - 1ull);
+		const uint32_t next = after ? (uint32_t)__builtin_ctzll(after)
+			: (uint32_t)WAVE;
+		const uint32_t run = next - (uint32_t)lane;
+		const uint32_t o = bkt - w0;
+		if (bkt >= w0 && o < WIN) atomicAdd(&s_win[o], run);
+		else atomicAdd(&g_cnt[bkt], run);
+	}
 }
 
 /* ------------------------------------------------------------------- init */
@@ -454,65 +497,50 @@ __global__ void k_rep_init_done(Ctl *ctl, Tables T, uint32_t slot,
 }
 
 /* Move for one replica's freshly initialised agents at batch positions
- * [pos, pos + n0) (SoA in local id order): their pairs, the pass-0 digit
- * histogram of every sort tile, and the live count of this tick. Each thread
- * moves M_IPT consecutive agents. */
+ * [pos, pos + n0) (SoA in local id order): their pairs and the live count of
+ * this tick. They are appended after the agents of the batch, out of cell
+ * order, so the next tick sorts with the general radix sort. */
 __global__ __launch_bounds__(WG) void k_move0(const uint32_t *__restrict__ cell,
 	const int32_t *__restrict__ energy, const uint8_t *__restrict__ type,
-	uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-	uint32_t *__restrict__ hist, Ctl *ctl, Tables T, uint32_t slot,
-	uint32_t pos, Params P) {
-	__shared__ uint32_t s_hist[2 * RADIX_MAX];
+	uint32_t *__restrict__ keys, uint32_t *__restrict__ vals, Ctl *ctl,
+	Tables T, uint32_t slot, uint32_t pos, Params P) {
 	__shared__ unsigned long long s_red[NWAVE];
 	if (ctl->error) return;
 	const uint32_t tick = ctl->tick;
 	const Rep &rp = T.rep[slot];
-	const RM m = load_rm(rp, T.rctl[slot], tick);
+	const RepCtl &rc = T.rctl[slot];
+	RM m;
+	m.cb = rp.cb;
+	m.C = rp.C;
+	m.gx = rp.gx;
+	m.gy = rp.gy;
+	m.sheep_prob = rp.sheep_prob;
+	m.wolves_prob = rp.wolves_prob;
+	m.key0 = rp.key0;
+	m.key1 = rp.key1;
+	m.step = tick - rp.t0;
+	m.base = pos;
+	m.bbase = 0;
+	m.nb = row_blocks(rp.gx, P.bshift);
+	m.active = rc.status == RS_ACTIVE;
 	const uint32_t n = rp.n0;
-	const int tid = threadIdx.x;
-	const uint32_t radix = 1u << P.digit_bits, dmask = radix - 1u;
 	unsigned long long live = 0;
 	bool overflow = false;
-	constexpr uint32_t CHUNK = WG * M_IPT;
-	for (uint32_t cb = blockIdx.x * CHUNK; cb < n; cb += gridDim.x * CHUNK) {
-		for (uint32_t i = tid; i < 2 * radix; i += WG) s_hist[i] = 0;
-		__syncthreads();
-		const uint32_t l0 = cb + tid * M_IPT;
-		uint32_t k[M_IPT];
-#pragma unroll
-		for (int j = 0; j < M_IPT; j++) {
-			const uint32_t l = l0 + j;
-			k[j] = DEAD_KEY;
-			if (l < n) {
-				const uint32_t g = pos + l;
-				uint32_t v = 0;
-				if (m.active)
-					k[j] = move_agent(l, cell[g] - m.cb, energy[g], type[g], m, &v,
-						overflow);
-				keys[g] = k[j];
-				vals[g] = v;
-				live += k[j] != DEAD_KEY;
-			}
+	for (uint32_t l = blockIdx.x * WG + threadIdx.x; l < n; l += gridDim.x * WG) {
+		const uint32_t g = pos + l;
+		uint32_t k = DEAD_KEY, v = 0, bkt;
+		if (m.active) {
+			const uint32_t c = cell[g] - m.cb;
+			const uint32_t y = c / m.gx, x = c - y * m.gx;
+			k = move_agent(l, x, y, energy[g], type[g], m, P.bshift, &v, &bkt,
+				overflow);
 		}
-		/* positions [pos + cb, pos + cb + CHUNK) span at most two sort tiles
-		 * (CHUNK <= sort_tile) */
-		const uint32_t t0 = (pos + cb) / P.sort_tile;
-#pragma unroll
-		for (int j = 0; j < M_IPT; j++) {
-			uint32_t d = NO_DIGIT;
-			if (k[j] != DEAD_KEY)
-				d = ((pos + l0 + j) / P.sort_tile - t0) * radix + (k[j] & dmask);
-			wave_hist_add(s_hist, d);
-		}
-		__syncthreads();
-		for (uint32_t i = tid; i < 2 * radix; i += WG)
-			if (s_hist[i])
-				atomicAdd(&hist[(size_t)(i % radix) * P.hist_cols + t0 + i / radix],
-					s_hist[i]);
-		__syncthreads();
+		keys[g] = k;
+		vals[g] = v;
+		live += k != DEAD_KEY;
 	}
 	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, (int32_t)slot);
-	const int lane = tid & (WAVE - 1), w = tid / WAVE;
+	const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
 	const unsigned long long tsum = wave_sum(live);
 	if (lane == 0) s_red[w] = tsum;
 	__syncthreads();
@@ -522,13 +550,14 @@ __global__ __launch_bounds__(WG) void k_move0(const uint32_t *__restrict__ cell,
 		if (sum) atomicAdd(&ctl->live[tick & 1], (uint32_t)sum);
 	}
 }
+
 /* ------------------------------------------------------------------- sort */
 
 /* Radix sort of the (cell, value) pairs by cell, least significant digit
  * first, digit_bits per pass. Every pass is stable, so survivors end ordered
  * by (cell, agent id) -- the key (cell << 32 | agent id) -- and the first pass
  * drops dead agents (DEAD_KEY). A pass is three kernels:
- *   k_sort_up    digit histogram of every tile (pass 0: built by the move)
+ *   k_sort_up    digit histogram of every tile
  *   k_sort_scan  per digit, exclusive prefix over tiles and the digit total
  *   k_sort_down  per tile: rank in LDS, scatter to bucket start + tile prefix
  * Tile t covers positions [t * sort_tile, (t + 1) * sort_tile). The table
@@ -756,6 +785,470 @@ __global__ __launch_bounds__(WG) void k_sort_down(
 		__syncthreads();
 	}
 	if (bad) set_error(ctl, PP_LIBRARY_ERROR, ctl->tick, -1);
+}
+
+/* --------------------------------------------------------- bucket sort */
+
+/* Agents move at most one cell per iteration, so the sort after the first
+ * iteration needs no general sort. The cells of every replica's grid rows are
+ * cut into blocks of W = 2^S cells; bucket (replica, row y, block j) holds the
+ * cells [x0, x1) of row y. The moved agents lie in the order of their cells
+ * before the move, so the agents that can land in bucket (y, j) are those of
+ * three old buckets, (y-1, j), (y, j) and (y+1, j), plus those of the two
+ * cells just left and right of the block in row y (rows and columns wrap on
+ * the torus): five contiguous ranges of the moved order. The emit counts, per
+ * bucket, the moved agents by old cell (ocnt) and the survivors by new cell
+ * (ncnt); k_bucket_scan turns both into start positions; k_bucket_sort gives
+ * every bucket one work-group, which gathers the agents of its ranges that
+ * land in it, in range order (= agent id order), sorts them by cell in LDS
+ * (stable LSD radix on the cell offset) and writes them at the bucket's
+ * start: the order of the key (cell << 32 | id), as the radix sort gives. */
+
+constexpr uint32_t BCAP = 1024;  /* agents of one bucket sorted in LDS */
+constexpr uint32_t BG_IPT = 8;   /* source positions per thread per round */
+/* Most buckets the automatic choice sorts by buckets (above, the radix sort
+ * is faster: each bucket costs a few dependent round trips to memory). */
+constexpr uint64_t BUCKET_AUTO_MAX = 16384;
+
+/* Exclusive prefix sums of the bucket counts the emit gathered, each with its
+ * total at [NB] (ostart: first moved position of each bucket's old cells;
+ * nstart: first sorted position of each bucket), in chunks of BSC buckets:
+ * k_bscan_part sums each chunk, k_bscan_top scans the chunk sums (one
+ * work-group), k_bscan_down scans inside each chunk and clears the counts
+ * for the next emit. With a single chunk, k_bscan_down alone does all. */
+constexpr uint32_t BSC_IPT = 8;
+constexpr uint32_t BSC = WG * BSC_IPT;   /* buckets per scan chunk */
+
+__global__ __launch_bounds__(WG) void k_bscan_part(const uint32_t *ocnt,
+	const uint32_t *ncnt, uint32_t *part, const Ctl *ctl) {
+	__shared__ unsigned long long s_red[NWAVE][2];
+	if (ctl->error) return;
+	const uint32_t NB = ctl->nbuckets, nch = (NB + BSC - 1) / BSC;
+	for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+		unsigned long long v[2] = {0, 0};
+#pragma unroll
+		for (uint32_t i = 0; i < BSC_IPT; i++) {
+			const uint32_t b = c * BSC + i * WG + threadIdx.x;
+			if (b < NB) {
+				v[0] += ocnt[b];
+				v[1] += ncnt[b];
+			}
+		}
+		const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+#pragma unroll
+		for (int k = 0; k < 2; k++) {
+			const unsigned long long t = wave_sum(v[k]);
+			if (lane == 0) s_red[w][k] = t;
+		}
+		__syncthreads();
+		if (threadIdx.x < 2) {
+			unsigned long long t = 0;
+			for (int i = 0; i < NWAVE; i++) t += s_red[i][threadIdx.x];
+			part[2 * c + threadIdx.x] = (uint32_t)t;
+		}
+		__syncthreads();
+	}
+}
+
+__global__ __launch_bounds__(SCAN_WG) void k_bscan_top(uint32_t *part,
+	uint32_t *ostart, uint32_t *nstart, const Ctl *ctl) {
+	__shared__ uint32_t s_tmp[SCAN_WG / WAVE];
+	if (ctl->error) return;
+	const uint32_t NB = ctl->nbuckets, nch = (NB + BSC - 1) / BSC;
+	uint32_t carry[2] = {0, 0};
+	for (uint32_t c0 = 0; c0 < nch; c0 += SCAN_WG) {
+		const uint32_t c = c0 + threadIdx.x;
+#pragma unroll
+		for (int k = 0; k < 2; k++) {
+			const uint32_t v = c < nch ? part[2 * c + k] : 0u;
+			uint32_t tot;
+			const uint32_t pre = wg_excl_scan<SCAN_WG>(v, s_tmp, tot);
+			if (c < nch) part[2 * c + k] = carry[k] + pre;
+			carry[k] += tot;
+		}
+	}
+	if (threadIdx.x == 0) {
+		ostart[NB] = carry[0];
+		nstart[NB] = carry[1];
+	}
+}
+
+__global__ __launch_bounds__(WG) void k_bscan_down(uint32_t *ocnt,
+	uint32_t *ncnt, uint32_t *ostart, uint32_t *nstart, const uint32_t *part,
+	const Ctl *ctl, uint32_t single) {
+	__shared__ uint32_t s_tmp[NWAVE];
+	if (ctl->error) return;
+	const uint32_t NB = ctl->nbuckets, nch = (NB + BSC - 1) / BSC;
+	for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+		const uint32_t b0 = c * BSC + threadIdx.x * BSC_IPT;
+#pragma unroll
+		for (int k = 0; k < 2; k++) {
+			uint32_t *cnt = k ? ncnt : ocnt;
+			uint32_t *st = k ? nstart : ostart;
+			uint32_t v[BSC_IPT], sum = 0;
+#pragma unroll
+			for (uint32_t i = 0; i < BSC_IPT; i++) {
+				v[i] = b0 + i < NB ? cnt[b0 + i] : 0u;
+				sum += v[i];
+			}
+			uint32_t tot;
+			uint32_t run = wg_excl_scan<WG>(sum, s_tmp, tot) +
+				(single ? 0u : part[2 * c + k]);
+#pragma unroll
+			for (uint32_t i = 0; i < BSC_IPT; i++) {
+				if (b0 + i < NB) {
+					st[b0 + i] = run;
+					cnt[b0 + i] = 0;
+				}
+				run += v[i];
+			}
+			if (single && threadIdx.x == 0) st[NB] = tot;
+		}
+	}
+}
+
+/* Clears the count tables (a tick sorted by the radix sort after a change
+ * of the bucket layout: what the previous emit counted is void). */
+__global__ __launch_bounds__(WG) void k_bscan_clear(uint32_t *ocnt,
+	uint32_t *ncnt, uint32_t words, const Ctl *ctl) {
+	if (ctl->error) return;
+	for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < words; i += gridDim.x * WG) {
+		ocnt[i] = 0;
+		ncnt[i] = 0;
+	}
+}
+
+/* First moved position of cell `c` at the end of the moved range [s, e)
+ * whose cells ascend to c (one wave; 64 positions per probe). */
+__device__ __forceinline__ uint32_t tail_of_cell(const uint32_t *ocell,
+	uint32_t s, uint32_t e, uint32_t c) {
+	const int lane = threadIdx.x & (WAVE - 1);
+	uint32_t p = e;
+	for (;;) {
+		const uint32_t q0 = p > s + WAVE ? p - WAVE : s;
+		const uint32_t pos = q0 + (uint32_t)lane;
+		const bool in = pos < p && ocell[pos] == c;
+		const uint32_t n = (uint32_t)__popcll(__ballot(in));
+		const uint32_t win = p - q0;
+		p -= n;
+		if (n != win || q0 <= s) break;
+	}
+	return p;
+}
+
+/* End of the run of cell `c` at the start of the moved range [s, e) whose
+ * cells ascend from c (one wave). */
+__device__ __forceinline__ uint32_t head_of_cell(const uint32_t *ocell,
+	uint32_t s, uint32_t e, uint32_t c) {
+	const int lane = threadIdx.x & (WAVE - 1);
+	uint32_t p = s;
+	for (;;) {
+		const uint32_t q1 = min(p + WAVE, e);
+		const uint32_t pos = p + (uint32_t)lane;
+		const bool in = pos < q1 && ocell[pos] == c;
+		const uint32_t n = (uint32_t)__popcll(__ballot(in));
+		const uint32_t win = q1 - p;
+		p += n;
+		if (n != win || q1 >= e) break;
+	}
+	return p;
+}
+
+/* Stable rank of digit d (0 .. nd; nd = no element) among the lanes of the
+ * wave: peers with the same digit, and this lane's rank among them. */
+__device__ __forceinline__ void wave_match(uint32_t d, uint32_t bits,
+	uint32_t &rank, uint32_t &count) {
+	const int lane = threadIdx.x & (WAVE - 1);
+	uint64_t peers = ~0ull;
+	for (uint32_t b = 0; b <= bits; b++) {
+		const bool bit = (d >> b) & 1u;
+		const uint64_t bal = __ballot(bit);
+		peers &= bit ? bal : ~bal;
+	}
+	rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+	count = (uint32_t)__popcll(peers);
+}
+
+template <typename GT>
+__global__ __launch_bounds__(WG) void k_bucket_sort(
+	const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+	const uint32_t *__restrict__ ocell, const uint32_t *__restrict__ ostart,
+	const uint32_t *__restrict__ nstart, uint32_t *__restrict__ okeys,
+	uint32_t *__restrict__ ovals, Ctl *ctl, Tables T, Params P,
+	uint32_t *__restrict__ act_cnt, uint32_t act_words, GT *grass_at) {
+	__shared__ uint32_t s_buf[4 * BCAP];  /* offsets A | values A | B | B */
+	__shared__ uint32_t s_wh[NWAVE][65];
+	__shared__ uint32_t s_pos[64];
+	__shared__ uint32_t s_ctot[64];
+	__shared__ uint32_t s_rng[10];
+	__shared__ uint32_t s_nrng;
+	__shared__ uint32_t s_tmp[NWAVE];
+	if (ctl->error) return;
+	const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
+	const uint32_t tick = ctl->tick;
+	/* The tick's first kernel: clear the action tiles' counters and re-stamp
+	 * the alive cells of grass slice (t mod REFRESH) with t (keeps stored
+	 * regrow ticks within the modular window; the state does not change). */
+	for (uint32_t i = blockIdx.x * WG + tid; i < act_words; i += gridDim.x * WG)
+		act_cnt[i] = 0;
+	{
+		const uint32_t per = (P.C + REFRESH - 1) / REFRESH;
+		const uint32_t c0 = (tick % REFRESH) * per;
+		const uint32_t c1 = min(c0 + per, P.C);
+		for (uint32_t c = c0 + blockIdx.x * WG + tid; c < c1; c += gridDim.x * WG) {
+			const GT g = grass_at[c];
+			if (grass_alive_at<GT>(g, tick, P.R) && g != (GT)tick)
+				grass_at[c] = (GT)tick;
+		}
+	}
+	const uint32_t NB = ctl->nbuckets, nrep = ctl->nrep, S = P.bshift;
+	const uint32_t W = 1u << S;
+	/* LSD passes over the S offset bits, at most 6 bits each */
+	const uint32_t npass = (S + 5) / 6;
+	const uint32_t dbits = (S + npass - 1) / npass;
+	const uint32_t nd = 1u << dbits;
+	for (uint32_t b = blockIdx.x; b < NB; b += gridDim.x) {
+		/* the bucket's replica (largest k with bbase[k] <= b), row and block */
+		uint32_t k = 0;
+		{
+			uint32_t lo = 0, hi = nrep;
+			while (hi - lo > 1) {
+				const uint32_t mid = (lo + hi) >> 1;
+				if (T.bbase[mid] <= b) lo = mid;
+				else hi = mid;
+			}
+			k = lo;
+		}
+		const Rep &rp = T.rep[T.order[k]];
+		const uint32_t gx = rp.gx, gy = rp.gy, bb = T.bbase[k];
+		const uint32_t nb = row_blocks(gx, S);
+		const uint32_t local = b - bb;
+		const uint32_t y = local / nb, j = local - y * nb;
+		const uint32_t x0 = j << S, x1 = min(x0 + W, gx);
+		const uint32_t row0 = rp.cb + y * gx;
+		const uint32_t cs = row0 + x0, ce = row0 + x1;
+		const uint32_t out0 = nstart[b], expect = nstart[b + 1] - out0;
+		/* source ranges of the moved order, sorted, without repeats */
+		if (w == 0) {
+			uint32_t rs[5], re[5];
+			const uint32_t bu = bb + ((y + gy - 1) % gy) * nb + j;
+			const uint32_t bm = bb + y * nb + j;
+			const uint32_t bd = bb + ((y + 1) % gy) * nb + j;
+			rs[0] = ostart[bu]; re[0] = ostart[bu + 1];
+			rs[1] = ostart[bm]; re[1] = ostart[bm + 1];
+			rs[2] = ostart[bd]; re[2] = ostart[bd + 1];
+			uint32_t n = 3;
+			if (nb >= 2) {
+				const uint32_t xl = x0 ? x0 - 1 : gx - 1;   /* cell left of the block */
+				const uint32_t bl = bb + y * nb + (xl >> S);
+				const uint32_t el = ostart[bl + 1];
+				rs[3] = tail_of_cell(ocell, ostart[bl], el, row0 + xl);
+				re[3] = el;
+				const uint32_t xr = x1 < gx ? x1 : 0;     /* cell right of it */
+				const uint32_t br = bb + y * nb + (xr >> S);
+				rs[4] = ostart[br];
+				re[4] = head_of_cell(ocell, rs[4], ostart[br + 1], row0 + xr);
+				n = 5;
+			}
+			if (lane == 0) {
+				for (uint32_t a = 1; a < n; a++)
+					for (uint32_t c = a; c > 0 && rs[c] < rs[c - 1]; c--) {
+						const uint32_t ts = rs[c], te = re[c];
+						rs[c] = rs[c - 1]; re[c] = re[c - 1];
+						rs[c - 1] = ts; re[c - 1] = te;
+					}
+				uint32_t m = 0;
+				for (uint32_t a = 0; a < n; a++) {
+					if (rs[a] >= re[a]) continue;
+					if (m && rs[a] == s_rng[2 * (m - 1)]) continue;
+					s_rng[2 * m] = rs[a];
+					s_rng[2 * m + 1] = re[a];
+					m++;
+				}
+				s_nrng = m;
+			}
+		}
+		__syncthreads();
+		/* gather the agents landing in [cs, ce), in range order: the ranges
+		 * as one sequence of positions, BG_IPT consecutive ones per thread,
+		 * all loads of a round issued together */
+		uint32_t *ka = s_buf, *va = s_buf + BCAP;
+		uint32_t *kb = s_buf + 2 * BCAP, *vb = s_buf + 3 * BCAP;
+		uint32_t cnt = 0;
+		const uint32_t nr = s_nrng;
+		uint32_t nsrc = 0;
+		for (uint32_t r = 0; r < nr; r++) nsrc += s_rng[2 * r + 1] - s_rng[2 * r];
+		for (uint32_t v0 = 0; v0 < nsrc; v0 += WG * BG_IPT) {
+			/* position of this thread's first element */
+			uint32_t v = v0 + tid * BG_IPT, r = 0, p = 0, pend = 0;
+			{
+				uint32_t off = v;
+				while (r < nr && off >= s_rng[2 * r + 1] - s_rng[2 * r]) {
+					off -= s_rng[2 * r + 1] - s_rng[2 * r];
+					r++;
+				}
+				if (r < nr) {
+					p = s_rng[2 * r] + off;
+					pend = s_rng[2 * r + 1];
+				}
+			}
+			uint32_t kk[BG_IPT], vv[BG_IPT];
+#pragma unroll
+			for (uint32_t i = 0; i < BG_IPT; i++) {
+				kk[i] = DEAD_KEY;
+				vv[i] = 0u;
+				if (v + i < nsrc) {
+					kk[i] = keys[p];
+					vv[i] = vals[p];
+				}
+				p++;
+				if (p == pend && r + 1 < nr) {
+					r++;
+					p = s_rng[2 * r];
+					pend = s_rng[2 * r + 1];
+				}
+			}
+			uint32_t n_in = 0;
+#pragma unroll
+			for (uint32_t i = 0; i < BG_IPT; i++) n_in += kk[i] >= cs && kk[i] < ce;
+			uint32_t tot;
+			uint32_t slot = cnt + wg_excl_scan<WG>(n_in, s_tmp, tot);
+#pragma unroll
+			for (uint32_t i = 0; i < BG_IPT; i++) {
+				if (kk[i] >= cs && kk[i] < ce) {
+					if (slot < BCAP) {
+						ka[slot] = kk[i] - cs;
+						va[slot] = vv[i];
+					}
+					slot++;
+				}
+			}
+			cnt += tot;
+		}
+		/* the last chunk's LDS writes land before anyone reuses the buffer */
+		__syncthreads();
+		if (cnt != expect) {
+			/* the counts do not describe the agents */
+			if (tid == 0) set_error(ctl, PP_LIBRARY_ERROR, tick, -2 - (int32_t)b);
+			__syncthreads();
+			continue;
+		}
+		if (cnt <= BCAP) {
+			/* stable LSD radix sort of the offsets in LDS */
+			for (uint32_t pass = 0; pass < npass; pass++) {
+				const uint32_t sh = pass * dbits;
+				if (tid < (int)nd) s_pos[tid] = 0;
+				__syncthreads();
+				for (uint32_t i = tid; i < cnt; i += WG)
+					atomicAdd(&s_pos[(ka[i] >> sh) & (nd - 1)], 1u);
+				__syncthreads();
+				if (w == 0) {
+					uint32_t v = lane < (int)nd ? s_pos[lane] : 0u, x = v;
+					for (int o = 1; o < WAVE; o <<= 1) {
+						const uint32_t t = __shfl_up(x, o);
+						if (lane >= o) x += t;
+					}
+					if (lane < (int)nd) s_pos[lane] = x - v;
+				}
+				__syncthreads();
+				for (uint32_t c0 = 0; c0 < cnt; c0 += WG) {
+					const uint32_t i = c0 + tid;
+					const bool valid = i < cnt;
+					const uint32_t off = valid ? ka[i] : 0u;
+					const uint32_t val = valid ? va[i] : 0u;
+					const uint32_t d = valid ? (off >> sh) & (nd - 1) : nd;
+					for (uint32_t q = tid; q < NWAVE * 65; q += WG)
+						(&s_wh[0][0])[q] = 0;
+					__syncthreads();
+					uint32_t rank, count;
+					wave_match(d, dbits, rank, count);
+					if (valid && rank == 0) s_wh[w][d] = count;
+					__syncthreads();
+					if (tid < (int)nd) {
+						uint32_t acc = 0;
+						for (int ww = 0; ww < NWAVE; ww++) {
+							const uint32_t t = s_wh[ww][tid];
+							s_wh[ww][tid] = acc;
+							acc += t;
+						}
+						s_ctot[tid] = acc;
+					}
+					__syncthreads();
+					if (valid) {
+						const uint32_t dst = s_pos[d] + s_wh[w][d] + rank;
+						kb[dst] = off;
+						vb[dst] = val;
+					}
+					__syncthreads();
+					if (tid < (int)nd) s_pos[tid] += s_ctot[tid];
+					__syncthreads();
+				}
+				uint32_t *t;
+				t = ka; ka = kb; kb = t;
+				t = va; va = vb; vb = t;
+			}
+			for (uint32_t i = tid; i < cnt; i += WG) {
+				okeys[out0 + i] = cs + ka[i];
+				ovals[out0 + i] = va[i];
+			}
+		} else {
+			/* More agents than LDS holds (a dense bucket): count per cell,
+			 * then one wave places them in range order straight to HBM. */
+			uint32_t *s_run = s_buf;     /* W <= 4 BCAP counters */
+			for (uint32_t i = tid; i < W; i += WG) s_run[i] = 0;
+			__syncthreads();
+			for (uint32_t r = 0; r < nr; r++) {
+				const uint32_t r1 = s_rng[2 * r + 1];
+				for (uint32_t p = s_rng[2 * r] + tid; p < r1; p += WG) {
+					const uint32_t key = keys[p];
+					if (key >= cs && key < ce) atomicAdd(&s_run[key - cs], 1u);
+				}
+			}
+			__syncthreads();
+			if (w == 0) {
+				const uint32_t per = W / WAVE;   /* W >= 64 */
+				uint32_t sum = 0;
+				for (uint32_t i = 0; i < per; i++) sum += s_run[lane * per + i];
+				uint32_t x = sum;
+				for (int o = 1; o < WAVE; o <<= 1) {
+					const uint32_t t = __shfl_up(x, o);
+					if (lane >= o) x += t;
+				}
+				uint32_t run = x - sum;
+				for (uint32_t i = 0; i < per; i++) {
+					const uint32_t v = s_run[lane * per + i];
+					s_run[lane * per + i] = run;
+					run += v;
+				}
+				__builtin_amdgcn_wave_barrier();
+				for (uint32_t r = 0; r < nr; r++) {
+					const uint32_t r1 = s_rng[2 * r + 1];
+					for (uint32_t p0 = s_rng[2 * r]; p0 < r1; p0 += WAVE) {
+						const uint32_t p = p0 + (uint32_t)lane;
+						const uint32_t key = p < r1 ? keys[p] : DEAD_KEY;
+						const bool in = key >= cs && key < ce;
+						const uint32_t off = in ? key - cs : W;
+						uint32_t rank, count;
+						wave_match(off, S, rank, count);
+						if (in) {
+							const uint32_t dst = s_run[off] + rank;
+							if (dst < expect) {
+								okeys[out0 + dst] = key;
+								ovals[out0 + dst] = vals[p];
+							} else {
+								set_error(ctl, PP_LIBRARY_ERROR, tick,
+									-(int32_t)(1u << 30) - (int32_t)b);
+							}
+						}
+						__builtin_amdgcn_wave_barrier();
+						if (in && rank == 0) s_run[off] += count;
+						__builtin_amdgcn_wave_barrier();
+					}
+				}
+			}
+		}
+		__syncthreads();
+	}
 }
 
 /* ----------------------------------------------------------------- action */
@@ -1091,18 +1584,14 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
  * are summed into its control words, in registers over the tiles of one
  * replica. Tiles holding agents of several replicas are listed for
  * k_act_mixed (segments never span replicas, so the two kernels touch
- * disjoint cells and values). This kernel also clears the pass-0 histogram
- * table that k_emit_move fills for the next iteration. */
+ * disjoint cells and values). */
 template <typename GT>
 __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 	uint32_t *vals, GT *grass_at, uint32_t *act_cnt, uint32_t *mixed_list,
-	uint32_t *__restrict__ hist, uint32_t hist_words, Ctl *ctl, Tables T,
-	Params P) {
+	Ctl *ctl, Tables T, Params P) {
 	__shared__ ActLds<GT> S;
 	if (ctl->error) return;
 	const int tid = threadIdx.x;
-	for (uint32_t i = blockIdx.x * WG + tid; i < hist_words; i += gridDim.x * WG)
-		hist[i] = 0;
 	const uint32_t tick = ctl->tick, nrep = ctl->nrep;
 	const uint32_t L = ctl->live[tick & 1];
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
@@ -1187,8 +1676,6 @@ __global__ __launch_bounds__(WG) void k_act_mixed(
  * that reached its iteration count is done, one above its capacity failed,
  * and the emit drops the agents of both. The batch fails when its outputs
  * exceed the agent buffers. */
-constexpr int SCAN_WG = 1024;
-constexpr int SCAN_UNROLL = 8;
 
 __global__ __launch_bounds__(SCAN_WG) void k_act_scan(
 	uint32_t *__restrict__ act_cnt, Ctl *ctl, Tables T, Params P) {
@@ -1302,23 +1789,28 @@ __global__ __launch_bounds__(SCAN_WG) void k_act_scan(
 	ctl->tick = tick + 1;
 }
 
+constexpr uint32_t BWIN = 512;   /* buckets per LDS count window */
+
 /* Writes the staged outputs [0, total) of an action tile (batch positions
- * goff + i) to the SoA state, moves each for the next iteration and counts
- * its pass-0 digit. MIXED = false: every output belongs to replica m (its
- * constants stay uniform across the group); true: each looks up its own.
- * Returns the number of survivors of the move. */
-template <uint32_t R, bool MIXED>
+ * goff + i) to the SoA state, moves each for the next iteration, and counts
+ * each output in the bucket of its cell (ocnt: the order the next sort reads
+ * them in) and each survivor of the move in the bucket of its new cell
+ * (ncnt), through LDS windows of BWIN buckets from w0. MIXED = false: every
+ * output belongs to replica m_u (its constants stay uniform across the
+ * group); true: each looks up its own. Returns the survivors of the move. */
+template <bool MIXED>
 __device__ __forceinline__ unsigned long long emit_outputs(
-	const uint32_t *s_oc, const uint32_t *s_ov, uint32_t *s_h, uint32_t total,
-	uint32_t goff, uint32_t to0, uint32_t dmask, uint32_t *__restrict__ ocell,
+	const uint32_t *s_oc, const uint32_t *s_ov, uint32_t *s_win, uint32_t w0,
+	uint32_t total, uint32_t goff, uint32_t *__restrict__ ocell,
 	int32_t *__restrict__ oenergy, uint8_t *__restrict__ otype,
 	uint32_t *__restrict__ mkeys, uint32_t *__restrict__ mvals,
-	const Tables &T, uint32_t nrep, uint32_t tick, const RM &m_u,
-	const Params &P, bool &overflow) {
+	uint32_t *ocnt, uint32_t *ncnt, const Tables &T, uint32_t nrep,
+	uint32_t tick, const RM &m_u, const Params &P, bool &overflow) {
 	unsigned long long live = 0;
+	const uint32_t S = P.bshift;
 	for (uint32_t i0 = 0; i0 < total; i0 += WG) {
 		const uint32_t i = i0 + threadIdx.x;
-		uint32_t d = NO_DIGIT;
+		uint32_t ob = NO_DIGIT, nbk = NO_DIGIT;
 		if (i < total) {
 			const uint32_t id = goff + i;
 			const uint32_t cc = s_oc[i], ov = s_ov[i];
@@ -1327,24 +1819,21 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 			oenergy[id] = (int32_t)en;
 			otype[id] = (uint8_t)t;
 			uint32_t mv = 0, mk = DEAD_KEY;
-			if (MIXED) {
-				const uint32_t sl = T.order[rep_index(T, nrep, cc)];
-				const RM m = load_rm(T.rep[sl], T.rctl[sl], tick);
-				if (m.active)
-					mk = move_agent(id - m.base, cc - m.cb, (int32_t)en, t, m, &mv,
-						overflow);
-			} else if (m_u.active) {
-				mk = move_agent(id - m_u.base, cc - m_u.cb, (int32_t)en, t, m_u,
-					&mv, overflow);
-			}
+			const RM m = MIXED ? load_rm(T, rep_index(T, nrep, cc), tick, S) : m_u;
+			const uint32_t l = cc - m.cb;
+			const uint32_t y = l / m.gx, x = l - y * m.gx;
+			if (m.active)
+				mk = move_agent(id - m.base, x, y, (int32_t)en, t, m, S, &mv, &nbk,
+					overflow);
+			ob = m.bbase + y * m.nb + (x >> S);
 			mkeys[id] = mk;
 			mvals[id] = mv;
-			if (mk != DEAD_KEY) {
-				live += 1;
-				d = (id / P.sort_tile - to0) * R + (mk & dmask);
-			}
+			live += mk != DEAD_KEY;
 		}
-		wave_hist_add(s_h, d);
+		if (P.bucket_on) {
+			wave_bucket_add<BWIN>(s_win, ocnt, w0, ob);
+			wave_bucket_add<BWIN>(s_win + BWIN, ncnt, w0, nbk);
+		}
 	}
 	return live;
 }
@@ -1355,21 +1844,17 @@ __device__ __forceinline__ unsigned long long emit_outputs(
  * offset from k_act_scan plus a work-group scan), and an agent's local id is
  * its position minus its replica's base. Each output is staged in LDS, then
  * written with coalesced stores to the SoA state and immediately moved for
- * tick t+1: its (key, value) pair goes to its position and its pass-0 digit
- * is counted for the sort tile holding it. Agents of a replica that is done
- * or failed get DEAD_KEY. The outputs of one action tile (at most
- * 2 x A_TILE) span at most two sort tiles. */
-template <int RBITS>
+ * tick t+1; the bucket counts of the next sort are summed on the way.
+ * Agents of a replica that is done or failed get DEAD_KEY. */
 __global__ __launch_bounds__(WG) void k_emit_move(
 	const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
 	const uint32_t *__restrict__ act_off, uint32_t *__restrict__ ocell,
 	int32_t *__restrict__ oenergy, uint8_t *__restrict__ otype,
 	uint32_t *__restrict__ mkeys, uint32_t *__restrict__ mvals,
-	uint32_t *__restrict__ hist, Ctl *ctl, Tables T, Params P) {
-	constexpr uint32_t R = 1u << RBITS;
+	uint32_t *ocnt, uint32_t *ncnt, Ctl *ctl, Tables T, Params P) {
 	__shared__ uint32_t s_oc[A_STAGE];
 	__shared__ uint32_t s_ov[A_STAGE];
-	__shared__ uint32_t s_h[2 * R];
+	__shared__ uint32_t s_win[2 * BWIN];
 	__shared__ uint32_t s_tmp[NWAVE];
 	__shared__ unsigned long long s_red[NWAVE];
 
@@ -1379,17 +1864,14 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	const uint32_t L = ctl->live[(tick - 1) & 1];
 	const uint32_t nrep = ctl->nrep;
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
-	const uint32_t dmask = (1u << P.digit_bits) - 1u;
+	const uint32_t S = P.bshift;
 	unsigned long long live = 0;
 	bool overflow = false;
 	/* A batch of one replica (every plain simulation) needs no lookups. */
 	RM m_one{};
-	if (nrep == 1 && ntiles > blockIdx.x) {
-		const uint32_t s0 = T.order[0];
-		m_one = load_rm(T.rep[s0], T.rctl[s0], tick);
-	}
+	if (nrep == 1 && ntiles > blockIdx.x) m_one = load_rm(T, 0, tick, S);
 
-	for (uint32_t i = tid; i < 2 * R; i += WG) s_h[i] = 0;
+	for (uint32_t i = tid; i < 2 * BWIN; i += WG) s_win[i] = 0;
 	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 		const uint32_t base = tile * A_TILE;
 		const uint32_t goff = act_off[tile];
@@ -1430,37 +1912,46 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 			}
 		}
 		__syncthreads();
-		/* Replicas of this tile's positions (keys ascend); uniform over the
-		 * group. */
+		/* Replicas of this tile's positions (keys ascend), and the first
+		 * bucket of the count window: the bucket of the tile's first cell,
+		 * less BWIN/4 for moves to the row above. Uniform over the group. */
+		const uint32_t c_first = __builtin_amdgcn_readfirstlane(keys[base]);
 		bool mixed = false;
 		RM m_t = m_one;
+		uint32_t k_first = 0;
 		if (nrep != 1) {
-			const uint32_t k_first = rep_index(T, nrep,
-				__builtin_amdgcn_readfirstlane(keys[base]));
+			k_first = rep_index(T, nrep, c_first);
 			const uint32_t k_last = rep_index(T, nrep,
 				__builtin_amdgcn_readfirstlane(keys[min(base + A_TILE, L) - 1]));
 			mixed = k_first != k_last;
-			const uint32_t s0 = T.order[k_first];
-			m_t = load_rm(T.rep[s0], T.rctl[s0], tick);
+			m_t = load_rm(T, k_first, tick, S);
 		}
-		const uint32_t to0 = goff / P.sort_tile;
+		uint32_t w0 = 0;
+		{
+			const uint32_t l = c_first - m_t.cb;
+			const uint32_t y = l / m_t.gx, x = l - y * m_t.gx;
+			const uint32_t b0 = m_t.bbase + y * m_t.nb + (x >> S);
+			w0 = b0 > BWIN / 4 ? b0 - BWIN / 4 : 0u;
+		}
 		if (!mixed)
-			live += emit_outputs<R, false>(s_oc, s_ov, s_h, total, goff, to0, dmask,
-				ocell, oenergy, otype, mkeys, mvals, T, nrep, tick, m_t, P,
+			live += emit_outputs<false>(s_oc, s_ov, s_win, w0, total, goff, ocell,
+				oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
 				overflow);
 		else
-			live += emit_outputs<R, true>(s_oc, s_ov, s_h, total, goff, to0, dmask,
-				ocell, oenergy, otype, mkeys, mvals, T, nrep, tick, m_t, P,
+			live += emit_outputs<true>(s_oc, s_ov, s_win, w0, total, goff, ocell,
+				oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
 				overflow);
 		__syncthreads();
-		for (uint32_t i = tid; i < 2 * R; i += WG) {
-			const uint32_t h = s_h[i];
-			if (h) {
-				atomicAdd(&hist[(size_t)(i % R) * P.hist_cols + to0 + i / R], h);
-				s_h[i] = 0;
+		if (P.bucket_on) {
+			for (uint32_t i = tid; i < 2 * BWIN; i += WG) {
+				const uint32_t h = s_win[i];
+				if (h) {
+					atomicAdd(&(i < BWIN ? ocnt : ncnt)[w0 + (i % BWIN)], h);
+					s_win[i] = 0;
+				}
 			}
+			__syncthreads();
 		}
-		__syncthreads();
 	}
 	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
 	const int lane = tid & (WAVE - 1), w = tid / WAVE;
@@ -1488,12 +1979,15 @@ enum KernelId {
 	KI_ACT,
 	KI_ACT_SCAN,
 	KI_EMIT_MOVE,
+	KI_BUCKET_SCAN,
+	KI_BUCKET_SORT,
 	KI_COUNT
 };
 
 const char *const kernel_names[KI_COUNT] = {
 	"init_reset", "init_cells", "init_agents", "init_done", "move0",
 	"sort_up", "sort_scan", "sort_down", "act", "act_scan", "emit_move",
+	"bucket_scan", "bucket_sort",
 };
 
 struct PendingEvent {
@@ -1622,6 +2116,17 @@ struct pp_batch {
 	uint32_t *act_cnt = nullptr;
 	uint32_t *mixed_list = nullptr;  /* action tiles of several replicas */
 	uint32_t act_words = 0;
+	/* bucket sort: per-bucket counts (emit) and starts (scan), nb_cap + 1
+	 * entries each; first bucket of each replica of the table */
+	uint32_t *ocnt = nullptr, *ncnt = nullptr;
+	uint32_t *ostart = nullptr, *nstart = nullptr;
+	uint32_t *bbase_dev = nullptr;
+	uint32_t *bpart = nullptr;        /* scan chunk sums, 2 per chunk */
+	int need_radix = 1;   /* the next tick sorts with the radix sort */
+	/* 0: radix sort every tick; 1: automatic; 8..11: row buckets of
+	 * 2^mode cells */
+	uint32_t sort_mode = 1;
+	uint32_t grid_bucket = 0;
 	Ctl *ctl = nullptr;
 	Ctl *ctl_host = nullptr;        /* pinned mirror */
 	int ctl_fresh = 0;              /* mirror matches the device */
@@ -1634,7 +2139,8 @@ struct pp_batch {
 	pp_stats64_t *stats_dev = nullptr;
 	pp_stats64_t *stats_host = nullptr; /* pinned, same layout */
 
-	hipGraphExec_t graph_exec = nullptr;
+	hipGraphExec_t graph_exec = nullptr;   /* 8 bucket-sort ticks */
+	hipGraphExec_t graph_radix = nullptr;  /* 8 radix-sort ticks */
 	int graph_steps = 0;
 
 	int profiling = 0;
@@ -1650,7 +2156,7 @@ struct pp_batch {
 	uint32_t next_id = 0;
 
 	Tables tables() const {
-		return Tables{rep_dev, rctl_dev, cbs_dev, order_dev};
+		return Tables{rep_dev, rctl_dev, cbs_dev, order_dev, bbase_dev};
 	}
 };
 
@@ -1736,12 +2242,10 @@ template <int KPT, int RBITS>
 static void launch_sort(pp_batch_t *b, uint32_t pass) {
 	PendingEvent pe;
 	const int src = pass_src(pass), dst = pass_dst(pass);
-	if (pass > 0) {
-		prof_begin(b, KI_SORT_UP, pe);
-		k_sort_up<KPT><<<b->grid_up, WG, 0, b->stream>>>(b->keys[src], b->hist,
-			b->ctl, pass, b->P);
-		prof_end(b, pe);
-	}
+	prof_begin(b, KI_SORT_UP, pe);
+	k_sort_up<KPT><<<b->grid_up, WG, 0, b->stream>>>(b->keys[src], b->hist,
+		b->ctl, pass, b->P);
+	prof_end(b, pe);
 	prof_begin(b, KI_SORT_SCAN, pe);
 	if (b->grass_bytes == 1) launch_sort_scan<uint8_t>(b, pass);
 	else if (b->grass_bytes == 2) launch_sort_scan<uint16_t>(b, pass);
@@ -1764,25 +2268,11 @@ static void launch_sort_pass(pp_batch_t *b, uint32_t pass) {
 	}
 }
 
-/* Pass-0 histograms of the pairs in buffer 0 for the current shape (after
- * a change of shape; otherwise the emit builds them). */
-static void launch_hist0(pp_batch_t *b) {
-	PendingEvent pe;
-	prof_begin(b, KI_SORT_UP, pe);
-	if (b->shape.kpt == 8)
-		k_sort_up<8><<<b->grid_up, WG, 0, b->stream>>>(b->keys[0], b->hist,
-			b->ctl, 0, b->P);
-	else
-		k_sort_up<16><<<b->grid_up, WG, 0, b->stream>>>(b->keys[0], b->hist,
-			b->ctl, 0, b->P);
-	prof_end(b, pe);
-}
-
 template <typename GT>
 static void launch_act(pp_batch_t *b, int sorted) {
 	k_act<GT><<<b->grid_act, WG, 0, b->stream>>>(b->keys[sorted],
-		b->vals[sorted], (GT *)b->grass_at, b->act_cnt, b->mixed_list, b->hist,
-		b->hist_words, b->ctl, b->tables(), b->P);
+		b->vals[sorted], (GT *)b->grass_at, b->act_cnt, b->mixed_list, b->ctl,
+		b->tables(), b->P);
 	/* A batch of one replica never has a tile of several. */
 	if (b->max_rep > 1)
 		k_act_mixed<GT><<<min_u32(b->max_rep, 256), WG, 0, b->stream>>>(
@@ -1790,13 +2280,49 @@ static void launch_act(pp_batch_t *b, int sorted) {
 			b->mixed_list, b->ctl, b->tables(), b->P);
 }
 
+template <typename GT>
+static void launch_bucket_sort(pp_batch_t *b) {
+	k_bucket_sort<GT><<<b->grid_bucket, WG, 0, b->stream>>>(b->keys[0],
+		b->vals[0], b->cell, b->ostart, b->nstart, b->keys[1], b->vals[1],
+		b->ctl, b->tables(), b->P, b->act_cnt, b->act_words, (GT *)b->grass_at);
+}
+
 /* Enqueue one tick (no synchronisation): the sort of the pairs the previous
- * tick's emit (or the moves at admission) produced, the actions, the scan +
- * end of tick, and the emit + move of the next tick. */
-static int enqueue_step(pp_batch_t *b) {
+ * tick's emit (or the moves at admission) produced -- the bucket sort, or the
+ * radix sort when `radix` -- the actions, the scan + end of tick, and the
+ * emit + move of the next tick. */
+static int enqueue_step(pp_batch_t *b, int radix) {
 	PendingEvent pe;
-	for (uint32_t pass = 0; pass < b->P.passes; pass++) launch_sort_pass(b, pass);
-	const int sorted = pass_dst(b->P.passes - 1);
+	int sorted = 1;
+	prof_begin(b, KI_BUCKET_SCAN, pe);
+	if (radix) {
+		/* counts of a previous layout are void */
+		if (b->P.bucket_on)
+			k_bscan_clear<<<b->grid_bucket, WG, 0, b->stream>>>(b->ocnt, b->ncnt,
+				b->P.nb_cap + 1, b->ctl);
+	} else if (b->P.bscan_one) {
+		k_bscan_down<<<1, WG, 0, b->stream>>>(b->ocnt, b->ncnt, b->ostart,
+			b->nstart, b->bpart, b->ctl, 1);
+	} else {
+		k_bscan_part<<<b->grid_bucket, WG, 0, b->stream>>>(b->ocnt, b->ncnt,
+			b->bpart, b->ctl);
+		k_bscan_top<<<1, SCAN_WG, 0, b->stream>>>(b->bpart, b->ostart,
+			b->nstart, b->ctl);
+		k_bscan_down<<<b->grid_bucket, WG, 0, b->stream>>>(b->ocnt, b->ncnt,
+			b->ostart, b->nstart, b->bpart, b->ctl, 0);
+	}
+	prof_end(b, pe);
+	if (radix) {
+		for (uint32_t pass = 0; pass < b->P.passes; pass++)
+			launch_sort_pass(b, pass);
+		sorted = pass_dst(b->P.passes - 1);
+	} else {
+		prof_begin(b, KI_BUCKET_SORT, pe);
+		if (b->grass_bytes == 1) launch_bucket_sort<uint8_t>(b);
+		else if (b->grass_bytes == 2) launch_bucket_sort<uint16_t>(b);
+		else launch_bucket_sort<uint32_t>(b);
+		prof_end(b, pe);
+	}
 	prof_begin(b, KI_ACT, pe);
 	if (b->grass_bytes == 1) launch_act<uint8_t>(b, sorted);
 	else if (b->grass_bytes == 2) launch_act<uint16_t>(b, sorted);
@@ -1807,14 +2333,9 @@ static int enqueue_step(pp_batch_t *b) {
 		b->P);
 	prof_end(b, pe);
 	prof_begin(b, KI_EMIT_MOVE, pe);
-	if (b->rbits == 11)
-		k_emit_move<11><<<b->grid_emit, WG, 0, b->stream>>>(b->keys[sorted],
-			b->vals[sorted], b->act_cnt, b->cell, b->energy, b->type,
-			b->keys[0], b->vals[0], b->hist, b->ctl, b->tables(), b->P);
-	else
-		k_emit_move<8><<<b->grid_emit, WG, 0, b->stream>>>(b->keys[sorted],
-			b->vals[sorted], b->act_cnt, b->cell, b->energy, b->type,
-			b->keys[0], b->vals[0], b->hist, b->ctl, b->tables(), b->P);
+	k_emit_move<<<b->grid_emit, WG, 0, b->stream>>>(b->keys[sorted],
+		b->vals[sorted], b->act_cnt, b->cell, b->energy, b->type, b->keys[0],
+		b->vals[0], b->ocnt, b->ncnt, b->ctl, b->tables(), b->P);
 	prof_end(b, pe);
 	HIP_OK(hipGetLastError());
 	return PP_SUCCESS;
@@ -1822,7 +2343,9 @@ static int enqueue_step(pp_batch_t *b) {
 
 static void drop_graph(pp_batch_t *b) {
 	if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+	if (b->graph_radix) (void)hipGraphExecDestroy(b->graph_radix);
 	b->graph_exec = nullptr;
+	b->graph_radix = nullptr;
 	b->graph_steps = 0;
 }
 
@@ -1842,39 +2365,49 @@ static void apply_shape(pp_batch_t *b, const SortShape &sh) {
 	b->grid_down = min_u32(P.hist_cols, (b->rbits == 11 ? 1 : 2) * cu);
 	b->grid_act = min_u32(ceil_div(P.cap, A_TILE), 8 * cu);
 	b->grid_emit = min_u32(ceil_div(P.cap, A_TILE), 8 * cu);
+	b->grid_bucket = min_u32(P.nb_cap, 8 * cu);
 }
 
 /* Capture `steps` ticks into one executable graph (kernels only; no memset
  * or copy nodes). */
-static int build_graph(pp_batch_t *b, int steps) {
+static int build_graph(pp_batch_t *b, int steps, int radix,
+	hipGraphExec_t *out) {
 	hipGraph_t g = nullptr;
 	HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
 	int rc = PP_SUCCESS;
-	for (int i = 0; i < steps && rc == PP_SUCCESS; i++) rc = enqueue_step(b);
+	for (int i = 0; i < steps && rc == PP_SUCCESS; i++) rc = enqueue_step(b, radix);
 	hipError_t e = hipStreamEndCapture(b->stream, &g);
 	if (rc != PP_SUCCESS) {
 		if (g) (void)hipGraphDestroy(g);
 		return rc;
 	}
 	HIP_OK(e);
-	e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+	e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
 	(void)hipGraphDestroy(g);
 	HIP_OK(e);
 	b->graph_steps = steps;
 	return PP_SUCCESS;
 }
 
-/* Enqueue `ticks` ticks: whole graphs of 8, then single launches. */
+/* Enqueue `ticks` ticks: after a change of the replica table the first one
+ * sorts with the radix sort (the moved order is not the cell order yet, or
+ * the bucket layout changed); then whole graphs of 8, then single launches. */
 static int enqueue_ticks(pp_batch_t *b, uint32_t ticks) {
 	b->ctl_fresh = 0;
 	const int use_graph = !b->profiling && getenv("PPHPC_NO_GRAPH") == nullptr;
+	const int radix = !b->P.bucket_on;
 	uint32_t i = 0;
-	if (use_graph && ticks >= 8) {
-		if (!b->graph_exec) PP_TRY(build_graph(b, 8));
-		for (; i + (uint32_t)b->graph_steps <= ticks; i += b->graph_steps)
-			HIP_OK(hipGraphLaunch(b->graph_exec, b->stream));
+	if (ticks && b->need_radix && !radix) {
+		PP_TRY(enqueue_step(b, 1));
+		i = 1;
 	}
-	for (; i < ticks; i++) PP_TRY(enqueue_step(b));
+	b->need_radix = 0;
+	if (use_graph && ticks - i >= 8) {
+		hipGraphExec_t *g = radix ? &b->graph_radix : &b->graph_exec;
+		if (!*g) PP_TRY(build_graph(b, 8, radix, g));
+		for (; i + 8 <= ticks; i += 8) HIP_OK(hipGraphLaunch(*g, b->stream));
+	}
+	for (; i < ticks; i++) PP_TRY(enqueue_step(b, radix));
 	return PP_SUCCESS;
 }
 
@@ -1907,8 +2440,38 @@ static int upload_table(pp_batch_t *b) {
 			hipMemcpyHostToDevice));
 	}
 	const uint32_t nrep = (uint32_t)t.size();
+	/* bucket layout: rows of blocks of 2^bshift cells, replicas in table
+	 * order */
+	const uint32_t shift = b->sort_mode >= 8 ? b->sort_mode : 11u;
+	std::vector<uint32_t> bbase(nrep + 1);
+	uint64_t nbk = 0;
+	for (uint32_t i = 0; i < nrep; i++) {
+		bbase[i] = (uint32_t)std::min<uint64_t>(nbk, 0xFFFFFFFFull);
+		const Rep &rp = b->rep_host[order[i]];
+		const uint64_t W = 1ull << shift;
+		nbk += (uint64_t)rp.gy * ((rp.gx + W - 1) / W);
+	}
+	bbase[nrep] = (uint32_t)std::min<uint64_t>(nbk, 0xFFFFFFFFull);
+	HIP_OK(hipMemcpy(b->bbase_dev, bbase.data(), (nrep + 1) * 4,
+		hipMemcpyHostToDevice));
+	/* Row buckets when chosen, or, automatically, while few enough that
+	 * their per-bucket latency stays below the radix sort's passes. */
+	const uint32_t on = (b->sort_mode != 0 && nbk <= b->P.nb_cap &&
+		(b->sort_mode != 1 || nbk <= BUCKET_AUTO_MAX)) ? 1u : 0u;
+	const uint32_t one = nbk <= BSC ? 1u : 0u;
+	if (on != b->P.bucket_on || one != b->P.bscan_one ||
+		shift != b->P.bshift) {
+		b->P.bucket_on = on;
+		b->P.bscan_one = one;
+		b->P.bshift = shift;
+		drop_graph(b);     /* Params are launch arguments of the graphs */
+	}
+	const uint32_t nb32 = on ? (uint32_t)nbk : 0u;
 	HIP_OK(hipMemcpy(&b->ctl->nrep, &nrep, 4, hipMemcpyHostToDevice));
+	HIP_OK(hipMemcpy(&b->ctl->nbuckets, &nb32, 4, hipMemcpyHostToDevice));
 	b->ctl_host->nrep = nrep;
+	b->ctl_host->nbuckets = nb32;
+	b->need_radix = 1;
 	return PP_SUCCESS;
 }
 
@@ -1953,6 +2516,9 @@ static void release_slot(pp_batch_t *b, Replica &r) {
  * and reservations of replicas that are done or failed. */
 static int collect(pp_batch_t *b) {
 	PP_TRY(read_ctl(b));
+	if (b->ctl_host->error && getenv("PPHPC_DEBUG"))
+		fprintf(stderr, "pphpc: batch error %d at tick %u (detail %d)\n",
+			b->ctl_host->error, b->ctl_host->err_step, b->ctl_host->err_rep);
 	std::vector<Replica *> got;
 	for (auto &kv : b->reps) {
 		Replica &r = kv.second;
@@ -2059,6 +2625,12 @@ static void batch_free(pp_batch_t *b) {
 	(void)hipFree(b->totals);
 	(void)hipFree(b->act_cnt);
 	(void)hipFree(b->mixed_list);
+	(void)hipFree(b->ocnt);
+	(void)hipFree(b->ncnt);
+	(void)hipFree(b->ostart);
+	(void)hipFree(b->nstart);
+	(void)hipFree(b->bbase_dev);
+	(void)hipFree(b->bpart);
 	(void)hipFree(b->ctl);
 	(void)hipFree(b->rep_dev);
 	(void)hipFree(b->rctl_dev);
@@ -2073,10 +2645,19 @@ static void batch_free(pp_batch_t *b) {
 }
 
 /* Device bytes of a batch (the allocations of batch_alloc). */
+/* Bucket capacity of the count tables: one bucket per 64 cells and 4096
+ * more (rows narrower than a block have one bucket each; a layout with more
+ * buckets than this sorts with the radix sort). */
+static uint32_t nb_cap_for(uint64_t cells) {
+	return (uint32_t)std::min<uint64_t>(cells / 64 + 4096, 0x7FFFFFFFull);
+}
+
 static uint64_t batch_bytes(uint64_t A, uint64_t C, uint32_t max_rep,
 	uint32_t R, uint32_t stats_rows, uint64_t hist_words) {
 	const uint64_t A4 = (A + 3) & ~3ull;
 	uint64_t t = A4 * (4 + 4 + 1) + 6 * A4 * 4;
+	t += 16ull * ((uint64_t)nb_cap_for(C) + 1) + 4ull * (max_rep + 1);
+	t += 8ull * ((uint64_t)nb_cap_for(C) / BSC + 2);
 	t += C * grass_bytes_for(R);
 	t += (uint64_t)max_rep * (R + 1) * 8;
 	t += hist_words * 4 + RADIX_MAX * 4;
@@ -2117,6 +2698,15 @@ static int batch_alloc(pp_batch_t *b) {
 	b->act_words = ceil_div(A, A_TILE) + 1;
 	HIP_OK(hipMalloc(&b->act_cnt, (size_t)b->act_words * 4));
 	HIP_OK(hipMalloc(&b->mixed_list, (size_t)b->act_words * 4));
+	const size_t nbw = (size_t)b->P.nb_cap + 1;
+	HIP_OK(hipMalloc(&b->ocnt, nbw * 4));
+	HIP_OK(hipMalloc(&b->ncnt, nbw * 4));
+	HIP_OK(hipMalloc(&b->ostart, nbw * 4));
+	HIP_OK(hipMalloc(&b->nstart, nbw * 4));
+	HIP_OK(hipMemset(b->ocnt, 0, nbw * 4));
+	HIP_OK(hipMemset(b->ncnt, 0, nbw * 4));
+	HIP_OK(hipMalloc(&b->bbase_dev, ((size_t)b->max_rep + 1) * 4));
+	HIP_OK(hipMalloc(&b->bpart, ((size_t)b->P.nb_cap / BSC + 2) * 8));
 	HIP_OK(hipMalloc(&b->ctl, sizeof(Ctl)));
 	HIP_OK(hipMalloc(&b->rep_dev, (size_t)b->max_rep * sizeof(Rep)));
 	HIP_OK(hipMalloc(&b->rctl_dev, (size_t)b->max_rep * sizeof(RepCtl)));
@@ -2171,6 +2761,13 @@ static int batch_create(uint64_t agents, uint64_t cells, uint32_t max_rep,
 	b->P.cap = (uint32_t)agents;
 	b->P.C = (uint32_t)cells;
 	b->P.R = max_R;
+	b->P.nb_cap = nb_cap_for(cells);
+	b->P.bshift = 11;
+	const char *bs_env = getenv("PPHPC_SORT");
+	if (bs_env) {
+		const int v = atoi(bs_env);
+		if (v == 0 || v == 1 || (v >= 8 && v <= 11)) b->sort_mode = (uint32_t)v;
+	}
 	b->max_rep = max_rep;
 	b->stats_rows = stats_rows_for(max_rep);
 	b->key_bits = key_bits_for(cells);
@@ -2217,6 +2814,7 @@ static int batch_reset(pp_batch_t *b) {
 	b->free_cells.clear();
 	b->free_cells[0] = b->P.C;
 	b->reserved = 0;
+	b->need_radix = 1;
 	return PP_SUCCESS;
 }
 
@@ -2347,8 +2945,8 @@ static int admit(pp_batch_t *b, const pp_params_t *p, uint64_t seed,
 	k_rep_init_done<<<1, 64, 0, b->stream>>>(b->ctl, T, (uint32_t)slot, pos);
 	prof_end(b, pe);
 	prof_begin(b, KI_MOVE0, pe);
-	k_move0<<<min_u32(ceil_div(n0 ? n0 : 1, WG * M_IPT), 8 * cu), WG, 0,
-		b->stream>>>(b->cell, b->energy, b->type, b->keys[0], b->vals[0], b->hist,
+	k_move0<<<min_u32(ceil_div(n0 ? n0 : 1, WG), 8 * cu), WG, 0,
+		b->stream>>>(b->cell, b->energy, b->type, b->keys[0], b->vals[0],
 		b->ctl, T, (uint32_t)slot, pos, b->P);
 	prof_end(b, pe);
 	HIP_OK(hipGetLastError());
@@ -2603,6 +3201,11 @@ int pp_sim_info(const pp_sim_t *s, uint32_t *sort_passes, uint32_t *sort_tile,
 	return pp_batch_info(s->b, sort_passes, sort_tile, action_tile, grid_move);
 }
 
+int pp_sim_set_sort(pp_sim_t *s, uint32_t bucket_shift) {
+	if (!s) return PP_INVALID_ARGS;
+	return pp_batch_set_sort(s->b, bucket_shift);
+}
+
 int pp_sim_sync(pp_sim_t *s) {
 	if (!s) return PP_INVALID_ARGS;
 	return pp_batch_sync(s->b);
@@ -2742,37 +3345,53 @@ int pp_batch_set_shape(pp_batch_t *b, uint32_t keys_per_thread,
 	if (sh.kpt == b->shape.kpt && sh.max_bits == b->shape.max_bits)
 		return PP_SUCCESS;
 	apply_shape(b, sh);
-	/* The emit built the pass-0 histograms for the old shape. */
-	launch_hist0(b);
-	HIP_OK(hipGetLastError());
-	b->ctl_fresh = 0;
 	return PP_SUCCESS;
 }
 
-int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks, uint32_t *keys_per_thread,
-	uint32_t *max_digit_bits, double *sec_per_tick) {
-	if (!b || trial_ticks == 0 || trial_ticks > b->stats_rows / 2)
+int pp_batch_set_sort(pp_batch_t *b, uint32_t mode) {
+	if (!b) return PP_INVALID_ARGS;
+	if (mode > 1 && (mode < 8 || mode > 11)) return PP_INVALID_ARGS;
+	HIP_OK(hipSetDevice(b->device));
+	if (mode == b->sort_mode) return PP_SUCCESS;
+	if (!b->ctl_fresh) PP_TRY(read_ctl(b));
+	b->sort_mode = mode;
+	return upload_table(b);    /* new layout; the next tick re-sorts */
+}
+
+int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks, uint32_t *bucket_shift,
+	uint32_t *keys_per_thread, uint32_t *max_digit_bits, double *sec_per_tick) {
+	if (!b || trial_ticks == 0 || trial_ticks + 8 > b->stats_rows / 2)
 		return PP_INVALID_ARGS;
 	HIP_OK(hipSetDevice(b->device));
-	const std::vector<SortShape> cands = tune_candidates(b->key_bits, b->P.cap);
-	SortShape best = b->shape;
+	/* candidates: the bucket sort at each block width, then the radix sort
+	 * at each shape */
+	struct Cand { uint32_t shift; SortShape sh; };
+	std::vector<Cand> cands;
+	for (uint32_t sft = 8; sft <= 11; sft++) cands.push_back({sft, b->shape});
+	for (const SortShape &sh : tune_candidates(b->key_bits, b->P.cap))
+		cands.push_back({0, sh});
+	Cand best{b->P.bucket_on ? b->P.bshift : 0u, b->shape};
 	double best_t = -1.0;
-	for (const SortShape &sh : cands) {
+	for (const Cand &c : cands) {
 		if (active_count(b) == 0) break;
-		PP_TRY(pp_batch_set_shape(b, (uint32_t)sh.kpt, (uint32_t)sh.max_bits));
-		/* one untimed tick builds the graph and warms the caches */
+		PP_TRY(pp_batch_set_sort(b, c.shift));
+		PP_TRY(pp_batch_set_shape(b, (uint32_t)c.sh.kpt, (uint32_t)c.sh.max_bits));
+		if (c.shift && !b->P.bucket_on) continue;   /* layout too fine */
+		/* 8 untimed ticks re-sort if needed, build the graph, warm caches */
 		PP_TRY(enqueue_ticks(b, 8));
 		PP_TRY(collect(b));
 		double t = 0;
 		PP_TRY(time_ticks(b, trial_ticks, &t));
 		if (best_t < 0 || t < best_t) {
 			best_t = t;
-			best = sh;
+			best = c;
 		}
 	}
-	PP_TRY(pp_batch_set_shape(b, (uint32_t)best.kpt, (uint32_t)best.max_bits));
-	if (keys_per_thread) *keys_per_thread = (uint32_t)best.kpt;
-	if (max_digit_bits) *max_digit_bits = (uint32_t)best.max_bits;
+	PP_TRY(pp_batch_set_sort(b, best.shift));
+	PP_TRY(pp_batch_set_shape(b, (uint32_t)best.sh.kpt, (uint32_t)best.sh.max_bits));
+	if (bucket_shift) *bucket_shift = best.shift;
+	if (keys_per_thread) *keys_per_thread = (uint32_t)best.sh.kpt;
+	if (max_digit_bits) *max_digit_bits = (uint32_t)best.sh.max_bits;
 	if (sec_per_tick) *sec_per_tick = best_t < 0 ? 0.0 : best_t / trial_ticks;
 	return b->ctl_host->error;
 }

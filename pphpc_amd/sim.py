@@ -122,6 +122,7 @@ def _lib():
         "pp_sim_info": (i32, [vp, P(u32), P(u32), P(u32), P(u32)]),
         "pp_sim_destroy": (None, [vp]),
         "pp_sim_sync": (i32, [vp]),
+        "pp_sim_set_sort": (i32, [vp, u32]),
         "pp_batch_footprint": (u64, [u64, u64, u32, u32]),
         "pp_batch_create": (i32, [u64, u64, u32, u32, i32, P(vp)]),
         "pp_batch_add": (i32, [vp, P(Params), u64, u64, P(u32)]),
@@ -135,7 +136,9 @@ def _lib():
         "pp_batch_counters": (i32, [vp, P(u64)]),
         "pp_batch_info": (i32, [vp, P(u32), P(u32), P(u32), P(u32)]),
         "pp_batch_set_shape": (i32, [vp, u32, u32]),
-        "pp_batch_tune": (i32, [vp, u32, P(u32), P(u32), P(ctypes.c_double)]),
+        "pp_batch_tune": (i32, [vp, u32, P(u32), P(u32), P(u32),
+                                P(ctypes.c_double)]),
+        "pp_batch_set_sort": (i32, [vp, u32]),
         "pp_batch_set_profiling": (i32, [vp, i32]),
         "pp_batch_prof_get": (i32, [vp, i32, P(ctypes.c_char_p), P(u64),
                                     P(ctypes.c_double)]),
@@ -310,6 +313,11 @@ class Simulation:
     def sync(self):
         self._check(_lib().pp_sim_sync(self._h), "sync")
 
+    def set_sort(self, mode: int):
+        """0: radix sort every iteration; 8..11: row buckets of 2**mode
+        cells; 1: automatic (the default). Results do not depend on it."""
+        self._check(_lib().pp_sim_set_sort(self._h, mode), "set_sort")
+
     # -- profiling ---------------------------------------------------------
     def set_profiling(self, enable: bool):
         self._check(_lib().pp_sim_set_profiling(self._h, int(enable)),
@@ -472,13 +480,19 @@ class Batch:
         self._check(_lib().pp_batch_set_shape(self._h, keys_per_thread,
                                               max_digit_bits), "set_shape")
 
+    def set_sort(self, mode: int):
+        """0: radix sort every iteration; 8..11: row buckets of 2**mode
+        cells; 1: automatic (the default)."""
+        self._check(_lib().pp_batch_set_sort(self._h, mode), "set_sort")
+
     def tune(self, trial_ticks: int = 32) -> dict:
-        k, b, t = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
-        self._check(_lib().pp_batch_tune(self._h, trial_ticks, ctypes.byref(k),
-                                         ctypes.byref(b), ctypes.byref(t)),
-                    "tune")
-        return {"keys_per_thread": k.value, "max_digit_bits": b.value,
-                "sec_per_tick": t.value}
+        s, k, b, t = (ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(),
+                      ctypes.c_double())
+        self._check(_lib().pp_batch_tune(self._h, trial_ticks, ctypes.byref(s),
+                                         ctypes.byref(k), ctypes.byref(b),
+                                         ctypes.byref(t)), "tune")
+        return {"bucket_shift": s.value, "keys_per_thread": k.value,
+                "max_digit_bits": b.value, "sec_per_tick": t.value}
 
     def set_profiling(self, enable: bool):
         self._check(_lib().pp_batch_set_profiling(self._h, int(enable)),

@@ -8,7 +8,8 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import GOLDEN, ROOT, assert_state_equal, gpu_params, v1, v2s
+from helpers import (GOLDEN, ROOT, assert_state_equal, gpu_params, v1, v2ref,
+                     v2s)
 
 pytestmark = pytest.mark.gpu
 
@@ -148,9 +149,38 @@ def test_reinit_graph_and_profiling_modes_agree():
         np.array_equal(a, e)
     assert prof["act"][0] == 300 and prof["emit_move"][0] == 300
     assert prof["act_scan"][0] == 300 and prof["move0"][0] == 1
-    # 40000 cells: 16 bits = 2 passes of 8; only pass 1 needs an upsweep
-    assert prof["sort_down"][0] == 600 and prof["sort_scan"][0] == 600
-    assert prof["sort_up"][0] == 300
+    # iteration 1 sorts the freshly placed agents with the radix sort
+    # (40000 cells: 16 bits = 2 passes of 8), the 299 others by row buckets
+    assert prof["sort_down"][0] == 2 and prof["sort_scan"][0] == 2
+    assert prof["sort_up"][0] == 2
+    assert prof["bucket_sort"][0] == 299 and prof["bucket_scan"][0] == 300
+
+
+@pytest.mark.parametrize("shift", [0, 8, 9, 11])
+def test_sort_variants_agree(shift):
+    """The radix sort every iteration (0) and row buckets of 2^8..2^11 cells
+    give the oracle's rows and state, on grids narrower and wider than a
+    block, one and two rows high, and with a dense corner."""
+    from oracle.pp_oracle import Oracle
+    _case(f"sort variant {shift}")
+    cases = [(v2s(600), 1), (v2s(gx=3000, gy=20), 2), (v2s(gx=257, gy=1), 3),
+             (v2s(gx=700, gy=2), 4),
+             (dict(v2ref(60), grid_x=60, grid_y=60, init_sheep=6000,
+                   init_wolves=300), 5),
+             # rows of ~5000 agents: more than a bucket sorts in LDS at 2^11
+             (dict(v2ref(60), grid_x=2048, grid_y=4, grid_xy=8192,
+                   init_sheep=16000, init_wolves=4000), 6)]
+    for d, seed in cases:
+        sim = _sim(d, seed=seed)
+        sim.set_sort(shift)
+        rows = _rows(sim, 120)
+        o = Oracle(d, seed=seed, capacity=sim.capacity)
+        rc, orows = o.run(120)
+        assert rc == 0
+        assert np.array_equal(rows[1:], orows), (d["grid_x"], d["grid_y"])
+        assert_state_equal(sim.state(), o.state(), f"{shift} {d['grid_x']}")
+        o.close()
+        sim.close()
 
 
 def test_step_by_step_equals_run():
