@@ -1600,12 +1600,41 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 	RA ra{};
 	bool overflow = false;
 	/* Each group takes a contiguous run of tiles, so it crosses few replica
-	 * boundaries. */
+	 * boundaries. The next tile's pairs are loaded into registers before
+	 * this tile's stores are issued: the loads then do not wait behind
+	 * them. (The values of a segment running on from this tile into the
+	 * next are read stale there, but never used: that tile neither acts for
+	 * nor writes back the positions of an earlier tile's segment.) */
 	const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
-	const uint32_t t_end = min(ntiles, (blockIdx.x + 1) * per);
+	const uint32_t t_beg = blockIdx.x * per;
+	const uint32_t t_end = min(ntiles, t_beg + per);
+	uint32_t pk[A_IPT], pv[A_IPT], pprev = DEAD_KEY;
+	auto fetch = [&](uint32_t t) {
+		const uint32_t base = t * A_TILE;
+#pragma unroll
+		for (int j = 0; j < A_IPT; j++) {
+			const uint32_t i = base + tid + j * WG;
+			pk[j] = DEAD_KEY;
+			pv[j] = 0u;
+			if (i < L) {
+				pk[j] = keys[i];
+				pv[j] = vals[i];
+			}
+		}
+		if (tid == 0) pprev = base ? keys[base - 1] : DEAD_KEY;
+	};
+	if (t_beg < t_end) fetch(t_beg);
 
-	for (uint32_t tile = blockIdx.x * per; tile < t_end; tile++) {
-		const uint32_t cnt = load_tile(keys, vals, S.key, S.val, tile * A_TILE, L);
+	for (uint32_t tile = t_beg; tile < t_end; tile++) {
+		const uint32_t cnt = min((uint32_t)A_TILE, L - tile * A_TILE);
+#pragma unroll
+		for (int j = 0; j < A_IPT; j++) {
+			S.key[1 + tid + j * WG] = pk[j];
+			S.val[tid + j * WG] = pv[j];
+		}
+		if (tid == 0) S.key[0] = pprev;
+		__syncthreads();
+		if (tile + 1 < t_end) fetch(tile + 1);
 		/* Replicas of this tile: keys ascend, so they are the table entries
 		 * k_first..k_last. Every branch below is uniform over the group. */
 		const uint32_t k_first = rep_index(T, nrep,
@@ -2349,6 +2378,29 @@ static void drop_graph(pp_batch_t *b) {
 	b->graph_steps = 0;
 }
 
+/* Work-groups of kernel f (WG threads) that the GPU holds at once. */
+static uint32_t resident(const pp_batch_t *b, const void *f) {
+	int per_cu = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, WG, 0) !=
+			hipSuccess || per_cu <= 0)
+		per_cu = 2;
+	return (uint32_t)per_cu * (uint32_t)b->num_cu;
+}
+
+/* The action and emit kernels take four times the groups the GPU holds at
+ * once: groups that finish early are replaced, which evens out tiles of
+ * uneven work (measured at 12800 x 12800: 7% faster than one round). The
+ * bucket sort takes one round of resident groups. */
+constexpr uint32_t TILE_ROUNDS = 4;
+
+template <typename GT>
+static void set_grids_gt(pp_batch_t *b) {
+	const Params &P = b->P;
+	b->grid_act = min_u32(ceil_div(P.cap, A_TILE),
+		TILE_ROUNDS * resident(b, (const void *)k_act<GT>));
+	b->grid_bucket = min_u32(P.nb_cap, resident(b, (const void *)k_bucket_sort<GT>));
+}
+
 /* Sets the sort shape and the launch grids that depend on it. */
 static void apply_shape(pp_batch_t *b, const SortShape &sh) {
 	drop_graph(b);
@@ -2363,9 +2415,11 @@ static void apply_shape(pp_batch_t *b, const SortShape &sh) {
 	const uint32_t cu = (uint32_t)b->num_cu;
 	b->grid_up = min_u32(P.hist_cols, 8 * cu);
 	b->grid_down = min_u32(P.hist_cols, (b->rbits == 11 ? 1 : 2) * cu);
-	b->grid_act = min_u32(ceil_div(P.cap, A_TILE), 8 * cu);
-	b->grid_emit = min_u32(ceil_div(P.cap, A_TILE), 8 * cu);
-	b->grid_bucket = min_u32(P.nb_cap, 8 * cu);
+	b->grid_emit = min_u32(ceil_div(P.cap, A_TILE),
+		TILE_ROUNDS * resident(b, (const void *)k_emit_move));
+	if (b->grass_bytes == 1) set_grids_gt<uint8_t>(b);
+	else if (b->grass_bytes == 2) set_grids_gt<uint16_t>(b);
+	else set_grids_gt<uint32_t>(b);
 }
 
 /* Capture `steps` ticks into one executable graph (kernels only; no memset

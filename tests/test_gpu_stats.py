@@ -30,7 +30,7 @@ def _case(msg):
 
 
 @pytest.mark.skipif(not os.path.exists(FIXTURE), reason="fixture not written")
-def test_v2s_1600_replicas_equivalent_to_netlogo_rules():
+def test_v2s_1600_replicas_equivalent_to_reference_rules():
     from pphpc_amd import Batch, Params
     from pphpc_amd.replicas import compare, focal_measures
     from pphpc_amd.workloads import reserve_for
