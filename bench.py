@@ -17,8 +17,10 @@ Also reported (one JSON line on stdout, progress on stderr):
                 counters) / average launch duration (HIP events on the
                 simulation's stream), against 8 TB/s
   roofline_12800  the same for the 12800x12800 run (HBM-resident working set)
-  cpu_baseline  the single-threaded C oracle on a bounded sample of the same
-                workload, on this host's cores
+  cpu_baseline  the OpenMP restatement of the step (row bands, as the
+                reference's CPU simulator) on this host's cores, and the
+                single-threaded C oracle, each on the first iterations of the
+                same workload
   batched       64 replicas of grid sides drawn from {100 .. 3200}, v1 or v2s
                 parameters 50/50, seeds 1..64, 4000 iterations each: one
                 batch (one set of launches per iteration, sort shape from the
@@ -80,25 +82,34 @@ WORKLOADS = {
 }
 
 
-def kernel_bytes(name, c, passes):
+def kernel_bytes(name, c, passes, radix_all=False):
     """Algorithmic (compulsory) HBM bytes of all launches of one kernel over
-    iterations that all sort by row buckets, from the exact event sums c of
-    those iterations (agent-steps N, live after move L, agents after actions
-    N', occupied cells O, grass eaten E):
+    a run of iterations, from the exact event sums c of those iterations
+    (agent-steps N, live after move L, agents after actions N', occupied
+    cells O, grass eaten E):
       bucket_sort  read the N moved pairs (8 B), write the L survivors in
                    cell order: 8N + 8L
+      sort_up      radix pass 0 reads the N moved keys, later passes the L
+                   survivors' keys: 4N + 4L(P-1)
+      sort_down    pass 0 reads the N moved pairs and writes the L survivors,
+                   later passes read and write L pairs: 8N + 8L + 16L(P-1)
       act          read L sorted pairs, write the L updated values back, read
                    the grass of each occupied cell, write each eaten one:
                    12L + 4O + 4E
       emit_move    read L sorted pairs, write the new state (9 B) and its
                    moved pair (8 B) per agent: 8L + 17N'
-    The radix-sort kernels run only in the first iteration after a change of
-    the replicas and get no byte count here.
+    The radix kernels get a count only when every iteration of the run
+    sorted by radix (radix_all); otherwise they ran only in the first
+    iteration after a change of the replicas.
     """
     N, L, Np, O, E = (c["agent_steps"], c["live"], c["out"], c["occupied"],
                       c["eaten"])
     if name == "bucket_sort":
         return 8 * N + 8 * L
+    if name == "sort_up" and radix_all:
+        return 4 * N + 4 * L * (passes - 1)
+    if name == "sort_down" and radix_all:
+        return 8 * N + 8 * L + 16 * L * (passes - 1)
     if name == "act":
         return 12 * L + 4 * O + 4 * E
     if name == "emit_move":
@@ -126,13 +137,15 @@ def measure_roofline(params, capacity, steps, seed=0):
     c = {k: c1[k] - c0[k] for k in c1}
     step_kernels = {k: v for k, v in prof.items()
                     if k in STEP_KERNELS and v[0] > 0}
-    dom = max((k for k in step_kernels if kernel_bytes(k, c, passes)),
+    radix_all = step_kernels.get("sort_down", (0, 0))[0] == passes * steps
+    dom = max((k for k in step_kernels
+               if kernel_bytes(k, c, passes, radix_all)),
               key=lambda k: step_kernels[k][1])
     launches, ns = step_kernels[dom]
-    total_bytes = kernel_bytes(dom, c, passes)
+    total_bytes = kernel_bytes(dom, c, passes, radix_all)
     per_kernel = {}
     for k, (nl, t) in step_kernels.items():
-        b = kernel_bytes(k, c, passes)
+        b = kernel_bytes(k, c, passes, radix_all)
         per_kernel[k] = {
             "launches": nl, "avg_us": round(t / max(nl, 1) / 1e3, 3),
             "GBps": round(b / t, 1) if b and t else None,
@@ -153,12 +166,17 @@ def measure_roofline(params, capacity, steps, seed=0):
     }
 
 
-def cpu_baseline(params, capacity, sample_steps):
-    """The single-threaded C oracle (a port of the reference step semantics)
-    on the first `sample_steps` iterations of the same workload."""
-    exe = os.path.join(ROOT, "oracle", "build", "pp_oracle")
-    if not os.path.exists(exe):
-        return None
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _run_cpu(exe, params, capacity, sample_steps, extra=()):
     keys = ("INIT_SHEEP", "SHEEP_GAIN_FROM_FOOD", "SHEEP_REPRODUCE_THRESHOLD",
             "SHEEP_REPRODUCE_PROB", "INIT_WOLVES", "WOLVES_GAIN_FROM_FOOD",
             "WOLVES_REPRODUCE_THRESHOLD", "WOLVES_REPRODUCE_PROB",
@@ -174,16 +192,38 @@ def cpu_baseline(params, capacity, sample_steps):
         path = f.name
     try:
         out = subprocess.run([exe, "-p", path, "-n", str(sample_steps), "-q",
-                              "-m", str(capacity)], capture_output=True,
-                             text=True, timeout=600, check=True).stdout
+                              "-m", str(capacity), *extra],
+                             capture_output=True, text=True, timeout=600,
+                             check=True).stdout
     finally:
         os.unlink(path)
-    kv = dict(tok.split("=") for tok in out.split())
+    return dict(tok.split("=") for tok in out.split())
+
+
+def cpu_baseline(params, capacity, sample_steps, single_steps, threads):
+    """CPU restatements of the step on this host, on the first iterations of
+    the same workload: the OpenMP row-band restatement (oracle/pp_omp.c) on
+    `threads` threads, and the single-threaded C oracle. Both compute the
+    same results as the GPU (same semantics and random streams)."""
+    omp = os.path.join(ROOT, "oracle", "build", "pp_omp")
+    one = os.path.join(ROOT, "oracle", "build", "pp_oracle")
+    if not (os.path.exists(omp) and os.path.exists(one)):
+        return None
+    kv = _run_cpu(omp, params, capacity, sample_steps, ("-t", str(threads)))
+    kv1 = _run_cpu(one, params, capacity, single_steps)
     return {"value": float(kv["agent_steps_per_s"]), "unit": "agent-steps/s",
-            "cores": 1, "kind": "port",
+            "cores": int(kv["threads"]), "kind": "port",
             "sample": f"first {sample_steps} iterations of the same workload "
-                      f"(seed 0), single-threaded C oracle; "
-                      f"{float(kv['seconds']):.1f} s"}
+                      f"(seed 0), OpenMP row-band restatement of the step "
+                      f"(oracle/pp_omp.c) on {kv['threads']} threads: "
+                      f"{float(kv['seconds']):.1f} s",
+            "cpu_model": _cpu_model(),
+            "host_cpus": os.cpu_count(),
+            "single_thread": {
+                "value": float(kv1["agent_steps_per_s"]),
+                "unit": "agent-steps/s", "cores": 1, "kind": "port",
+                "sample": f"first {single_steps} iterations, single-threaded "
+                          f"C oracle: {float(kv1['seconds']):.1f} s"}}
 
 
 def batched_leg(n, iters, draw_key=4, tune_ticks=16):
@@ -270,7 +310,11 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--roofline-steps", type=int, default=400)
     ap.add_argument("--big-roofline-steps", type=int, default=6)
-    ap.add_argument("--cpu-steps", type=int, default=150)
+    ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--cpu-single-steps", type=int, default=150)
+    ap.add_argument("--cpu-threads", type=int,
+                    default=min(16, int(os.environ.get("OMP_NUM_THREADS", 0))
+                                or os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-big", action="store_true")
     ap.add_argument("--batch-replicas", type=int, default=64)
@@ -343,8 +387,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log(f"cpu baseline: oracle, {args.cpu_steps} iterations")
-        cpu = cpu_baseline(params, capacity, args.cpu_steps)
+        log(f"cpu baseline: {args.cpu_threads} threads, {args.cpu_steps} "
+            f"iterations; single thread, {args.cpu_single_steps}")
+        cpu = cpu_baseline(params, capacity, args.cpu_steps,
+                           args.cpu_single_steps, args.cpu_threads)
 
     out = {
         "metric": METRIC,

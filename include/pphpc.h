@@ -140,6 +140,31 @@ int pp_sim_counters(const pp_sim_t *s, uint64_t out[5]);
 int pp_sim_get_state(pp_sim_t *s, uint32_t *cell, int32_t *energy,
 	uint8_t *type, uint32_t *grass);
 
+/* Debug dump of the current state, the counterpart of the reference's
+ * PPG_DUMP build option (pp_gpu.c:36-46; ppg_dump at :438-555). Appends one
+ * block per call to agents_path:
+ *     "\nIter %u, agents=%llu\n", then per agent, in state order,
+ *     "[%4llu] (%4u, %4u) type=%u energy=%d\n"      (index, x, y, type, energy)
+ * and to cells_path:
+ *     "\nIteration %u\n", then per cell, row-major,
+ *     "(%u, %u) -> (%llu, %llu) %s [Grass: %u]\n"
+ * with the first and last index of the cell's agents ("(-, -)" for a cell
+ * without agents), "More than 1 agent present" when they differ, and the
+ * grass countdown (0 = alive). The iteration is the number of iterations done
+ * (0 = the initial state). dump_type bit 0x10 keeps only the cells that hold
+ * agents, as pp_gpu.c:521-523 does, with a blank line where cells were left
+ * out; the state holds no dead agents, so bit 0x01 changes nothing. Either
+ * path may be NULL; append = 0 truncates the files first. */
+int pp_sim_dump(pp_sim_t *s, const char *agents_path, const char *cells_path,
+	uint32_t dump_type, int append);
+
+/* The formatter of pp_sim_dump, for a state held in host memory (n agents,
+ * p->grid_xy cells). */
+int pp_dump_state(const char *agents_path, const char *cells_path, int append,
+	uint32_t iter, uint32_t dump_type, const pp_params_t *p, uint64_t n,
+	const uint32_t *cell, const int32_t *energy, const uint8_t *type,
+	const uint32_t *grass);
+
 /* Per-kernel timing, the counterpart of the reference's profiler summary and
  * --prof-agg export (pp_gpu.c:318-321, :1868-1881; pp_common.c:248-260).
  * When enabled, each kernel launch of init/step/run is bracketed by HIP events

@@ -467,6 +467,65 @@ void ppo_state_get(const ppo_sim *s, uint32_t *cell, int32_t *energy,
 	if (grass) memcpy(grass, s->grass, (size_t)s->p.grid_xy * sizeof(uint32_t));
 }
 
+/* Text dump of the state (the format of pp_sim_dump, after the reference's
+ * PPG_DUMP output, opencl/pp/pp_gpu.c:438-555): one block per call appended
+ * to each file; bit 0x10 of `type` lists only the cells holding agents. */
+int ppo_dump(const ppo_sim *s, const char *agents_path, const char *cells_path,
+	uint32_t type, int append) {
+	const char *mode = append ? "a" : "w";
+	const uint32_t gx = s->p.grid_x, C = s->p.grid_xy;
+	if (agents_path) {
+		FILE *f = fopen(agents_path, mode);
+		if (!f) return -6;
+		fprintf(f, "\nIter %u, agents=%llu\n", s->step,
+			(unsigned long long)s->n);
+		for (uint64_t i = 0; i < s->n; i++)
+			fprintf(f, "[%4llu] (%4u, %4u) type=%u energy=%d\n",
+				(unsigned long long)i, s->cell[i] % gx, s->cell[i] / gx,
+				(unsigned)s->type[i], s->energy[i]);
+		fclose(f);
+	}
+	if (cells_path) {
+		/* first and last slot of each cell (after init the state is in
+		 * creation order, after a step in cell order) */
+		uint64_t *lo = xcalloc(C, sizeof(uint64_t)), *hi = xcalloc(C, sizeof(uint64_t));
+		for (uint32_t c = 0; c < C; c++) lo[c] = UINT64_MAX;
+		for (uint64_t i = 0; i < s->n; i++) {
+			const uint32_t c = s->cell[i];
+			if (lo[c] == UINT64_MAX) lo[c] = i;
+			hi[c] = i;
+		}
+		FILE *f = fopen(cells_path, mode);
+		if (!f) {
+			free(lo);
+			free(hi);
+			return -6;
+		}
+		fprintf(f, "\nIteration %u\n", s->step);
+		int gap = 0;
+		for (uint32_t c = 0; c < C; c++) {
+			const int empty = lo[c] == UINT64_MAX;
+			if (empty && (type & 0x10)) {
+				gap = 1;
+				continue;
+			}
+			if (gap) fputc('\n', f);
+			gap = 0;
+			if (empty)
+				fprintf(f, "(%u, %u) -> (-, -)  [Grass: %u]\n", c % gx, c / gx,
+					s->grass[c]);
+			else
+				fprintf(f, "(%u, %u) -> (%llu, %llu) %s [Grass: %u]\n", c % gx,
+					c / gx, (unsigned long long)lo[c], (unsigned long long)hi[c],
+					hi[c] > lo[c] ? "More than 1 agent present" : "", s->grass[c]);
+		}
+		fclose(f);
+		free(lo);
+		free(hi);
+	}
+	return 0;
+}
+
 /* ------------------------------------------------------------------- CLI */
 #ifdef PPO_MAIN
 #include <getopt.h>
@@ -480,12 +539,16 @@ static double now_s(void) {
 int main(int argc, char **argv) {
 	/* Same flags as the reference executable (pp_gpu.c:310-345): -p params
 	 * (default config.txt), -s stats (default stats.txt), -r seed (default
-	 * 0), -m capacity; plus -n to override ITERS and -q to skip the file. */
+	 * 0), -m capacity; plus -n to override ITERS, -q to skip the file, and
+	 * -D PREFIX [-T TYPE] to dump every state to PREFIX_agents.txt and
+	 * PREFIX_cells.txt. */
 	const char *params = "config.txt", *stats = "stats.txt";
 	uint64_t seed = 0, cap = 0;
 	long iters_override = -1;
+	const char *dump = NULL;
+	uint32_t dump_type = 0x01;
 	int quiet = 0, opt;
-	while ((opt = getopt(argc, argv, "p:s:r:m:n:q")) != -1) {
+	while ((opt = getopt(argc, argv, "p:s:r:m:n:qD:T:")) != -1) {
 		switch (opt) {
 		case 'p': params = optarg; break;
 		case 's': stats = optarg; break;
@@ -493,6 +556,8 @@ int main(int argc, char **argv) {
 		case 'm': cap = strtoull(optarg, NULL, 0); break;
 		case 'n': iters_override = strtol(optarg, NULL, 0); break;
 		case 'q': quiet = 1; break;
+		case 'D': dump = optarg; break;
+		case 'T': dump_type = (uint32_t)strtoul(optarg, NULL, 0); break;
 		default: return -1;
 		}
 	}
@@ -507,7 +572,20 @@ int main(int argc, char **argv) {
 	ppo_stats *rows = xcalloc((size_t)p.iters + 1, sizeof(ppo_stats));
 	ppo_stats_get(s, &rows[0]);
 	double t0 = now_s();
-	rc = ppo_run(s, p.iters, rows + 1);
+	if (dump) {
+		char *fa = xcalloc(strlen(dump) + 16, 1), *fc = xcalloc(strlen(dump) + 16, 1);
+		sprintf(fa, "%s_agents.txt", dump);
+		sprintf(fc, "%s_cells.txt", dump);
+		rc = ppo_dump(s, fa, fc, dump_type, 0);
+		for (uint32_t t = 0; !rc && t < p.iters; t++) {
+			rc = ppo_run(s, 1, rows + 1 + t);
+			if (!rc) rc = ppo_dump(s, fa, fc, dump_type, 1);
+		}
+		free(fa);
+		free(fc);
+	} else {
+		rc = ppo_run(s, p.iters, rows + 1);
+	}
 	double t1 = now_s();
 	if (rc) fprintf(stderr, "Error: out of resources at step %u\n",
 		ppo_step_count(s));

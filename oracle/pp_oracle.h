@@ -97,6 +97,11 @@ int ppo_state_set(ppo_sim *s, uint64_t n, const uint32_t *cell,
  * [1] grass eaten, [2] sheep eaten, [3] sheep births, [4] wolf births. */
 void ppo_last_events(const ppo_sim *s, uint64_t out[5]);
 
+/* Appends the text dump of the state (format of pp_sim_dump) to the files;
+ * either may be NULL; append = 0 truncates them. 0 or a negative code. */
+int ppo_dump(const ppo_sim *s, const char *agents_path, const char *cells_path,
+	uint32_t type, int append);
+
 /* Sum over all steps of the live agents at the start of each step. */
 uint64_t ppo_agent_steps(const ppo_sim *s);
 

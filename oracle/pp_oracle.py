@@ -68,6 +68,8 @@ def lib():
             "ppo_last_events": (None, [vp, P(u64)]),
             "ppo_agent_steps": (u64, [vp]),
             "ppo_set_fault": (None, [vp, i32]),
+            "ppo_dump": (i32, [vp, ctypes.c_char_p, ctypes.c_char_p, u32,
+                               i32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -186,6 +188,11 @@ class Oracle:
 
     def set_fault(self, mode: int):
         lib().ppo_set_fault(self._h, mode)
+
+    def dump(self, agents_path, cells_path, dump_type=0x01, append=False):
+        enc = (lambda x: x.encode() if x is not None else None)
+        return lib().ppo_dump(self._h, enc(agents_path), enc(cells_path),
+                              dump_type, int(append))
 
 
 def simulate(params, seed=0, capacity=0, iters=None):

@@ -3,11 +3,15 @@
  *
  * Keeps the reference's file formats: the 12-key "KEY = value" parameter file
  * read by pp_load_params (opencl/pp/pp_common.c:39-195) and the tab-separated
- * per-iteration statistics written by pp_stats_save (pp_common.c:205-240).
+ * per-iteration statistics written by pp_stats_save (pp_common.c:205-240);
+ * and the text state dump of pp_dump_state, after the reference's PPG_DUMP
+ * output (pp_gpu.c:438-555).
  */
 #include "pphpc.h"
 
+#include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 /* Keys in the order of the PPParameters fields (pp_common.in.h:129-157),
@@ -107,4 +111,74 @@ const char *pp_strerror(int code) {
 	case PP_OUT_OF_RESOURCES: return "agents above capacity (out of resources)";
 	default: return "unknown error code";
 	}
+}
+
+int pp_dump_state(const char *agents_path, const char *cells_path, int append,
+	uint32_t iter, uint32_t dump_type, const pp_params_t *p, uint64_t n,
+	const uint32_t *cell, const int32_t *energy, const uint8_t *type,
+	const uint32_t *grass) {
+	if (!p || !p->grid_x || (n && (!cell || !energy || !type)) ||
+		(cells_path && !grass))
+		return PP_INVALID_ARGS;
+	const uint64_t cells = (uint64_t)p->grid_x * p->grid_y;
+	const char *mode = append ? "a" : "w";
+	if (agents_path) {
+		FILE *fp = fopen(agents_path, mode);
+		if (!fp) return PP_UNABLE_SAVE_STATS;
+		fprintf(fp, "\nIter %u, agents=%llu\n", iter, (unsigned long long)n);
+		for (uint64_t k = 0; k < n; k++)
+			fprintf(fp, "[%4llu] (%4u, %4u) type=%u energy=%d\n",
+				(unsigned long long)k, cell[k] % p->grid_x, cell[k] / p->grid_x,
+				(unsigned)type[k], energy[k]);
+		if (fclose(fp)) return PP_UNABLE_SAVE_STATS;
+	}
+	if (cells_path) {
+		/* first and last index of each cell's agents */
+		uint64_t *first = malloc(cells * sizeof(uint64_t));
+		uint64_t *last = malloc(cells * sizeof(uint64_t));
+		if (!first || !last) {
+			free(first);
+			free(last);
+			return PP_OUT_OF_RESOURCES;
+		}
+		for (uint64_t c = 0; c < cells; c++) first[c] = UINT64_MAX;
+		for (uint64_t k = 0; k < n; k++) {
+			if (cell[k] >= cells) {
+				free(first);
+				free(last);
+				return PP_INVALID_ARGS;
+			}
+			if (first[cell[k]] == UINT64_MAX) first[cell[k]] = k;
+			last[cell[k]] = k;
+		}
+		FILE *fp = fopen(cells_path, mode);
+		if (!fp) {
+			free(first);
+			free(last);
+			return PP_UNABLE_SAVE_STATS;
+		}
+		fprintf(fp, "\nIteration %u\n", iter);
+		int blank_line = 0;
+		for (uint64_t c = 0; c < cells; c++) {
+			const unsigned x = (unsigned)(c % p->grid_x);
+			const unsigned y = (unsigned)(c / p->grid_x);
+			if ((dump_type & 0x10) && first[c] == UINT64_MAX) {
+				blank_line = 1;
+				continue;
+			}
+			if (blank_line) fprintf(fp, "\n");
+			blank_line = 0;
+			if (first[c] == UINT64_MAX)
+				fprintf(fp, "(%u, %u) -> (-, -)  [Grass: %u]\n", x, y, grass[c]);
+			else
+				fprintf(fp, "(%u, %u) -> (%llu, %llu) %s [Grass: %u]\n", x, y,
+					(unsigned long long)first[c], (unsigned long long)last[c],
+					first[c] != last[c] ? "More than 1 agent present" : "",
+					grass[c]);
+		}
+		free(first);
+		free(last);
+		if (fclose(fp)) return PP_UNABLE_SAVE_STATS;
+	}
+	return PP_SUCCESS;
 }

@@ -3,6 +3,7 @@
  * command line (opencl/pp/pp_gpu.c:310-345, main at :1700-1948).
  *
  *   pp [-p FILE] [-s FILE] [-r SEED] [-m MAX_AGENTS] [-d INDEX] [-i FILE]
+ *      [-D PREFIX [-T TYPE]]
  *
  * -p/--params     parameters file (default config.txt)
  * -s/--stats      statistics output file (default stats.txt)
@@ -11,6 +12,13 @@
  * -d/--device     HIP device index (default: current device)
  * -i/--prof-agg   export per-kernel times as "name"\tns\trelative
  *                 (pp_common.c:248-260)
+ * -D/--dump       write PREFIX_agents.txt and PREFIX_cells.txt: the state
+ *                 after init and after every iteration, as the reference's
+ *                 PPG_DUMP build writes dump_agents.txt / dump_cells.txt
+ *                 (pp_gpu.c:36-46, :644-657, :924-931); runs the iterations
+ *                 one at a time
+ * -T/--dump-type  PPG_DUMP's type bits (default 0x01; 0x10: only the cells
+ *                 holding agents)
  *
  * The number of iterations is ITERS of the parameters file. The exit status
  * is the PP_* code of the first error (pp_common.in.h:87-104), 0 on success.
@@ -19,6 +27,7 @@
 #include <getopt.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include <time.h>
 
 #include "pphpc.h"
@@ -32,13 +41,38 @@ static double now_s(void) {
 static void usage(FILE *fp) {
 	fprintf(fp,
 		"Usage: pp [-p FILE] [-s FILE] [-r SEED] [-m MAX_AGENTS] [-d INDEX] "
-		"[-i FILE]\n"
+		"[-i FILE] [-D PREFIX [-T TYPE]]\n"
 		"  -p, --params FILE      parameters file (default config.txt)\n"
 		"  -s, --stats FILE       statistics output (default stats.txt)\n"
 		"  -r, --rng-seed SEED    random seed (default 0)\n"
 		"  -m, --max-agents N     agent capacity\n"
 		"  -d, --device INDEX     HIP device index\n"
-		"  -i, --prof-agg FILE    export aggregate per-kernel times\n");
+		"  -i, --prof-agg FILE    export aggregate per-kernel times\n"
+		"  -D, --dump PREFIX      dump the state of every iteration to\n"
+		"                         PREFIX_agents.txt and PREFIX_cells.txt\n"
+		"  -T, --dump-type TYPE   dump type bits (0x10: cells with agents)\n");
+}
+
+/* pp_sim_simulate one iteration at a time, dumping every state. */
+static int simulate_dumping(pp_sim_t *sim, const pp_params_t *p,
+	pp_stats64_t *rows, const char *prefix, uint32_t dump_type) {
+	char *agents = malloc(strlen(prefix) + 16), *cells = malloc(strlen(prefix) + 16);
+	if (!agents || !cells) {
+		free(agents);
+		free(cells);
+		return PP_OUT_OF_RESOURCES;
+	}
+	sprintf(agents, "%s_agents.txt", prefix);
+	sprintf(cells, "%s_cells.txt", prefix);
+	int rc = pp_sim_init(sim, &rows[0]);
+	if (!rc) rc = pp_sim_dump(sim, agents, cells, dump_type, 0);
+	for (uint32_t i = 1; !rc && i <= p->iters; i++) {
+		rc = pp_sim_step(sim, &rows[i]);
+		if (!rc) rc = pp_sim_dump(sim, agents, cells, dump_type, 1);
+	}
+	free(agents);
+	free(cells);
+	return rc;
 }
 
 int main(int argc, char **argv) {
@@ -49,14 +83,18 @@ int main(int argc, char **argv) {
 		{"max-agents", required_argument, NULL, 'm'},
 		{"device", required_argument, NULL, 'd'},
 		{"prof-agg", required_argument, NULL, 'i'},
+		{"dump", required_argument, NULL, 'D'},
+		{"dump-type", required_argument, NULL, 'T'},
 		{"help", no_argument, NULL, 'h'},
 		{NULL, 0, NULL, 0}};
 	const char *params_file = NULL, *stats_file = NULL, *prof_file = NULL;
+	const char *dump_prefix = NULL;
+	unsigned long dump_type = 0x01;
 	unsigned long long seed = 0, cap = 0;
 	int device = -1, opt;
 	char *endp;
 
-	while ((opt = getopt_long(argc, argv, "p:s:r:m:d:i:h", opts, NULL)) != -1) {
+	while ((opt = getopt_long(argc, argv, "p:s:r:m:d:i:D:T:h", opts, NULL)) != -1) {
 		switch (opt) {
 		case 'p': params_file = optarg; break;
 		case 's': stats_file = optarg; break;
@@ -73,6 +111,11 @@ int main(int argc, char **argv) {
 			if (*endp) { fprintf(stderr, "Error: invalid -d\n"); return PP_INVALID_ARGS; }
 			break;
 		case 'i': prof_file = optarg; break;
+		case 'D': dump_prefix = optarg; break;
+		case 'T':
+			dump_type = strtoul(optarg, &endp, 0);
+			if (*endp) { fprintf(stderr, "Error: invalid -T\n"); return PP_INVALID_ARGS; }
+			break;
 		case 'h': usage(stdout); return PP_SUCCESS;
 		default: usage(stderr); return PP_UNKNOWN_ARGS;
 		}
@@ -112,7 +155,10 @@ int main(int argc, char **argv) {
 		(unsigned long long)pp_sim_capacity(sim), p.iters, passes, tile);
 
 	double t0 = now_s();
-	rc = pp_sim_simulate(sim, rows);
+	if (dump_prefix)
+		rc = simulate_dumping(sim, &p, rows, dump_prefix, (uint32_t)dump_type);
+	else
+		rc = pp_sim_simulate(sim, rows);
 	double t1 = now_s();
 	if (rc) {
 		if (rc == PP_OUT_OF_RESOURCES)

@@ -3225,6 +3225,25 @@ int pp_sim_get_state(pp_sim_t *s, uint32_t *cell, int32_t *energy,
 	return state_of(s->b, *r, cell, energy, type, grass);
 }
 
+int pp_sim_dump(pp_sim_t *s, const char *agents_path, const char *cells_path,
+	uint32_t dump_type, int append) {
+	Replica *r = sim_rep(s);
+	if (!r) return PP_INVALID_ARGS;
+	HIP_OK(hipSetDevice(s->b->device));
+	PP_TRY(read_ctl(s->b));
+	if (r->slot < 0) return PP_INVALID_ARGS;
+	const uint64_t n = s->b->rctl_host[r->slot].n;
+	const uint64_t cells = (uint64_t)s->prm.grid_x * s->prm.grid_y;
+	std::vector<uint32_t> cell(n), grass(cells);
+	std::vector<int32_t> energy(n);
+	std::vector<uint8_t> type(n);
+	PP_TRY(state_of(s->b, *r, cell.data(), energy.data(), type.data(),
+		grass.data()));
+	return pp_dump_state(agents_path, cells_path, append, r->rows_have - 1,
+		dump_type, &s->prm, n, cell.data(), energy.data(), type.data(),
+		grass.data());
+}
+
 int pp_sim_set_profiling(pp_sim_t *s, int enable) {
 	if (!s) return PP_INVALID_ARGS;
 	s->b->profiling = enable ? 1 : 0;

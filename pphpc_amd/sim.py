@@ -123,6 +123,10 @@ def _lib():
         "pp_sim_destroy": (None, [vp]),
         "pp_sim_sync": (i32, [vp]),
         "pp_sim_set_sort": (i32, [vp, u32]),
+        "pp_sim_dump": (i32, [vp, ctypes.c_char_p, ctypes.c_char_p, u32,
+                              i32]),
+        "pp_dump_state": (i32, [ctypes.c_char_p, ctypes.c_char_p, i32, u32,
+                                u32, P(Params), u64, vp, vp, vp, vp]),
         "pp_batch_footprint": (u64, [u64, u64, u32, u32]),
         "pp_batch_create": (i32, [u64, u64, u32, u32, i32, P(vp)]),
         "pp_batch_add": (i32, [vp, P(Params), u64, u64, P(u32)]),
@@ -342,6 +346,14 @@ class Simulation:
     def profile_export(self, path: str):
         self._check(_lib().pp_sim_prof_export(self._h, path.encode()),
                     "prof_export")
+
+    def dump(self, agents_path=None, cells_path=None, dump_type=0x01,
+             append=False):
+        """Appends the text dump of the current state (pp_sim_dump)."""
+        enc = (lambda x: x.encode() if x is not None else None)
+        self._check(_lib().pp_sim_dump(self._h, enc(agents_path),
+                                       enc(cells_path), dump_type,
+                                       int(append)), "dump")
 
 
 # Replica states (PP_REPLICA_* of pphpc.h).

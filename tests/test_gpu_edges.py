@@ -65,6 +65,37 @@ def test_cli_writes_the_oracle_stats_file(tmp_path):
     assert abs(sum(float(ln.split("\t")[2]) for ln in lines) - 1.0) < 1e-5
 
 
+@pytest.mark.parametrize("dump_type", [0x01, 0x10])
+def test_cli_dump_equals_the_oracle_dump(tmp_path, dump_type):
+    """`pp -D PREFIX` dumps every state (after init and each iteration) with
+    the same bytes as the oracle CLI's dump."""
+    _case(f"cli dump type {dump_type:#x}")
+    d = v2s(24, iters=15)
+    path = tmp_path / "p.txt"
+    keys = ("INIT_SHEEP", "SHEEP_GAIN_FROM_FOOD", "SHEEP_REPRODUCE_THRESHOLD",
+            "SHEEP_REPRODUCE_PROB", "INIT_WOLVES", "WOLVES_GAIN_FROM_FOOD",
+            "WOLVES_REPRODUCE_THRESHOLD", "WOLVES_REPRODUCE_PROB",
+            "GRASS_RESTART", "GRID_X", "GRID_Y", "ITERS")
+    path.write_text("".join(f"{k} = {d[k.lower()]}\n" for k in keys))
+    ours, ref = str(tmp_path / "gpu"), str(tmp_path / "oracle")
+    r = subprocess.run([os.path.join(ROOT, "pphpc_amd", "bin", "pp"), "-p",
+                        str(path), "-s", str(tmp_path / "s1.txt"), "-r", "7",
+                        "-d", "0", "-m", "4096", "-D", ours, "-T",
+                        str(dump_type)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    subprocess.run([os.path.join(ROOT, "oracle", "build", "pp_oracle"), "-p",
+                    str(path), "-s", str(tmp_path / "s2.txt"), "-r", "7",
+                    "-m", "4096", "-D", ref, "-T", str(dump_type)],
+                   check=True, capture_output=True, timeout=120)
+    for part in ("_agents.txt", "_cells.txt"):
+        a, b = open(ours + part).read(), open(ref + part).read()
+        assert a.count("\nIter") == 16
+        assert a == b, part
+    assert (open(tmp_path / "s1.txt", "rb").read()
+            == open(tmp_path / "s2.txt", "rb").read())
+
+
 def test_extinction_and_empty_population():
     """Wolves without sheep starve: population reaches zero and stays; an
     empty initial population is valid."""
