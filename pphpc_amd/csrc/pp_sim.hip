@@ -176,7 +176,7 @@ struct Params {
 	uint32_t bshift;     /* bucket blocks are 2^bshift cells of a grid row */
 	uint32_t bucket_on;  /* sort by row buckets; the emit counts for them */
 	uint32_t nb_cap;     /* bucket capacity of the count tables */
-	uint32_t bscan_one;  /* the buckets fit one scan chunk */
+	uint32_t cnt_top;    /* highest level of the bucket count tables in use */
 };
 
 /* Replica tables in device memory: constants and control words by slot, and
@@ -366,13 +366,46 @@ __device__ __forceinline__ uint32_t move_agent(uint32_t id, uint32_t x,
 	return m.cb + ny * m.gx + nx;
 }
 
+/* Bucket count tables: four levels of 64-ary sums, so that any prefix sum
+ * is at most 4 x 63 reads that one wave does at once. Level k holds, for
+ * every q, the count of buckets [64^k q, 64^k (q + 1)); it has
+ * (nb_cap >> 6k) + 1 entries and starts at cnt_off(nb_cap, k). Layouts of
+ * up to 2^24 buckets fit. Only levels 0 .. cnt_top are kept, cnt_top the
+ * highest with an entry above 0 for the layout (NB > 64^cnt_top). */
+constexpr int CNT_LEVELS = 4;
+constexpr uint32_t CNT_MAX_BUCKETS = 1u << 24;
+
+__host__ __device__ __forceinline__ uint32_t cnt_off(uint32_t nbc, int lvl) {
+	uint32_t o = 0;
+	for (int k = 0; k < lvl; k++) o += (nbc >> (6 * k)) + 1;
+	return o;
+}
+
+/* Sum of the counts of buckets [0, q) (one wave; every lane gets it), from
+ * levels 0 .. top (q < 64^(top + 1)). */
+__device__ __forceinline__ uint32_t cnt_prefix(const uint32_t *cnt,
+	uint32_t nbc, uint32_t top, uint32_t q) {
+	const uint32_t lane = threadIdx.x & (WAVE - 1);
+	uint32_t v = 0;
+#pragma unroll
+	for (int k = 0; k < CNT_LEVELS; k++) {
+		if ((uint32_t)k > top) break;
+		const uint32_t sh = 6 * k;
+		const uint32_t base = (q >> (sh + 6)) << 6, lim = (q >> sh) & 63u;
+		if (lane < lim) v += cnt[cnt_off(nbc, k) + base + lane];
+	}
+#pragma unroll
+	for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+	return v;
+}
+
 /* Adds 1 to the count of bucket `bkt` for every lane of the wave (NO_DIGIT
  * lanes count nothing), merging runs of equal buckets in consecutive lanes
  * first: into the LDS window s_win of WIN buckets from w0 when it falls
- * there, else straight into the global table g_cnt. */
+ * there, else straight into every level of the global table g_cnt. */
 template <uint32_t WIN>
 __device__ __forceinline__ void wave_bucket_add(uint32_t *s_win,
-	uint32_t *g_cnt, uint32_t w0, uint32_t bkt) {
+	uint32_t *g_cnt, uint32_t nbc, uint32_t top, uint32_t w0, uint32_t bkt) {
 	const int lane = threadIdx.x & (WAVE - 1);
 	const uint32_t prev = __shfl_up(bkt, 1);
 	const bool head = lane == 0 || bkt != prev;
@@ -383,8 +416,14 @@ __device__ __forceinline__ void wave_bucket_add(uint32_t *s_win,
 			: (uint32_t)WAVE;
 		const uint32_t run = next - (uint32_t)lane;
 		const uint32_t o = bkt - w0;
-		if (bkt >= w0 && o < WIN) atomicAdd(&s_win[o], run);
-		else atomicAdd(&g_cnt[bkt], run);
+		if (bkt >= w0 && o < WIN) {
+			atomicAdd(&s_win[o], run);
+		} else {
+			/* every level of the count tables in use */
+			atomicAdd(&g_cnt[bkt], run);
+			for (uint32_t k = 1; k <= top; k++)
+				atomicAdd(&g_cnt[cnt_off(nbc, (int)k) + (bkt >> (6 * k))], run);
+		}
 	}
 }
 
@@ -798,117 +837,24 @@ __global__ __launch_bounds__(WG) void k_sort_down(
  * cells just left and right of the block in row y (rows and columns wrap on
  * the torus): five contiguous ranges of the moved order. The emit counts, per
  * bucket, the moved agents by old cell (ocnt) and the survivors by new cell
- * (ncnt); k_bucket_scan turns both into start positions; k_bucket_sort gives
- * every bucket one work-group, which gathers the agents of its ranges that
- * land in it, in range order (= agent id order), sorts them by cell in LDS
- * (stable LSD radix on the cell offset) and writes them at the bucket's
- * start: the order of the key (cell << 32 | id), as the radix sort gives. */
+ * (ncnt); the prefix sums of both give the start positions (k_bscan_*, or
+ * the emit's last work-group when the buckets fit one scan chunk).
+ * k_bucket_sort gives every bucket one work-group, which
+ * gathers the agents of its ranges that land in it, in range order (= agent
+ * id order), sorts them stably by cell in LDS (a counting sort) and writes
+ * them at the bucket's start: the order of the key (cell << 32 | id), as the
+ * radix sort gives. */
 
 constexpr uint32_t BCAP = 1024;  /* agents of one bucket sorted in LDS */
+constexpr uint32_t W_MAX = 2048; /* cells of a bucket block: 2^S, S <= 11 */
+static_assert(W_MAX <= 2 * BCAP, "a dense bucket's counters reuse the staging");
 constexpr uint32_t BG_IPT = 8;   /* source positions per thread per round */
 /* Most buckets the automatic choice sorts by buckets (above, the radix sort
  * is faster: each bucket costs a few dependent round trips to memory). */
 constexpr uint64_t BUCKET_AUTO_MAX = 16384;
 
-/* Exclusive prefix sums of the bucket counts the emit gathered, each with its
- * total at [NB] (ostart: first moved position of each bucket's old cells;
- * nstart: first sorted position of each bucket), in chunks of BSC buckets:
- * k_bscan_part sums each chunk, k_bscan_top scans the chunk sums (one
- * work-group), k_bscan_down scans inside each chunk and clears the counts
- * for the next emit. With a single chunk, k_bscan_down alone does all. */
-constexpr uint32_t BSC_IPT = 8;
-constexpr uint32_t BSC = WG * BSC_IPT;   /* buckets per scan chunk */
-
-__global__ __launch_bounds__(WG) void k_bscan_part(const uint32_t *ocnt,
-	const uint32_t *ncnt, uint32_t *part, const Ctl *ctl) {
-	__shared__ unsigned long long s_red[NWAVE][2];
-	if (ctl->error) return;
-	const uint32_t NB = ctl->nbuckets, nch = (NB + BSC - 1) / BSC;
-	for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
-		unsigned long long v[2] = {0, 0};
-#pragma unroll
-		for (uint32_t i = 0; i < BSC_IPT; i++) {
-			const uint32_t b = c * BSC + i * WG + threadIdx.x;
-			if (b < NB) {
-				v[0] += ocnt[b];
-				v[1] += ncnt[b];
-			}
-		}
-		const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-#pragma unroll
-		for (int k = 0; k < 2; k++) {
-			const unsigned long long t = wave_sum(v[k]);
-			if (lane == 0) s_red[w][k] = t;
-		}
-		__syncthreads();
-		if (threadIdx.x < 2) {
-			unsigned long long t = 0;
-			for (int i = 0; i < NWAVE; i++) t += s_red[i][threadIdx.x];
-			part[2 * c + threadIdx.x] = (uint32_t)t;
-		}
-		__syncthreads();
-	}
-}
-
-__global__ __launch_bounds__(SCAN_WG) void k_bscan_top(uint32_t *part,
-	uint32_t *ostart, uint32_t *nstart, const Ctl *ctl) {
-	__shared__ uint32_t s_tmp[SCAN_WG / WAVE];
-	if (ctl->error) return;
-	const uint32_t NB = ctl->nbuckets, nch = (NB + BSC - 1) / BSC;
-	uint32_t carry[2] = {0, 0};
-	for (uint32_t c0 = 0; c0 < nch; c0 += SCAN_WG) {
-		const uint32_t c = c0 + threadIdx.x;
-#pragma unroll
-		for (int k = 0; k < 2; k++) {
-			const uint32_t v = c < nch ? part[2 * c + k] : 0u;
-			uint32_t tot;
-			const uint32_t pre = wg_excl_scan<SCAN_WG>(v, s_tmp, tot);
-			if (c < nch) part[2 * c + k] = carry[k] + pre;
-			carry[k] += tot;
-		}
-	}
-	if (threadIdx.x == 0) {
-		ostart[NB] = carry[0];
-		nstart[NB] = carry[1];
-	}
-}
-
-__global__ __launch_bounds__(WG) void k_bscan_down(uint32_t *ocnt,
-	uint32_t *ncnt, uint32_t *ostart, uint32_t *nstart, const uint32_t *part,
-	const Ctl *ctl, uint32_t single) {
-	__shared__ uint32_t s_tmp[NWAVE];
-	if (ctl->error) return;
-	const uint32_t NB = ctl->nbuckets, nch = (NB + BSC - 1) / BSC;
-	for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
-		const uint32_t b0 = c * BSC + threadIdx.x * BSC_IPT;
-#pragma unroll
-		for (int k = 0; k < 2; k++) {
-			uint32_t *cnt = k ? ncnt : ocnt;
-			uint32_t *st = k ? nstart : ostart;
-			uint32_t v[BSC_IPT], sum = 0;
-#pragma unroll
-			for (uint32_t i = 0; i < BSC_IPT; i++) {
-				v[i] = b0 + i < NB ? cnt[b0 + i] : 0u;
-				sum += v[i];
-			}
-			uint32_t tot;
-			uint32_t run = wg_excl_scan<WG>(sum, s_tmp, tot) +
-				(single ? 0u : part[2 * c + k]);
-#pragma unroll
-			for (uint32_t i = 0; i < BSC_IPT; i++) {
-				if (b0 + i < NB) {
-					st[b0 + i] = run;
-					cnt[b0 + i] = 0;
-				}
-				run += v[i];
-			}
-			if (single && threadIdx.x == 0) st[NB] = tot;
-		}
-	}
-}
-
-/* Clears the count tables (a tick sorted by the radix sort after a change
- * of the bucket layout: what the previous emit counted is void). */
+/* Clears the count tables, all levels (a tick sorted by the radix sort after
+ * a change of the bucket layout: what the previous emit counted is void). */
 __global__ __launch_bounds__(WG) void k_bscan_clear(uint32_t *ocnt,
 	uint32_t *ncnt, uint32_t words, const Ctl *ctl) {
 	if (ctl->error) return;
@@ -970,16 +916,21 @@ __device__ __forceinline__ void wave_match(uint32_t d, uint32_t bits,
 }
 
 template <typename GT>
-__global__ __launch_bounds__(WG) void k_bucket_sort(
+__global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
-	const uint32_t *__restrict__ ocell, const uint32_t *__restrict__ ostart,
-	const uint32_t *__restrict__ nstart, uint32_t *__restrict__ okeys,
-	uint32_t *__restrict__ ovals, Ctl *ctl, Tables T, Params P,
-	uint32_t *__restrict__ act_cnt, uint32_t act_words, GT *grass_at) {
-	__shared__ uint32_t s_buf[4 * BCAP];  /* offsets A | values A | B | B */
-	__shared__ uint32_t s_wh[NWAVE][65];
-	__shared__ uint32_t s_pos[64];
-	__shared__ uint32_t s_ctot[64];
+	const uint32_t *__restrict__ ocell, const uint32_t *__restrict__ ocnt,
+	const uint32_t *__restrict__ ncnt,
+	uint32_t *__restrict__ okeys, uint32_t *__restrict__ ovals, Ctl *ctl,
+	Tables T, Params P, uint32_t *__restrict__ act_cnt, uint32_t act_words,
+	GT *grass_at) {
+	/* cell offsets | values, in gather order (a dense bucket's per-cell
+	 * counters instead) */
+	__shared__ uint32_t s_buf[2 * BCAP];
+	__shared__ uint16_t s_ix[BCAP];        /* gather index of each sorted slot */
+	__shared__ uint32_t s_cnt2[W_MAX / 2]; /* per cell, 16 bits: count, start, end */
+	uint32_t *s_ka = s_buf, *s_va = s_buf + BCAP;
+	__shared__ uint32_t s_need[12];
+	__shared__ uint32_t s_wc[2][BG_IPT * NWAVE];
 	__shared__ uint32_t s_rng[10];
 	__shared__ uint32_t s_nrng;
 	__shared__ uint32_t s_tmp[NWAVE];
@@ -1003,51 +954,77 @@ __global__ __launch_bounds__(WG) void k_bucket_sort(
 	}
 	const uint32_t NB = ctl->nbuckets, nrep = ctl->nrep, S = P.bshift;
 	const uint32_t W = 1u << S;
-	/* LSD passes over the S offset bits, at most 6 bits each */
-	const uint32_t npass = (S + 5) / 6;
-	const uint32_t dbits = (S + npass - 1) / npass;
-	const uint32_t nd = 1u << dbits;
+
+	/* a plain simulation's one replica, loaded once */
+	uint32_t gx1 = 0, gy1 = 0, cb1 = 0;
+	if (nrep == 1) {
+		const Rep &rp = T.rep[T.order[0]];
+		gx1 = rp.gx;
+		gy1 = rp.gy;
+		cb1 = rp.cb;
+	}
+
 	for (uint32_t b = blockIdx.x; b < NB; b += gridDim.x) {
 		/* the bucket's replica (largest k with bbase[k] <= b), row and block */
-		uint32_t k = 0;
-		{
+		uint32_t gx = gx1, gy = gy1, cb = cb1, bb = 0;
+		if (nrep != 1) {
 			uint32_t lo = 0, hi = nrep;
 			while (hi - lo > 1) {
 				const uint32_t mid = (lo + hi) >> 1;
 				if (T.bbase[mid] <= b) lo = mid;
 				else hi = mid;
 			}
-			k = lo;
+			const Rep &rp = T.rep[T.order[lo]];
+			gx = rp.gx;
+			gy = rp.gy;
+			cb = rp.cb;
+			bb = T.bbase[lo];
 		}
-		const Rep &rp = T.rep[T.order[k]];
-		const uint32_t gx = rp.gx, gy = rp.gy, bb = T.bbase[k];
 		const uint32_t nb = row_blocks(gx, S);
 		const uint32_t local = b - bb;
 		const uint32_t y = local / nb, j = local - y * nb;
 		const uint32_t x0 = j << S, x1 = min(x0 + W, gx);
-		const uint32_t row0 = rp.cb + y * gx;
+		const uint32_t row0 = cb + y * gx;
 		const uint32_t cs = row0 + x0, ce = row0 + x1;
-		const uint32_t out0 = nstart[b], expect = nstart[b + 1] - out0;
+		/* buckets the agents come from: rows above and below, the block's
+		 * own, and (nb >= 2) those of the cells left and right of it */
+		const uint32_t xl = x0 ? x0 - 1 : gx - 1;
+		const uint32_t xr = x1 < gx ? x1 : 0;
+		uint32_t need[6];
+		need[0] = b;
+		need[1] = bb + ((y + gy - 1) % gy) * nb + j;
+		need[2] = b;
+		need[3] = bb + ((y + 1) % gy) * nb + j;
+		need[4] = bb + y * nb + (xl >> S);
+		need[5] = bb + y * nb + (xr >> S);
+		/* [start, end) of the survivors bound for bucket b (need[0], new
+		 * cells) and of the moved agents of need[1..5] (old cells): prefix
+		 * sums of the emit's counts, two needs per wave */
+#pragma unroll 1
+		for (int s = w; s < 6; s += NWAVE) {
+			const uint32_t q = need[s];
+			const uint32_t *cnt = s == 0 ? ncnt : ocnt;
+			const uint32_t c = cnt[q];
+			const uint32_t pre = cnt_prefix(cnt, P.nb_cap, P.cnt_top, q);
+			if (lane == 0) {
+				s_need[2 * s] = pre;
+				s_need[2 * s + 1] = pre + c;
+			}
+		}
+		__syncthreads();
+		const uint32_t out0 = s_need[0], expect = s_need[1] - out0;
 		/* source ranges of the moved order, sorted, without repeats */
 		if (w == 0) {
 			uint32_t rs[5], re[5];
-			const uint32_t bu = bb + ((y + gy - 1) % gy) * nb + j;
-			const uint32_t bm = bb + y * nb + j;
-			const uint32_t bd = bb + ((y + 1) % gy) * nb + j;
-			rs[0] = ostart[bu]; re[0] = ostart[bu + 1];
-			rs[1] = ostart[bm]; re[1] = ostart[bm + 1];
-			rs[2] = ostart[bd]; re[2] = ostart[bd + 1];
+#pragma unroll
+			for (int s = 1; s < 6; s++) {
+				rs[s - 1] = s_need[2 * s];
+				re[s - 1] = s_need[2 * s + 1];
+			}
 			uint32_t n = 3;
 			if (nb >= 2) {
-				const uint32_t xl = x0 ? x0 - 1 : gx - 1;   /* cell left of the block */
-				const uint32_t bl = bb + y * nb + (xl >> S);
-				const uint32_t el = ostart[bl + 1];
-				rs[3] = tail_of_cell(ocell, ostart[bl], el, row0 + xl);
-				re[3] = el;
-				const uint32_t xr = x1 < gx ? x1 : 0;     /* cell right of it */
-				const uint32_t br = bb + y * nb + (xr >> S);
-				rs[4] = ostart[br];
-				re[4] = head_of_cell(ocell, rs[4], ostart[br + 1], row0 + xr);
+				rs[3] = tail_of_cell(ocell, rs[3], re[3], row0 + xl);
+				re[4] = head_of_cell(ocell, rs[4], re[4], row0 + xr);
 				n = 5;
 			}
 			if (lane == 0) {
@@ -1068,64 +1045,69 @@ __global__ __launch_bounds__(WG) void k_bucket_sort(
 				s_nrng = m;
 			}
 		}
+		for (uint32_t i = tid; i < W / 2; i += WG) s_cnt2[i] = 0;
 		__syncthreads();
 		/* gather the agents landing in [cs, ce), in range order: the ranges
-		 * as one sequence of positions, BG_IPT consecutive ones per thread,
-		 * all loads of a round issued together */
-		uint32_t *ka = s_buf, *va = s_buf + BCAP;
-		uint32_t *kb = s_buf + 2 * BCAP, *vb = s_buf + 3 * BCAP;
+		 * as one sequence of positions, element e of a round at lane-
+		 * consecutive e = v0 + j WG + tid (coalesced), all loads of a round
+		 * issued together; the compaction keeps the sequence order */
 		uint32_t cnt = 0;
 		const uint32_t nr = s_nrng;
 		uint32_t nsrc = 0;
 		for (uint32_t r = 0; r < nr; r++) nsrc += s_rng[2 * r + 1] - s_rng[2 * r];
-		for (uint32_t v0 = 0; v0 < nsrc; v0 += WG * BG_IPT) {
-			/* position of this thread's first element */
-			uint32_t v = v0 + tid * BG_IPT, r = 0, p = 0, pend = 0;
-			{
-				uint32_t off = v;
-				while (r < nr && off >= s_rng[2 * r + 1] - s_rng[2 * r]) {
-					off -= s_rng[2 * r + 1] - s_rng[2 * r];
-					r++;
-				}
-				if (r < nr) {
-					p = s_rng[2 * r] + off;
-					pend = s_rng[2 * r + 1];
-				}
-			}
+		const uint64_t lt_mask = (1ull << lane) - 1ull;
+		for (uint32_t v0 = 0, rnd = 0; v0 < nsrc; v0 += WG * BG_IPT, rnd++) {
 			uint32_t kk[BG_IPT], vv[BG_IPT];
 #pragma unroll
-			for (uint32_t i = 0; i < BG_IPT; i++) {
-				kk[i] = DEAD_KEY;
-				vv[i] = 0u;
-				if (v + i < nsrc) {
-					kk[i] = keys[p];
-					vv[i] = vals[p];
-				}
-				p++;
-				if (p == pend && r + 1 < nr) {
-					r++;
-					p = s_rng[2 * r];
-					pend = s_rng[2 * r + 1];
-				}
-			}
-			uint32_t n_in = 0;
-#pragma unroll
-			for (uint32_t i = 0; i < BG_IPT; i++) n_in += kk[i] >= cs && kk[i] < ce;
-			uint32_t tot;
-			uint32_t slot = cnt + wg_excl_scan<WG>(n_in, s_tmp, tot);
-#pragma unroll
-			for (uint32_t i = 0; i < BG_IPT; i++) {
-				if (kk[i] >= cs && kk[i] < ce) {
-					if (slot < BCAP) {
-						ka[slot] = kk[i] - cs;
-						va[slot] = vv[i];
+			for (uint32_t jj = 0; jj < BG_IPT; jj++) {
+				const uint32_t e = v0 + jj * WG + tid;
+				kk[jj] = DEAD_KEY;
+				vv[jj] = 0u;
+				if (e < nsrc) {
+					uint32_t r = 0, c0 = 0;
+					while (r + 1 < nr && e >= c0 + s_rng[2 * r + 1] - s_rng[2 * r]) {
+						c0 += s_rng[2 * r + 1] - s_rng[2 * r];
+						r++;
 					}
-					slot++;
+					const uint32_t p = s_rng[2 * r] + (e - c0);
+					kk[jj] = keys[p];
+					vv[jj] = vals[p];
 				}
 			}
-			cnt += tot;
+			/* per element: its rank among the wave's landing ones; per
+			 * (element row jj, wave): their count, in LDS */
+			uint32_t *wc = s_wc[rnd & 1];
+			uint32_t rank[BG_IPT], in_mask = 0;
+#pragma unroll
+			for (uint32_t jj = 0; jj < BG_IPT; jj++) {
+				const bool in = kk[jj] >= cs && kk[jj] < ce;
+				const uint64_t bal = __ballot(in);
+				rank[jj] = (uint32_t)__popcll(bal & lt_mask);
+				in_mask |= in ? 1u << jj : 0u;
+				if (lane == 0) wc[jj * NWAVE + w] = (uint32_t)__popcll(bal);
+			}
+			__syncthreads();
+			uint32_t run = 0;
+#pragma unroll
+			for (uint32_t jj = 0; jj < BG_IPT; jj++) {
+				uint32_t off = 0;
+#pragma unroll
+				for (int ww = 0; ww < NWAVE; ww++) {
+					const uint32_t c = wc[jj * NWAVE + ww];
+					if (ww == w) off = run;
+					run += c;
+				}
+				if (in_mask & (1u << jj)) {
+					const uint32_t slot = cnt + off + rank[jj];
+					if (slot < BCAP) {
+						s_ka[slot] = kk[jj] - cs;
+						s_va[slot] = vv[jj];
+					}
+				}
+			}
+			cnt += run;
 		}
-		/* the last chunk's LDS writes land before anyone reuses the buffer */
+		/* the last chunk's LDS writes land before anyone reads them */
 		__syncthreads();
 		if (cnt != expect) {
 			/* the counts do not describe the agents */
@@ -1134,67 +1116,68 @@ __global__ __launch_bounds__(WG) void k_bucket_sort(
 			continue;
 		}
 		if (cnt <= BCAP) {
-			/* stable LSD radix sort of the offsets in LDS */
-			for (uint32_t pass = 0; pass < npass; pass++) {
-				const uint32_t sh = pass * dbits;
-				if (tid < (int)nd) s_pos[tid] = 0;
-				__syncthreads();
-				for (uint32_t i = tid; i < cnt; i += WG)
-					atomicAdd(&s_pos[(ka[i] >> sh) & (nd - 1)], 1u);
-				__syncthreads();
-				if (w == 0) {
-					uint32_t v = lane < (int)nd ? s_pos[lane] : 0u, x = v;
-					for (int o = 1; o < WAVE; o <<= 1) {
-						const uint32_t t = __shfl_up(x, o);
-						if (lane >= o) x += t;
-					}
-					if (lane < (int)nd) s_pos[lane] = x - v;
-				}
-				__syncthreads();
-				for (uint32_t c0 = 0; c0 < cnt; c0 += WG) {
-					const uint32_t i = c0 + tid;
-					const bool valid = i < cnt;
-					const uint32_t off = valid ? ka[i] : 0u;
-					const uint32_t val = valid ? va[i] : 0u;
-					const uint32_t d = valid ? (off >> sh) & (nd - 1) : nd;
-					for (uint32_t q = tid; q < NWAVE * 65; q += WG)
-						(&s_wh[0][0])[q] = 0;
-					__syncthreads();
-					uint32_t rank, count;
-					wave_match(d, dbits, rank, count);
-					if (valid && rank == 0) s_wh[w][d] = count;
-					__syncthreads();
-					if (tid < (int)nd) {
-						uint32_t acc = 0;
-						for (int ww = 0; ww < NWAVE; ww++) {
-							const uint32_t t = s_wh[ww][tid];
-							s_wh[ww][tid] = acc;
-							acc += t;
-						}
-						s_ctot[tid] = acc;
-					}
-					__syncthreads();
-					if (valid) {
-						const uint32_t dst = s_pos[d] + s_wh[w][d] + rank;
-						kb[dst] = off;
-						vb[dst] = val;
-					}
-					__syncthreads();
-					if (tid < (int)nd) s_pos[tid] += s_ctot[tid];
-					__syncthreads();
-				}
-				uint32_t *t;
-				t = ka; ka = kb; kb = t;
-				t = va; va = vb; vb = t;
-			}
+			/* counting sort by cell offset in LDS: counts, exclusive scan,
+			 * scatter of the gather indices, then each cell's few slots put
+			 * back in gather (= id) order. Cell c's counter is half c & 1
+			 * of word c / 2 (values stay below 2^16: cnt <= BCAP). */
 			for (uint32_t i = tid; i < cnt; i += WG) {
-				okeys[out0 + i] = cs + ka[i];
-				ovals[out0 + i] = va[i];
+				const uint32_t c = s_ka[i];
+				atomicAdd(&s_cnt2[c >> 1], 1u << ((c & 1u) * 16));
+			}
+			__syncthreads();
+			{
+				/* thread t: words t pw .. t pw + pw - 1 */
+				const uint32_t nw = W / 2, pw = nw > WG ? nw / WG : 1u;
+				const uint32_t w0 = tid * pw;
+				uint32_t v[W_MAX / 2 / WG], sum = 0;
+#pragma unroll
+				for (uint32_t i = 0; i < W_MAX / 2 / WG; i++) {
+					v[i] = i < pw && w0 + i < nw ? s_cnt2[w0 + i] : 0u;
+					sum += (v[i] & 0xFFFFu) + (v[i] >> 16);
+				}
+				uint32_t tot;
+				uint32_t run = wg_excl_scan<WG>(sum, s_tmp, tot);
+#pragma unroll
+				for (uint32_t i = 0; i < W_MAX / 2 / WG; i++) {
+					const uint32_t lo = v[i] & 0xFFFFu;
+					if (i < pw && w0 + i < nw) s_cnt2[w0 + i] = run | ((run + lo) << 16);
+					run += lo + (v[i] >> 16);
+				}
+			}
+			__syncthreads();
+			for (uint32_t i = tid; i < cnt; i += WG) {
+				const uint32_t c = s_ka[i], sh = (c & 1u) * 16;
+				const uint32_t old = atomicAdd(&s_cnt2[c >> 1], 1u << sh);
+				s_ix[(old >> sh) & 0xFFFFu] = (uint16_t)i;
+			}
+			__syncthreads();
+			/* each counter is now the end of its cell's slots; the thread of
+			 * a cell's first slot orders that cell's slots */
+			for (uint32_t q = tid; q < cnt; q += WG) {
+				const uint32_t c = s_ka[s_ix[q]];
+				const uint32_t e = (s_cnt2[c >> 1] >> ((c & 1u) * 16)) & 0xFFFFu;
+				const uint32_t s0 = c ? (s_cnt2[(c - 1) >> 1] >> (((c - 1) & 1u) * 16)) & 0xFFFFu : 0u;
+				if (q != s0 || e - s0 < 2) continue;
+				for (uint32_t a = s0 + 1; a < e; a++) {
+					const uint16_t x = s_ix[a];
+					uint32_t d = a;
+					while (d > s0 && s_ix[d - 1] > x) {
+						s_ix[d] = s_ix[d - 1];
+						d--;
+					}
+					s_ix[d] = x;
+				}
+			}
+			__syncthreads();
+			for (uint32_t q = tid; q < cnt; q += WG) {
+				const uint32_t i = s_ix[q];
+				okeys[out0 + q] = cs + s_ka[i];
+				ovals[out0 + q] = s_va[i];
 			}
 		} else {
 			/* More agents than LDS holds (a dense bucket): count per cell,
 			 * then one wave places them in range order straight to HBM. */
-			uint32_t *s_run = s_buf;     /* W <= 4 BCAP counters */
+			uint32_t *s_run = s_buf;     /* W <= W_MAX <= 2 BCAP counters */
 			for (uint32_t i = tid; i < W; i += WG) s_run[i] = 0;
 			__syncthreads();
 			for (uint32_t r = 0; r < nr; r++) {
@@ -1588,11 +1571,23 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 template <typename GT>
 __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 	uint32_t *vals, GT *grass_at, uint32_t *act_cnt, uint32_t *mixed_list,
-	Ctl *ctl, Tables T, Params P) {
+	uint32_t *ocnt, uint32_t *ncnt, Ctl *ctl, Tables T, Params P) {
 	__shared__ ActLds<GT> S;
 	if (ctl->error) return;
 	const int tid = threadIdx.x;
 	const uint32_t tick = ctl->tick, nrep = ctl->nrep;
+	/* The sort has read the bucket counts: clear the entries in use for
+	 * this tick's emit. */
+	if (P.bucket_on) {
+		const uint32_t NB = ctl->nbuckets;
+		for (uint32_t k = 0; k <= P.cnt_top; k++) {
+			const uint32_t o = cnt_off(P.nb_cap, (int)k), n = (NB >> (6 * k)) + 1;
+			for (uint32_t i = blockIdx.x * WG + tid; i < n; i += gridDim.x * WG) {
+				ocnt[o + i] = 0;
+				ncnt[o + i] = 0;
+			}
+		}
+	}
 	const uint32_t L = ctl->live[tick & 1];
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
 	unsigned long long acc[NACC] = {0, 0, 0, 0, 0, 0, 0};
@@ -1860,8 +1855,8 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 			live += mk != DEAD_KEY;
 		}
 		if (P.bucket_on) {
-			wave_bucket_add<BWIN>(s_win, ocnt, w0, ob);
-			wave_bucket_add<BWIN>(s_win + BWIN, ncnt, w0, nbk);
+			wave_bucket_add<BWIN>(s_win, ocnt, P.nb_cap, P.cnt_top, w0, ob);
+			wave_bucket_add<BWIN>(s_win + BWIN, ncnt, P.nb_cap, P.cnt_top, w0, nbk);
 		}
 	}
 	return live;
@@ -1882,6 +1877,7 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	uint32_t *__restrict__ mkeys, uint32_t *__restrict__ mvals,
 	uint32_t *ocnt, uint32_t *ncnt, Ctl *ctl, Tables T, Params P) {
 	__shared__ uint32_t s_oc[A_STAGE];
+	__shared__ uint32_t s_up[2][BWIN / 64 + 1];
 	__shared__ uint32_t s_ov[A_STAGE];
 	__shared__ uint32_t s_win[2 * BWIN];
 	__shared__ uint32_t s_tmp[NWAVE];
@@ -1901,6 +1897,7 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	if (nrep == 1 && ntiles > blockIdx.x) m_one = load_rm(T, 0, tick, S);
 
 	for (uint32_t i = tid; i < 2 * BWIN; i += WG) s_win[i] = 0;
+	if (tid < 2 * (BWIN / 64 + 1)) (&s_up[0][0])[tid] = 0;
 	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 		const uint32_t base = tile * A_TILE;
 		const uint32_t goff = act_off[tile];
@@ -1972,12 +1969,43 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 				overflow);
 		__syncthreads();
 		if (P.bucket_on) {
+			/* level 0 of the window, summed into its level-1 entries */
 			for (uint32_t i = tid; i < 2 * BWIN; i += WG) {
 				const uint32_t h = s_win[i];
 				if (h) {
-					atomicAdd(&(i < BWIN ? ocnt : ncnt)[w0 + (i % BWIN)], h);
+					const uint32_t k = i / BWIN, q = w0 + (i % BWIN);
+					atomicAdd(&(k ? ncnt : ocnt)[q], h);
+					atomicAdd(&s_up[k][(q >> 6) - (w0 >> 6)], h);
 					s_win[i] = 0;
 				}
+			}
+			__syncthreads();
+			if (P.cnt_top >= 1 && tid < 2) {
+				/* levels 1 .. cnt_top: the window's few level-1 entries, and
+				 * their sums for each higher entry they fall in */
+				uint32_t *cnt = tid ? ncnt : ocnt;
+				uint32_t run[CNT_LEVELS] = {0, 0, 0, 0}, at[CNT_LEVELS] = {0, 0, 0, 0};
+				for (uint32_t e = 0; e <= BWIN / 64; e++) {
+					const uint32_t h = s_up[tid][e];
+					if (!h) continue;
+					s_up[tid][e] = 0;
+					const uint32_t q1 = (w0 >> 6) + e;
+					atomicAdd(&cnt[cnt_off(P.nb_cap, 1) + q1], h);
+#pragma unroll
+					for (int k = 2; k < CNT_LEVELS; k++) {
+						if ((uint32_t)k > P.cnt_top) break;
+						const uint32_t q = q1 >> (6 * (k - 1));
+						if (run[k] && q != at[k]) {
+							atomicAdd(&cnt[cnt_off(P.nb_cap, k) + at[k]], run[k]);
+							run[k] = 0;
+						}
+						at[k] = q;
+						run[k] += h;
+					}
+				}
+#pragma unroll
+				for (int k = 2; k < CNT_LEVELS; k++)
+					if (run[k]) atomicAdd(&cnt[cnt_off(P.nb_cap, k) + at[k]], run[k]);
 			}
 			__syncthreads();
 		}
@@ -2145,12 +2173,10 @@ struct pp_batch {
 	uint32_t *act_cnt = nullptr;
 	uint32_t *mixed_list = nullptr;  /* action tiles of several replicas */
 	uint32_t act_words = 0;
-	/* bucket sort: per-bucket counts (emit) and starts (scan), nb_cap + 1
-	 * entries each; first bucket of each replica of the table */
+	/* bucket sort: per-bucket counts by old and new cell (emit), all
+	 * levels; first bucket of each replica of the table */
 	uint32_t *ocnt = nullptr, *ncnt = nullptr;
-	uint32_t *ostart = nullptr, *nstart = nullptr;
 	uint32_t *bbase_dev = nullptr;
-	uint32_t *bpart = nullptr;        /* scan chunk sums, 2 per chunk */
 	int need_radix = 1;   /* the next tick sorts with the radix sort */
 	/* 0: radix sort every tick; 1: automatic; 8..11: row buckets of
 	 * 2^mode cells */
@@ -2300,8 +2326,8 @@ static void launch_sort_pass(pp_batch_t *b, uint32_t pass) {
 template <typename GT>
 static void launch_act(pp_batch_t *b, int sorted) {
 	k_act<GT><<<b->grid_act, WG, 0, b->stream>>>(b->keys[sorted],
-		b->vals[sorted], (GT *)b->grass_at, b->act_cnt, b->mixed_list, b->ctl,
-		b->tables(), b->P);
+		b->vals[sorted], (GT *)b->grass_at, b->act_cnt, b->mixed_list, b->ocnt,
+		b->ncnt, b->ctl, b->tables(), b->P);
 	/* A batch of one replica never has a tile of several. */
 	if (b->max_rep > 1)
 		k_act_mixed<GT><<<min_u32(b->max_rep, 256), WG, 0, b->stream>>>(
@@ -2312,7 +2338,7 @@ static void launch_act(pp_batch_t *b, int sorted) {
 template <typename GT>
 static void launch_bucket_sort(pp_batch_t *b) {
 	k_bucket_sort<GT><<<b->grid_bucket, WG, 0, b->stream>>>(b->keys[0],
-		b->vals[0], b->cell, b->ostart, b->nstart, b->keys[1], b->vals[1],
+		b->vals[0], b->cell, b->ocnt, b->ncnt, b->keys[1], b->vals[1],
 		b->ctl, b->tables(), b->P, b->act_cnt, b->act_words, (GT *)b->grass_at);
 }
 
@@ -2323,24 +2349,13 @@ static void launch_bucket_sort(pp_batch_t *b) {
 static int enqueue_step(pp_batch_t *b, int radix) {
 	PendingEvent pe;
 	int sorted = 1;
-	prof_begin(b, KI_BUCKET_SCAN, pe);
-	if (radix) {
+	if (radix && b->P.bucket_on) {
 		/* counts of a previous layout are void */
-		if (b->P.bucket_on)
-			k_bscan_clear<<<b->grid_bucket, WG, 0, b->stream>>>(b->ocnt, b->ncnt,
-				b->P.nb_cap + 1, b->ctl);
-	} else if (b->P.bscan_one) {
-		k_bscan_down<<<1, WG, 0, b->stream>>>(b->ocnt, b->ncnt, b->ostart,
-			b->nstart, b->bpart, b->ctl, 1);
-	} else {
-		k_bscan_part<<<b->grid_bucket, WG, 0, b->stream>>>(b->ocnt, b->ncnt,
-			b->bpart, b->ctl);
-		k_bscan_top<<<1, SCAN_WG, 0, b->stream>>>(b->bpart, b->ostart,
-			b->nstart, b->ctl);
-		k_bscan_down<<<b->grid_bucket, WG, 0, b->stream>>>(b->ocnt, b->ncnt,
-			b->ostart, b->nstart, b->bpart, b->ctl, 0);
+		prof_begin(b, KI_BUCKET_SCAN, pe);
+		k_bscan_clear<<<b->grid_bucket, WG, 0, b->stream>>>(b->ocnt, b->ncnt,
+			cnt_off(b->P.nb_cap, CNT_LEVELS), b->ctl);
+		prof_end(b, pe);
 	}
-	prof_end(b, pe);
 	if (radix) {
 		for (uint32_t pass = 0; pass < b->P.passes; pass++)
 			launch_sort_pass(b, pass);
@@ -2511,13 +2526,14 @@ static int upload_table(pp_batch_t *b) {
 	/* Row buckets when chosen, or, automatically, while few enough that
 	 * their per-bucket latency stays below the radix sort's passes. */
 	const uint32_t on = (b->sort_mode != 0 && nbk <= b->P.nb_cap &&
+		nbk <= CNT_MAX_BUCKETS &&
 		(b->sort_mode != 1 || nbk <= BUCKET_AUTO_MAX)) ? 1u : 0u;
-	const uint32_t one = nbk <= BSC ? 1u : 0u;
-	if (on != b->P.bucket_on || one != b->P.bscan_one ||
-		shift != b->P.bshift) {
+	uint32_t top = 0;
+	while (top + 1 < (uint32_t)CNT_LEVELS && (nbk >> (6 * (top + 1))) > 0) top++;
+	if (on != b->P.bucket_on || shift != b->P.bshift || top != b->P.cnt_top) {
 		b->P.bucket_on = on;
-		b->P.bscan_one = one;
 		b->P.bshift = shift;
+		b->P.cnt_top = top;
 		drop_graph(b);     /* Params are launch arguments of the graphs */
 	}
 	const uint32_t nb32 = on ? (uint32_t)nbk : 0u;
@@ -2681,10 +2697,7 @@ static void batch_free(pp_batch_t *b) {
 	(void)hipFree(b->mixed_list);
 	(void)hipFree(b->ocnt);
 	(void)hipFree(b->ncnt);
-	(void)hipFree(b->ostart);
-	(void)hipFree(b->nstart);
 	(void)hipFree(b->bbase_dev);
-	(void)hipFree(b->bpart);
 	(void)hipFree(b->ctl);
 	(void)hipFree(b->rep_dev);
 	(void)hipFree(b->rctl_dev);
@@ -2710,8 +2723,7 @@ static uint64_t batch_bytes(uint64_t A, uint64_t C, uint32_t max_rep,
 	uint32_t R, uint32_t stats_rows, uint64_t hist_words) {
 	const uint64_t A4 = (A + 3) & ~3ull;
 	uint64_t t = A4 * (4 + 4 + 1) + 6 * A4 * 4;
-	t += 16ull * ((uint64_t)nb_cap_for(C) + 1) + 4ull * (max_rep + 1);
-	t += 8ull * ((uint64_t)nb_cap_for(C) / BSC + 2);
+	t += 8ull * cnt_off(nb_cap_for(C), CNT_LEVELS) + 4ull * (max_rep + 1);
 	t += C * grass_bytes_for(R);
 	t += (uint64_t)max_rep * (R + 1) * 8;
 	t += hist_words * 4 + RADIX_MAX * 4;
@@ -2752,15 +2764,12 @@ static int batch_alloc(pp_batch_t *b) {
 	b->act_words = ceil_div(A, A_TILE) + 1;
 	HIP_OK(hipMalloc(&b->act_cnt, (size_t)b->act_words * 4));
 	HIP_OK(hipMalloc(&b->mixed_list, (size_t)b->act_words * 4));
-	const size_t nbw = (size_t)b->P.nb_cap + 1;
+	const size_t nbw = cnt_off(b->P.nb_cap, CNT_LEVELS);
 	HIP_OK(hipMalloc(&b->ocnt, nbw * 4));
 	HIP_OK(hipMalloc(&b->ncnt, nbw * 4));
-	HIP_OK(hipMalloc(&b->ostart, nbw * 4));
-	HIP_OK(hipMalloc(&b->nstart, nbw * 4));
 	HIP_OK(hipMemset(b->ocnt, 0, nbw * 4));
 	HIP_OK(hipMemset(b->ncnt, 0, nbw * 4));
 	HIP_OK(hipMalloc(&b->bbase_dev, ((size_t)b->max_rep + 1) * 4));
-	HIP_OK(hipMalloc(&b->bpart, ((size_t)b->P.nb_cap / BSC + 2) * 8));
 	HIP_OK(hipMalloc(&b->ctl, sizeof(Ctl)));
 	HIP_OK(hipMalloc(&b->rep_dev, (size_t)b->max_rep * sizeof(Rep)));
 	HIP_OK(hipMalloc(&b->rctl_dev, (size_t)b->max_rep * sizeof(RepCtl)));

@@ -181,10 +181,12 @@ def test_reinit_graph_and_profiling_modes_agree():
     assert prof["act"][0] == 300 and prof["emit_move"][0] == 300
     assert prof["act_scan"][0] == 300 and prof["move0"][0] == 1
     # iteration 1 sorts the freshly placed agents with the radix sort
-    # (40000 cells: 16 bits = 2 passes of 8), the 299 others by row buckets
+    # (40000 cells: 16 bits = 2 passes of 8) after clearing the bucket
+    # counts, the 299 others by row buckets; the 200 buckets fit one scan
+    # chunk, which the bucket sort scans itself
     assert prof["sort_down"][0] == 2 and prof["sort_scan"][0] == 2
     assert prof["sort_up"][0] == 2
-    assert prof["bucket_sort"][0] == 299 and prof["bucket_scan"][0] == 300
+    assert prof["bucket_sort"][0] == 299 and prof["bucket_scan"][0] == 1
 
 
 @pytest.mark.parametrize("shift", [0, 8, 9, 11])
@@ -200,7 +202,9 @@ def test_sort_variants_agree(shift):
                    init_wolves=300), 5),
              # rows of ~5000 agents: more than a bucket sorts in LDS at 2^11
              (dict(v2ref(60), grid_x=2048, grid_y=4, grid_xy=8192,
-                   init_sheep=16000, init_wolves=4000), 6)]
+                   init_sheep=16000, init_wolves=4000), 6),
+             # more buckets than one scan chunk holds: the separate scans
+             (v2s(gx=300, gy=2500), 7)]
     for d, seed in cases:
         sim = _sim(d, seed=seed)
         sim.set_sort(shift)
