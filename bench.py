@@ -16,7 +16,11 @@ Also reported (one JSON line on stdout, progress on stderr):
                 algorithmic bytes per launch (exact, from the run's event
                 counters) / average launch duration (HIP events on the
                 simulation's stream), against 8 TB/s
-  roofline_12800  the same for the 12800x12800 run (HBM-resident working set)
+  roofline_12800  the same for the 12800x12800 run (HBM-resident working
+                set), every iteration sorted by the radix sort (the
+                bandwidth-bound path)
+  roofline_12800_auto  the same with the automatic sort (row buckets: fewer
+                bytes, latency-bound), and its time per iteration
   cpu_baseline  the OpenMP restatement of the step (row bands, as the
                 reference's CPU simulator) on this host's cores, and the
                 single-threaded C oracle, each on the first iterations of the
@@ -117,12 +121,22 @@ def kernel_bytes(name, c, passes, radix_all=False):
     return None
 
 
-def measure_roofline(params, capacity, steps, seed=0):
-    """Profiled run: per-kernel event timing on the simulation's stream."""
+def measure_roofline(params, capacity, steps, seed=0, sort_mode=None):
+    """Profiled run: per-kernel event timing on the simulation's stream, and
+    the unprofiled time per iteration of the same iterations (graphs)."""
     from pphpc_amd import Params, Simulation
     sim = Simulation(Params.from_values(**params), seed=seed,
                      capacity=capacity, device=0)
+    if sort_mode is not None:
+        sim.set_sort(sort_mode)
     passes = sim.info()["sort_passes"]
+    sim.init()
+    sim.run(1, collect=False)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(steps, collect=False)
+    sim.sync()
+    ms_tick = 1e3 * (time.perf_counter() - t0) / steps
     sim.init()
     sim.run(min(steps, 4), collect=False)      # warm the code paths
     sim.init()
@@ -162,6 +176,8 @@ def measure_roofline(params, capacity, steps, seed=0):
         "bytes_per_launch": int(total_bytes / launches),
         "avg_launch_us": round(ns / launches / 1e3, 3),
         "profiled_steps": steps,
+        "ms_per_iteration": round(ms_tick, 4),
+        "sort": "radix" if radix_all else "row buckets",
         "per_kernel": per_kernel,
     }
 
@@ -309,7 +325,7 @@ def main():
     ap.add_argument("--workload", default="v2s1600", choices=list(WORKLOADS))
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--roofline-steps", type=int, default=400)
-    ap.add_argument("--big-roofline-steps", type=int, default=6)
+    ap.add_argument("--big-roofline-steps", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--cpu-single-steps", type=int, default=150)
     ap.add_argument("--cpu-threads", type=int,
@@ -366,15 +382,20 @@ def main():
                                                   args.steps), args.seed)
     log(f"dominant kernel {roof['kernel']}: {roof['achieved']} GB/s")
 
-    roof_big = None
+    roof_big = roof_big_b = None
     if not args.no_big:
         big = v2s_params(12800)
         nb = big["init_sheep"] + big["init_wolves"]
-        log("profiled 12800x12800 run for the HBM roofline")
+        log("profiled 12800x12800 runs for the HBM roofline: radix sort")
         roof_big = measure_roofline(big, 4 * nb, args.big_roofline_steps,
-                                    args.seed)
-        log(f"12800^2 dominant kernel {roof_big['kernel']}: "
-            f"{roof_big['achieved']} GB/s")
+                                    args.seed, sort_mode=0)
+        log(f"12800^2 radix: dominant kernel {roof_big['kernel']}: "
+            f"{roof_big['achieved']} GB/s, {roof_big['ms_per_iteration']} ms")
+        roof_big_b = measure_roofline(big, 4 * nb, args.big_roofline_steps,
+                                      args.seed)
+        log(f"12800^2 auto (buckets): dominant kernel {roof_big_b['kernel']}: "
+            f"{roof_big_b['achieved']} GB/s, "
+            f"{roof_big_b['ms_per_iteration']} ms")
 
     batched = None
     if not args.no_batch:
@@ -421,6 +442,7 @@ def main():
         },
         "roofline": roof,
         "roofline_12800": roof_big,
+        "roofline_12800_auto": roof_big_b,
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -303,12 +303,26 @@ struct RM {
 	uint32_t base;   /* batch position of its local agent id 0 */
 	uint32_t bbase;  /* its first bucket */
 	uint32_t nb;     /* bucket blocks per grid row */
+	uint32_t h;      /* grid rows per bucket */
 	bool active;
 };
 
 /* Blocks of 2^S cells per grid row of width gx. */
-__device__ __forceinline__ uint32_t row_blocks(uint32_t gx, uint32_t S) {
+__host__ __device__ __forceinline__ uint32_t row_blocks(uint32_t gx, uint32_t S) {
 	return (gx + (1u << S) - 1u) >> S;
+}
+
+/* Grid rows per bucket: rows no wider than a block share a bucket, as many
+ * whole rows as fit in 2^S cells; a wider row is cut into blocks. */
+__host__ __device__ __forceinline__ uint32_t rows_per_bucket(uint32_t gx,
+	uint32_t S) {
+	return gx > (1u << S) ? 1u : (1u << S) / gx;
+}
+
+/* Bucket of local cell (x, y). */
+__device__ __forceinline__ uint32_t bucket_of(uint32_t bbase, uint32_t nb,
+	uint32_t h, uint32_t x, uint32_t y, uint32_t S) {
+	return bbase + (h == 1 ? y : y / h) * nb + (x >> S);
 }
 
 /* The move constants of the k-th replica of the table at tick `tick`. */
@@ -324,6 +338,7 @@ __device__ __forceinline__ RM load_rm(const Tables &T, uint32_t k,
 	m.gy = rp.gy;
 	m.bbase = T.bbase[k];
 	m.nb = row_blocks(rp.gx, S);
+	m.h = rows_per_bucket(rp.gx, S);
 	m.sheep_prob = rp.sheep_prob;
 	m.wolves_prob = rp.wolves_prob;
 	m.key0 = rp.key0;
@@ -362,7 +377,7 @@ __device__ __forceinline__ uint32_t move_agent(uint32_t id, uint32_t x,
 	overflow |= e > (int32_t)V_ENERGY;
 	*val = ((uint32_t)e & V_ENERGY) | (wolf ? V_WOLF : 0u)
 		| (dice < prob ? V_REPRO : 0u);
-	*bkt = m.bbase + ny * m.nb + (nx >> S);
+	*bkt = bucket_of(m.bbase, m.nb, m.h, nx, ny, S);
 	return m.cb + ny * m.gx + nx;
 }
 
@@ -399,13 +414,21 @@ __device__ __forceinline__ uint32_t cnt_prefix(const uint32_t *cnt,
 	return v;
 }
 
+/* LDS windows of the emit's bucket counts: WIN level-0 entries from w0 and
+ * L1W level-1 entries from u0 (level 1 covers 64 buckets an entry, so the
+ * rows above and below of any grid up to 4096 blocks wide fall in it). */
+constexpr uint32_t L1W = 130;
+constexpr uint32_t L2W = 4;      /* level-2 entries the level-1 window spans */
+
 /* Adds 1 to the count of bucket `bkt` for every lane of the wave (NO_DIGIT
  * lanes count nothing), merging runs of equal buckets in consecutive lanes
- * first: into the LDS window s_win of WIN buckets from w0 when it falls
- * there, else straight into every level of the global table g_cnt. */
+ * first: level 0 into the LDS window s_win of WIN buckets from w0 when it
+ * falls there, else straight into the global table; levels 1 .. top into
+ * the LDS window s_l1 from u0, else straight into the global levels. */
 template <uint32_t WIN>
 __device__ __forceinline__ void wave_bucket_add(uint32_t *s_win,
-	uint32_t *g_cnt, uint32_t nbc, uint32_t top, uint32_t w0, uint32_t bkt) {
+	uint32_t *s_l1, uint32_t *g_cnt, uint32_t nbc, uint32_t top, uint32_t w0,
+	uint32_t u0, uint32_t bkt) {
 	const int lane = threadIdx.x & (WAVE - 1);
 	const uint32_t prev = __shfl_up(bkt, 1);
 	const bool head = lane == 0 || bkt != prev;
@@ -416,13 +439,16 @@ __device__ __forceinline__ void wave_bucket_add(uint32_t *s_win,
 			: (uint32_t)WAVE;
 		const uint32_t run = next - (uint32_t)lane;
 		const uint32_t o = bkt - w0;
-		if (bkt >= w0 && o < WIN) {
-			atomicAdd(&s_win[o], run);
-		} else {
-			/* every level of the count tables in use */
-			atomicAdd(&g_cnt[bkt], run);
-			for (uint32_t k = 1; k <= top; k++)
-				atomicAdd(&g_cnt[cnt_off(nbc, (int)k) + (bkt >> (6 * k))], run);
+		if (bkt >= w0 && o < WIN) atomicAdd(&s_win[o], run);
+		else atomicAdd(&g_cnt[bkt], run);
+		if (top >= 1) {
+			const uint32_t q1 = bkt >> 6, o1 = q1 - u0;
+			if (q1 >= u0 && o1 < L1W) {
+				atomicAdd(&s_l1[o1], run);
+			} else {
+				for (uint32_t k = 1; k <= top; k++)
+					atomicAdd(&g_cnt[cnt_off(nbc, (int)k) + (bkt >> (6 * k))], run);
+			}
 		}
 	}
 }
@@ -561,6 +587,7 @@ __global__ __launch_bounds__(WG) void k_move0(const uint32_t *__restrict__ cell,
 	m.base = pos;
 	m.bbase = 0;
 	m.nb = row_blocks(rp.gx, P.bshift);
+	m.h = rows_per_bucket(rp.gx, P.bshift);
 	m.active = rc.status == RS_ACTIVE;
 	const uint32_t n = rp.n0;
 	unsigned long long live = 0;
@@ -849,8 +876,11 @@ constexpr uint32_t BCAP = 1024;  /* agents of one bucket sorted in LDS */
 constexpr uint32_t W_MAX = 2048; /* cells of a bucket block: 2^S, S <= 11 */
 static_assert(W_MAX <= 2 * BCAP, "a dense bucket's counters reuse the staging");
 constexpr uint32_t BG_IPT = 8;   /* source positions per thread per round */
-/* Most buckets the automatic choice sorts by buckets (above, the radix sort
- * is faster: each bucket costs a few dependent round trips to memory). */
+/* Most buckets the automatic choice sorts by buckets in a batch of several
+ * replicas (above, the radix sort is faster there: each bucket costs a few
+ * dependent round trips to memory, and many small replicas make many
+ * sparse buckets); one replica sorts by buckets at any size (12800 x 12800:
+ * 89,600 buckets, 0.67 ms against 1.34 ms for four radix passes). */
 constexpr uint64_t BUCKET_AUTO_MAX = 16384;
 
 /* Clears the count tables, all levels (a tick sorted by the radix sort after
@@ -980,21 +1010,25 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 			cb = rp.cb;
 			bb = T.bbase[lo];
 		}
-		const uint32_t nb = row_blocks(gx, S);
+		/* rows [y * h, y * h + h) of the grid, block j (nb >= 2: h = 1) */
+		const uint32_t nb = row_blocks(gx, S), h = rows_per_bucket(gx, S);
+		const uint32_t nby = (gy + h - 1) / h;
 		const uint32_t local = b - bb;
 		const uint32_t y = local / nb, j = local - y * nb;
 		const uint32_t x0 = j << S, x1 = min(x0 + W, gx);
-		const uint32_t row0 = cb + y * gx;
-		const uint32_t cs = row0 + x0, ce = row0 + x1;
-		/* buckets the agents come from: rows above and below, the block's
-		 * own, and (nb >= 2) those of the cells left and right of it */
+		const uint32_t row0 = cb + y * h * gx;
+		const uint32_t cs = row0 + x0;
+		const uint32_t ce = h == 1 ? row0 + x1 : cb + min(y * h + h, gy) * gx;
+		/* buckets the agents come from: the bucket rows above and below,
+		 * the block's own, and (nb >= 2) those of the cells left and right
+		 * of it */
 		const uint32_t xl = x0 ? x0 - 1 : gx - 1;
 		const uint32_t xr = x1 < gx ? x1 : 0;
 		uint32_t need[6];
 		need[0] = b;
-		need[1] = bb + ((y + gy - 1) % gy) * nb + j;
+		need[1] = bb + ((y + nby - 1) % nby) * nb + j;
 		need[2] = b;
-		need[3] = bb + ((y + 1) % gy) * nb + j;
+		need[3] = bb + ((y + 1) % nby) * nb + j;
 		need[4] = bb + y * nb + (xl >> S);
 		need[5] = bb + y * nb + (xr >> S);
 		/* [start, end) of the survivors bound for bucket b (need[0], new
@@ -1813,7 +1847,7 @@ __global__ __launch_bounds__(SCAN_WG) void k_act_scan(
 	ctl->tick = tick + 1;
 }
 
-constexpr uint32_t BWIN = 512;   /* buckets per LDS count window */
+constexpr uint32_t BWIN = 128;   /* buckets per LDS count window */
 
 /* Writes the staged outputs [0, total) of an action tile (batch positions
  * goff + i) to the SoA state, moves each for the next iteration, and counts
@@ -1824,7 +1858,8 @@ constexpr uint32_t BWIN = 512;   /* buckets per LDS count window */
  * group); true: each looks up its own. Returns the survivors of the move. */
 template <bool MIXED>
 __device__ __forceinline__ unsigned long long emit_outputs(
-	const uint32_t *s_oc, const uint32_t *s_ov, uint32_t *s_win, uint32_t w0,
+	const uint32_t *s_oc, const uint32_t *s_ov, uint32_t *s_win,
+	uint32_t *s_l1, uint32_t w0, uint32_t u0,
 	uint32_t total, uint32_t goff, uint32_t *__restrict__ ocell,
 	int32_t *__restrict__ oenergy, uint8_t *__restrict__ otype,
 	uint32_t *__restrict__ mkeys, uint32_t *__restrict__ mvals,
@@ -1849,14 +1884,16 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 			if (m.active)
 				mk = move_agent(id - m.base, x, y, (int32_t)en, t, m, S, &mv, &nbk,
 					overflow);
-			ob = m.bbase + y * m.nb + (x >> S);
+			ob = bucket_of(m.bbase, m.nb, m.h, x, y, S);
 			mkeys[id] = mk;
 			mvals[id] = mv;
 			live += mk != DEAD_KEY;
 		}
 		if (P.bucket_on) {
-			wave_bucket_add<BWIN>(s_win, ocnt, P.nb_cap, P.cnt_top, w0, ob);
-			wave_bucket_add<BWIN>(s_win + BWIN, ncnt, P.nb_cap, P.cnt_top, w0, nbk);
+			wave_bucket_add<BWIN>(s_win, s_l1, ocnt, P.nb_cap, P.cnt_top, w0, u0,
+				ob);
+			wave_bucket_add<BWIN>(s_win + BWIN, s_l1 + L1W, ncnt, P.nb_cap,
+				P.cnt_top, w0, u0, nbk);
 		}
 	}
 	return live;
@@ -1877,7 +1914,8 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	uint32_t *__restrict__ mkeys, uint32_t *__restrict__ mvals,
 	uint32_t *ocnt, uint32_t *ncnt, Ctl *ctl, Tables T, Params P) {
 	__shared__ uint32_t s_oc[A_STAGE];
-	__shared__ uint32_t s_up[2][BWIN / 64 + 1];
+	__shared__ uint32_t s_l1[2 * L1W];
+	__shared__ uint32_t s_l2[2 * L2W];
 	__shared__ uint32_t s_ov[A_STAGE];
 	__shared__ uint32_t s_win[2 * BWIN];
 	__shared__ uint32_t s_tmp[NWAVE];
@@ -1897,7 +1935,8 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	if (nrep == 1 && ntiles > blockIdx.x) m_one = load_rm(T, 0, tick, S);
 
 	for (uint32_t i = tid; i < 2 * BWIN; i += WG) s_win[i] = 0;
-	if (tid < 2 * (BWIN / 64 + 1)) (&s_up[0][0])[tid] = 0;
+	for (uint32_t i = tid; i < 2 * L1W; i += WG) s_l1[i] = 0;
+	if (tid < 2 * L2W) s_l2[tid] = 0;
 	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 		const uint32_t base = tile * A_TILE;
 		const uint32_t goff = act_off[tile];
@@ -1956,56 +1995,52 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		{
 			const uint32_t l = c_first - m_t.cb;
 			const uint32_t y = l / m_t.gx, x = l - y * m_t.gx;
-			const uint32_t b0 = m_t.bbase + y * m_t.nb + (x >> S);
+			const uint32_t b0 = bucket_of(m_t.bbase, m_t.nb, m_t.h, x, y, S);
 			w0 = b0 > BWIN / 4 ? b0 - BWIN / 4 : 0u;
 		}
+		const uint32_t u0 = (w0 >> 6) > L1W / 2 ? (w0 >> 6) - L1W / 2 : 0u;
 		if (!mixed)
-			live += emit_outputs<false>(s_oc, s_ov, s_win, w0, total, goff, ocell,
+			live += emit_outputs<false>(s_oc, s_ov, s_win, s_l1, w0, u0, total, goff, ocell,
 				oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
 				overflow);
 		else
-			live += emit_outputs<true>(s_oc, s_ov, s_win, w0, total, goff, ocell,
+			live += emit_outputs<true>(s_oc, s_ov, s_win, s_l1, w0, u0, total, goff, ocell,
 				oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
 				overflow);
 		__syncthreads();
 		if (P.bucket_on) {
-			/* level 0 of the window, summed into its level-1 entries */
+			/* flush the windows: level 0; level 1, summed into the level-2
+			 * window; levels 2 and 3 */
 			for (uint32_t i = tid; i < 2 * BWIN; i += WG) {
 				const uint32_t h = s_win[i];
 				if (h) {
-					const uint32_t k = i / BWIN, q = w0 + (i % BWIN);
-					atomicAdd(&(k ? ncnt : ocnt)[q], h);
-					atomicAdd(&s_up[k][(q >> 6) - (w0 >> 6)], h);
+					atomicAdd(&(i < BWIN ? ocnt : ncnt)[w0 + (i % BWIN)], h);
 					s_win[i] = 0;
 				}
 			}
-			__syncthreads();
-			if (P.cnt_top >= 1 && tid < 2) {
-				/* levels 1 .. cnt_top: the window's few level-1 entries, and
-				 * their sums for each higher entry they fall in */
-				uint32_t *cnt = tid ? ncnt : ocnt;
-				uint32_t run[CNT_LEVELS] = {0, 0, 0, 0}, at[CNT_LEVELS] = {0, 0, 0, 0};
-				for (uint32_t e = 0; e <= BWIN / 64; e++) {
-					const uint32_t h = s_up[tid][e];
-					if (!h) continue;
-					s_up[tid][e] = 0;
-					const uint32_t q1 = (w0 >> 6) + e;
-					atomicAdd(&cnt[cnt_off(P.nb_cap, 1) + q1], h);
-#pragma unroll
-					for (int k = 2; k < CNT_LEVELS; k++) {
-						if ((uint32_t)k > P.cnt_top) break;
-						const uint32_t q = q1 >> (6 * (k - 1));
-						if (run[k] && q != at[k]) {
-							atomicAdd(&cnt[cnt_off(P.nb_cap, k) + at[k]], run[k]);
-							run[k] = 0;
-						}
-						at[k] = q;
-						run[k] += h;
+			if (P.cnt_top >= 1) {
+				for (uint32_t i = tid; i < 2 * L1W; i += WG) {
+					const uint32_t h = s_l1[i];
+					if (h) {
+						const uint32_t k = i / L1W, q1 = u0 + (i % L1W);
+						atomicAdd(&(k ? ncnt : ocnt)[cnt_off(P.nb_cap, 1) + q1], h);
+						if (P.cnt_top >= 2)
+							atomicAdd(&s_l2[k * L2W + (q1 >> 6) - (u0 >> 6)], h);
+						s_l1[i] = 0;
 					}
 				}
-#pragma unroll
-				for (int k = 2; k < CNT_LEVELS; k++)
-					if (run[k]) atomicAdd(&cnt[cnt_off(P.nb_cap, k) + at[k]], run[k]);
+				__syncthreads();
+				if (P.cnt_top >= 2 && tid < 2 * L2W) {
+					const uint32_t h = s_l2[tid];
+					if (h) {
+						const uint32_t k = tid / L2W, q2 = (u0 >> 6) + tid % L2W;
+						uint32_t *cnt = k ? ncnt : ocnt;
+						atomicAdd(&cnt[cnt_off(P.nb_cap, 2) + q2], h);
+						if (P.cnt_top >= 3)
+							atomicAdd(&cnt[cnt_off(P.nb_cap, 3) + (q2 >> 6)], h);
+						s_l2[tid] = 0;
+					}
+				}
 			}
 			__syncthreads();
 		}
@@ -2517,17 +2552,18 @@ static int upload_table(pp_batch_t *b) {
 	for (uint32_t i = 0; i < nrep; i++) {
 		bbase[i] = (uint32_t)std::min<uint64_t>(nbk, 0xFFFFFFFFull);
 		const Rep &rp = b->rep_host[order[i]];
-		const uint64_t W = 1ull << shift;
-		nbk += (uint64_t)rp.gy * ((rp.gx + W - 1) / W);
+		const uint32_t h = rows_per_bucket(rp.gx, shift);
+		nbk += (uint64_t)((rp.gy + h - 1) / h) * row_blocks(rp.gx, shift);
 	}
 	bbase[nrep] = (uint32_t)std::min<uint64_t>(nbk, 0xFFFFFFFFull);
 	HIP_OK(hipMemcpy(b->bbase_dev, bbase.data(), (nrep + 1) * 4,
 		hipMemcpyHostToDevice));
-	/* Row buckets when chosen, or, automatically, while few enough that
-	 * their per-bucket latency stays below the radix sort's passes. */
+	/* Row buckets when chosen, or automatically: always for one replica,
+	 * for several while few enough that their per-bucket latency stays
+	 * below the radix sort's passes. */
 	const uint32_t on = (b->sort_mode != 0 && nbk <= b->P.nb_cap &&
 		nbk <= CNT_MAX_BUCKETS &&
-		(b->sort_mode != 1 || nbk <= BUCKET_AUTO_MAX)) ? 1u : 0u;
+		(b->sort_mode != 1 || nrep <= 1 || nbk <= BUCKET_AUTO_MAX)) ? 1u : 0u;
 	uint32_t top = 0;
 	while (top + 1 < (uint32_t)CNT_LEVELS && (nbk >> (6 * (top + 1))) > 0) top++;
 	if (on != b->P.bucket_on || shift != b->P.bshift || top != b->P.cnt_top) {
