@@ -381,14 +381,15 @@ __device__ __forceinline__ uint32_t move_agent(uint32_t id, uint32_t x,
 	return m.cb + ny * m.gx + nx;
 }
 
-/* Bucket count tables: four levels of 64-ary sums, so that any prefix sum
- * is at most 4 x 63 reads that one wave does at once. Level k holds, for
+/* Bucket count tables: three levels of 64-ary sums, so that any prefix sum
+ * is at most 3 x 63 reads that one wave does at once. Level k holds, for
  * every q, the count of buckets [64^k q, 64^k (q + 1)); it has
  * (nb_cap >> 6k) + 1 entries and starts at cnt_off(nb_cap, k). Layouts of
- * up to 2^24 buckets fit. Only levels 0 .. cnt_top are kept, cnt_top the
- * highest with an entry above 0 for the layout (NB > 64^cnt_top). */
-constexpr int CNT_LEVELS = 4;
-constexpr uint32_t CNT_MAX_BUCKETS = 1u << 24;
+ * up to 2^18 buckets fit (more sort with the radix sort). Only levels
+ * 0 .. cnt_top are kept, cnt_top the highest with an entry above 0 for the
+ * layout (NB > 64^cnt_top). */
+constexpr int CNT_LEVELS = 3;
+constexpr uint32_t CNT_MAX_BUCKETS = 1u << 18;
 
 __host__ __device__ __forceinline__ uint32_t cnt_off(uint32_t nbc, int lvl) {
 	uint32_t o = 0;
@@ -396,22 +397,18 @@ __host__ __device__ __forceinline__ uint32_t cnt_off(uint32_t nbc, int lvl) {
 	return o;
 }
 
-/* Sum of the counts of buckets [0, q) (one wave; every lane gets it), from
- * levels 0 .. top (q < 64^(top + 1)). */
-__device__ __forceinline__ uint32_t cnt_prefix(const uint32_t *cnt,
-	uint32_t nbc, uint32_t top, uint32_t q) {
-	const uint32_t lane = threadIdx.x & (WAVE - 1);
-	uint32_t v = 0;
-#pragma unroll
-	for (int k = 0; k < CNT_LEVELS; k++) {
-		if ((uint32_t)k > top) break;
-		const uint32_t sh = 6 * k;
-		const uint32_t base = (q >> (sh + 6)) << 6, lim = (q >> sh) & 63u;
-		if (lane < lim) v += cnt[cnt_off(nbc, k) + base + lane];
-	}
-#pragma unroll
-	for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
-	return v;
+/* After the levels of the old-cell table: per bucket, the moved agents on
+ * the first cell of its block and on the last (row blocks of grids wider
+ * than a block), where the agents of the neighbouring blocks' edge cells
+ * start and end. */
+__host__ __device__ __forceinline__ uint32_t edge_first_off(uint32_t nbc) {
+	return cnt_off(nbc, CNT_LEVELS);
+}
+__host__ __device__ __forceinline__ uint32_t edge_last_off(uint32_t nbc) {
+	return cnt_off(nbc, CNT_LEVELS) + nbc + 1;
+}
+__host__ __device__ __forceinline__ uint32_t cnt_words(uint32_t nbc) {
+	return cnt_off(nbc, CNT_LEVELS) + 2 * (nbc + 1);
 }
 
 /* LDS windows of the emit's bucket counts: WIN level-0 entries from w0 and
@@ -894,42 +891,6 @@ __global__ __launch_bounds__(WG) void k_bscan_clear(uint32_t *ocnt,
 	}
 }
 
-/* First moved position of cell `c` at the end of the moved range [s, e)
- * whose cells ascend to c (one wave; 64 positions per probe). */
-__device__ __forceinline__ uint32_t tail_of_cell(const uint32_t *ocell,
-	uint32_t s, uint32_t e, uint32_t c) {
-	const int lane = threadIdx.x & (WAVE - 1);
-	uint32_t p = e;
-	for (;;) {
-		const uint32_t q0 = p > s + WAVE ? p - WAVE : s;
-		const uint32_t pos = q0 + (uint32_t)lane;
-		const bool in = pos < p && ocell[pos] == c;
-		const uint32_t n = (uint32_t)__popcll(__ballot(in));
-		const uint32_t win = p - q0;
-		p -= n;
-		if (n != win || q0 <= s) break;
-	}
-	return p;
-}
-
-/* End of the run of cell `c` at the start of the moved range [s, e) whose
- * cells ascend from c (one wave). */
-__device__ __forceinline__ uint32_t head_of_cell(const uint32_t *ocell,
-	uint32_t s, uint32_t e, uint32_t c) {
-	const int lane = threadIdx.x & (WAVE - 1);
-	uint32_t p = s;
-	for (;;) {
-		const uint32_t q1 = min(p + WAVE, e);
-		const uint32_t pos = p + (uint32_t)lane;
-		const bool in = pos < q1 && ocell[pos] == c;
-		const uint32_t n = (uint32_t)__popcll(__ballot(in));
-		const uint32_t win = q1 - p;
-		p += n;
-		if (n != win || q1 >= e) break;
-	}
-	return p;
-}
-
 /* Stable rank of digit d (0 .. nd; nd = no element) among the lanes of the
  * wave: peers with the same digit, and this lane's rank among them. */
 __device__ __forceinline__ void wave_match(uint32_t d, uint32_t bits,
@@ -948,8 +909,7 @@ __device__ __forceinline__ void wave_match(uint32_t d, uint32_t bits,
 template <typename GT>
 __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
-	const uint32_t *__restrict__ ocell, const uint32_t *__restrict__ ocnt,
-	const uint32_t *__restrict__ ncnt,
+	const uint32_t *__restrict__ ocnt, const uint32_t *__restrict__ ncnt,
 	uint32_t *__restrict__ okeys, uint32_t *__restrict__ ovals, Ctl *ctl,
 	Tables T, Params P, uint32_t *__restrict__ act_cnt, uint32_t act_words,
 	GT *grass_at) {
@@ -959,7 +919,7 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	__shared__ uint16_t s_ix[BCAP];        /* gather index of each sorted slot */
 	__shared__ uint32_t s_cnt2[W_MAX / 2]; /* per cell, 16 bits: count, start, end */
 	uint32_t *s_ka = s_buf, *s_va = s_buf + BCAP;
-	__shared__ uint32_t s_need[12];
+	__shared__ uint32_t s_need[2][14];
 	__shared__ uint32_t s_wc[2][BG_IPT * NWAVE];
 	__shared__ uint32_t s_rng[10];
 	__shared__ uint32_t s_nrng;
@@ -993,10 +953,20 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 		gy1 = rp.gy;
 		cb1 = rp.cb;
 	}
-
-	for (uint32_t b = blockIdx.x; b < NB; b += gridDim.x) {
-		/* the bucket's replica (largest k with bbase[k] <= b), row and block */
-		uint32_t gx = gx1, gy = gy1, cb = cb1, bb = 0;
+	/* Geometry of bucket b: its replica (largest k with bbase[k] <= b), grid
+	 * rows [y h, y h + h), block j (nb >= 2: h = 1), and the buckets its
+	 * agents come from: the bucket rows above and below, its own, and
+	 * (nb >= 2) those of the cells left and right of the block. */
+	struct Geo {
+		uint32_t gx, gy, cb, nb, h, y, j;
+		uint32_t need[6];
+	};
+	auto geo = [&](uint32_t b) {
+		Geo g;
+		g.gx = gx1;
+		g.gy = gy1;
+		g.cb = cb1;
+		uint32_t bb = 0;
 		if (nrep != 1) {
 			uint32_t lo = 0, hi = nrep;
 			while (hi - lo > 1) {
@@ -1005,79 +975,126 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 				else hi = mid;
 			}
 			const Rep &rp = T.rep[T.order[lo]];
-			gx = rp.gx;
-			gy = rp.gy;
-			cb = rp.cb;
+			g.gx = rp.gx;
+			g.gy = rp.gy;
+			g.cb = rp.cb;
 			bb = T.bbase[lo];
 		}
-		/* rows [y * h, y * h + h) of the grid, block j (nb >= 2: h = 1) */
-		const uint32_t nb = row_blocks(gx, S), h = rows_per_bucket(gx, S);
-		const uint32_t nby = (gy + h - 1) / h;
+		g.nb = row_blocks(g.gx, S);
+		g.h = rows_per_bucket(g.gx, S);
+		const uint32_t nby = (g.gy + g.h - 1) / g.h;
 		const uint32_t local = b - bb;
-		const uint32_t y = local / nb, j = local - y * nb;
-		const uint32_t x0 = j << S, x1 = min(x0 + W, gx);
-		const uint32_t row0 = cb + y * h * gx;
-		const uint32_t cs = row0 + x0;
-		const uint32_t ce = h == 1 ? row0 + x1 : cb + min(y * h + h, gy) * gx;
-		/* buckets the agents come from: the bucket rows above and below,
-		 * the block's own, and (nb >= 2) those of the cells left and right
-		 * of it */
-		const uint32_t xl = x0 ? x0 - 1 : gx - 1;
-		const uint32_t xr = x1 < gx ? x1 : 0;
-		uint32_t need[6];
-		need[0] = b;
-		need[1] = bb + ((y + nby - 1) % nby) * nb + j;
-		need[2] = b;
-		need[3] = bb + ((y + 1) % nby) * nb + j;
-		need[4] = bb + y * nb + (xl >> S);
-		need[5] = bb + y * nb + (xr >> S);
-		/* [start, end) of the survivors bound for bucket b (need[0], new
-		 * cells) and of the moved agents of need[1..5] (old cells): prefix
-		 * sums of the emit's counts, two needs per wave */
-#pragma unroll 1
-		for (int s = w; s < 6; s += NWAVE) {
-			const uint32_t q = need[s];
+		g.y = local / g.nb;
+		g.j = local - g.y * g.nb;
+		const uint32_t x0 = g.j << S, x1 = min(x0 + W, g.gx);
+		const uint32_t xl = x0 ? x0 - 1 : g.gx - 1;
+		const uint32_t xr = x1 < g.gx ? x1 : 0;
+		g.need[0] = b;
+		g.need[1] = bb + ((g.y + nby - 1) % nby) * g.nb + g.j;
+		g.need[2] = b;
+		g.need[3] = bb + ((g.y + 1) % nby) * g.nb + g.j;
+		g.need[4] = bb + g.y * g.nb + (xl >> S);
+		g.need[5] = bb + g.y * g.nb + (xr >> S);
+		return g;
+	};
+	/* [start, end) of the survivors bound for bucket need[0] (new cells)
+	 * and of the moved agents of need[1..5] (old cells), from prefix sums of
+	 * the emit's counts; wave w takes needs w and w + 4. Issued in one step
+	 * (loads) and published to LDS in another, so that a bucket's loads
+	 * overlap the previous bucket's sort. Edge counts: agents on the last
+	 * cell of need[4] (left of the block) and on the first of need[5]. */
+	uint32_t pl[2][CNT_LEVELS], pc[2], pe[2];
+	auto issue = [&](const Geo &g) {
+#pragma unroll
+		for (int i = 0; i < 2; i++) {
+			const int s = w + NWAVE * i;
+			pc[i] = pe[i] = 0;
+#pragma unroll
+			for (int k = 0; k < CNT_LEVELS; k++) pl[i][k] = 0;
+			if (s >= 6) continue;
+			/* (constant indices: the array stays in registers) */
+			const uint32_t q = s == 0 ? g.need[0] : s == 1 ? g.need[1]
+				: s == 2 ? g.need[2] : s == 3 ? g.need[3] : s == 4 ? g.need[4]
+				: g.need[5];
 			const uint32_t *cnt = s == 0 ? ncnt : ocnt;
-			const uint32_t c = cnt[q];
-			const uint32_t pre = cnt_prefix(cnt, P.nb_cap, P.cnt_top, q);
-			if (lane == 0) {
-				s_need[2 * s] = pre;
-				s_need[2 * s + 1] = pre + c;
+			pc[i] = cnt[q];
+			if (s == 4) pe[i] = ocnt[edge_last_off(P.nb_cap) + q];
+			if (s == 5) pe[i] = ocnt[edge_first_off(P.nb_cap) + q];
+#pragma unroll
+			for (int k = 0; k < CNT_LEVELS; k++) {
+				if ((uint32_t)k > P.cnt_top) break;
+				const uint32_t sh = 6 * k;
+				const uint32_t base = (q >> (sh + 6)) << 6, lim = (q >> sh) & 63u;
+				if ((uint32_t)lane < lim) pl[i][k] = cnt[cnt_off(P.nb_cap, k) + base + lane];
 			}
 		}
-		__syncthreads();
-		const uint32_t out0 = s_need[0], expect = s_need[1] - out0;
-		/* source ranges of the moved order, sorted, without repeats */
-		if (w == 0) {
+	};
+	auto publish = [&](uint32_t *dst) {
+#pragma unroll
+		for (int i = 0; i < 2; i++) {
+			const int s = w + NWAVE * i;
+			if (s >= 6) continue;
+			uint32_t v = 0;
+#pragma unroll
+			for (int k = 0; k < CNT_LEVELS; k++) v += pl[i][k];
+#pragma unroll
+			for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+			if (lane == 0) {
+				dst[2 * s] = v;
+				dst[2 * s + 1] = v + pc[i];
+				if (s >= 4) dst[8 + s] = pe[i];
+			}
+		}
+	};
+
+	uint32_t b = blockIdx.x;
+	if (b < NB) {
+		issue(geo(b));
+		publish(s_need[0]);
+	}
+	__syncthreads();
+	for (uint32_t it = 0; b < NB; b += gridDim.x, it++) {
+		const uint32_t *nd = s_need[it & 1];
+		const Geo g = geo(b);
+		const uint32_t x0 = g.j << S, x1 = min(x0 + W, g.gx);
+		const uint32_t row0 = g.cb + g.y * g.h * g.gx;
+		const uint32_t cs = row0 + x0;
+		const uint32_t ce = g.h == 1 ? row0 + x1 : g.cb + min(g.y * g.h + g.h, g.gy) * g.gx;
+		const uint32_t out0 = nd[0], expect = nd[1] - out0;
+		/* source ranges of the moved order in position order (a sorting
+		 * network on the starts), without empty ones or repeats */
+		if (tid == 0) {
 			uint32_t rs[5], re[5];
 #pragma unroll
 			for (int s = 1; s < 6; s++) {
-				rs[s - 1] = s_need[2 * s];
-				re[s - 1] = s_need[2 * s + 1];
+				rs[s - 1] = nd[2 * s];
+				re[s - 1] = nd[2 * s + 1];
 			}
-			uint32_t n = 3;
-			if (nb >= 2) {
-				rs[3] = tail_of_cell(ocell, rs[3], re[3], row0 + xl);
-				re[4] = head_of_cell(ocell, rs[4], re[4], row0 + xr);
-				n = 5;
+			if (g.nb >= 2) {
+				rs[3] = re[3] - nd[12];
+				re[4] = rs[4] + nd[13];
+			} else {
+				rs[3] = re[3] = rs[4] = re[4] = 0xFFFFFFFFu;
 			}
-			if (lane == 0) {
-				for (uint32_t a = 1; a < n; a++)
-					for (uint32_t c = a; c > 0 && rs[c] < rs[c - 1]; c--) {
-						const uint32_t ts = rs[c], te = re[c];
-						rs[c] = rs[c - 1]; re[c] = re[c - 1];
-						rs[c - 1] = ts; re[c - 1] = te;
-					}
-				uint32_t m = 0;
-				for (uint32_t a = 0; a < n; a++) {
-					if (rs[a] >= re[a]) continue;
-					if (m && rs[a] == s_rng[2 * (m - 1)]) continue;
-					s_rng[2 * m] = rs[a];
-					s_rng[2 * m + 1] = re[a];
-					m++;
+			auto ce2 = [&](int a, int c) {
+				if (rs[c] < rs[a]) {
+					const uint32_t ts = rs[a], te = re[a];
+					rs[a] = rs[c]; re[a] = re[c];
+					rs[c] = ts; re[c] = te;
 				}
-				s_nrng = m;
+			};
+			ce2(0, 3); ce2(1, 4); ce2(0, 2); ce2(1, 3); ce2(0, 1);
+			ce2(2, 4); ce2(1, 2); ce2(3, 4); ce2(2, 3);
+			uint32_t m = 0, last = 0xFFFFFFFFu;
+#pragma unroll
+			for (int a = 0; a < 5; a++) {
+				if (rs[a] >= re[a] || rs[a] == last) continue;
+				s_rng[2 * m] = rs[a];
+				s_rng[2 * m + 1] = re[a];
+				last = rs[a];
+				m++;
 			}
+			s_nrng = m;
 		}
 		for (uint32_t i = tid; i < W / 2; i += WG) s_cnt2[i] = 0;
 		__syncthreads();
@@ -1146,9 +1163,11 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 		if (cnt != expect) {
 			/* the counts do not describe the agents */
 			if (tid == 0) set_error(ctl, PP_LIBRARY_ERROR, tick, -2 - (int32_t)b);
-			__syncthreads();
-			continue;
+			break;
 		}
+		/* the next bucket's loads, in flight during this one's sort */
+		const uint32_t bn = b + gridDim.x;
+		if (bn < NB) issue(geo(bn));
 		if (cnt <= BCAP) {
 			/* counting sort by cell offset in LDS: counts, exclusive scan,
 			 * scatter of the gather indices, then each cell's few slots put
@@ -1203,6 +1222,7 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 				}
 			}
 			__syncthreads();
+			if (bn < NB) publish(s_need[(it + 1) & 1]);
 			for (uint32_t q = tid; q < cnt; q += WG) {
 				const uint32_t i = s_ix[q];
 				okeys[out0 + q] = cs + s_ka[i];
@@ -1263,6 +1283,7 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 					}
 				}
 			}
+			if (bn < NB) publish(s_need[(it + 1) & 1]);
 		}
 		__syncthreads();
 	}
@@ -1621,6 +1642,10 @@ __global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
 				ncnt[o + i] = 0;
 			}
 		}
+		for (uint32_t i = blockIdx.x * WG + tid; i <= NB; i += gridDim.x * WG) {
+			ocnt[edge_first_off(P.nb_cap) + i] = 0;
+			ocnt[edge_last_off(P.nb_cap) + i] = 0;
+		}
 	}
 	const uint32_t L = ctl->live[tick & 1];
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
@@ -1885,6 +1910,12 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 				mk = move_agent(id - m.base, x, y, (int32_t)en, t, m, S, &mv, &nbk,
 					overflow);
 			ob = bucket_of(m.bbase, m.nb, m.h, x, y, S);
+			if (P.bucket_on && m.nb >= 2) {
+				const uint32_t xb = x & ((1u << S) - 1u);
+				if (xb == 0) atomicAdd(&ocnt[edge_first_off(P.nb_cap) + ob], 1u);
+				if (x + 1 == min(((x >> S) + 1) << S, m.gx))
+					atomicAdd(&ocnt[edge_last_off(P.nb_cap) + ob], 1u);
+			}
 			mkeys[id] = mk;
 			mvals[id] = mv;
 			live += mk != DEAD_KEY;
@@ -2036,8 +2067,6 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 						const uint32_t k = tid / L2W, q2 = (u0 >> 6) + tid % L2W;
 						uint32_t *cnt = k ? ncnt : ocnt;
 						atomicAdd(&cnt[cnt_off(P.nb_cap, 2) + q2], h);
-						if (P.cnt_top >= 3)
-							atomicAdd(&cnt[cnt_off(P.nb_cap, 3) + (q2 >> 6)], h);
 						s_l2[tid] = 0;
 					}
 				}
@@ -2373,7 +2402,7 @@ static void launch_act(pp_batch_t *b, int sorted) {
 template <typename GT>
 static void launch_bucket_sort(pp_batch_t *b) {
 	k_bucket_sort<GT><<<b->grid_bucket, WG, 0, b->stream>>>(b->keys[0],
-		b->vals[0], b->cell, b->ocnt, b->ncnt, b->keys[1], b->vals[1],
+		b->vals[0], b->ocnt, b->ncnt, b->keys[1], b->vals[1],
 		b->ctl, b->tables(), b->P, b->act_cnt, b->act_words, (GT *)b->grass_at);
 }
 
@@ -2388,7 +2417,7 @@ static int enqueue_step(pp_batch_t *b, int radix) {
 		/* counts of a previous layout are void */
 		prof_begin(b, KI_BUCKET_SCAN, pe);
 		k_bscan_clear<<<b->grid_bucket, WG, 0, b->stream>>>(b->ocnt, b->ncnt,
-			cnt_off(b->P.nb_cap, CNT_LEVELS), b->ctl);
+			cnt_words(b->P.nb_cap), b->ctl);
 		prof_end(b, pe);
 	}
 	if (radix) {
@@ -2759,7 +2788,7 @@ static uint64_t batch_bytes(uint64_t A, uint64_t C, uint32_t max_rep,
 	uint32_t R, uint32_t stats_rows, uint64_t hist_words) {
 	const uint64_t A4 = (A + 3) & ~3ull;
 	uint64_t t = A4 * (4 + 4 + 1) + 6 * A4 * 4;
-	t += 8ull * cnt_off(nb_cap_for(C), CNT_LEVELS) + 4ull * (max_rep + 1);
+	t += 8ull * cnt_words(nb_cap_for(C)) + 4ull * (max_rep + 1);
 	t += C * grass_bytes_for(R);
 	t += (uint64_t)max_rep * (R + 1) * 8;
 	t += hist_words * 4 + RADIX_MAX * 4;
@@ -2800,7 +2829,7 @@ static int batch_alloc(pp_batch_t *b) {
 	b->act_words = ceil_div(A, A_TILE) + 1;
 	HIP_OK(hipMalloc(&b->act_cnt, (size_t)b->act_words * 4));
 	HIP_OK(hipMalloc(&b->mixed_list, (size_t)b->act_words * 4));
-	const size_t nbw = cnt_off(b->P.nb_cap, CNT_LEVELS);
+	const size_t nbw = cnt_words(b->P.nb_cap);
 	HIP_OK(hipMalloc(&b->ocnt, nbw * 4));
 	HIP_OK(hipMalloc(&b->ncnt, nbw * 4));
 	HIP_OK(hipMemset(b->ocnt, 0, nbw * 4));
