@@ -91,6 +91,7 @@ constexpr int A_IPT = 4;            /* sorted positions per thread (action) */
 constexpr int A_TILE = WG * A_IPT;
 constexpr int A_STAGE = 2 * A_TILE; /* staged outputs per action tile */
 constexpr int MIX_SLOTS = 16;       /* replicas per action tile summed in LDS */
+constexpr uint32_t K_REG = 5;       /* agents of a cell acted on in registers */
 constexpr int SCAN_WG = 1024;       /* threads of the one-group scans */
 constexpr int SCAN_UNROLL = 8;
 /* Grass regrow ticks are stored modulo 2^(8 sizeof(GT)); alive cells are
@@ -1500,76 +1501,162 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 		const RA ra = MIXED ? load_ra(T, kk, tick) : ra_u;
 		const uint32_t end = seg_end(S.key, keys, lp, cnt, base, L, c);
 		const uint32_t k = end - lp;
-		if (k > 1) {
-			uint4 r = make_uint4(0, 0, 0, 0);
-			for (uint32_t i = k - 1, m = 0; i > 0; i--, m++) {
-				if ((m & 3u) == 0)
-					r = pp_philox(c - ra.cb, ra.step, PP_STREAM_SHUFFLE, m >> 2,
-						ra.key0, ra.key1);
-				const uint32_t word = (m & 3u) == 0 ? r.x
-					: (m & 3u) == 1 ? r.y : (m & 3u) == 2 ? r.z : r.w;
-				const uint32_t jj = pp_bounded(word, i + 1);
-				if (jj != i) {
-					const uint32_t a = sv.get(lp + i);
-					sv.set(lp + i, sv.get(lp + jj));
-					sv.set(lp + jj, a);
-				}
+		unsigned long long sc[NACC] = {0, 0, 0, 0, 0, 0, 0};
+		sc[AC_OCC] = 1;
+		if (k <= K_REG) {
+			/* the cell's values in registers (one Philox block covers the
+			 * k - 1 <= 4 draws): the same steps as below, without a chain
+			 * of dependent LDS accesses */
+			uint32_t V[K_REG];
+#pragma unroll
+			for (uint32_t q = 0; q < K_REG; q++) V[q] = q < k ? sv.get(lp + q) : 0u;
+			const uint4 r = pp_philox(c - ra.cb, ra.step, PP_STREAM_SHUFFLE, 0,
+				ra.key0, ra.key1);
+#pragma unroll
+			for (int i = K_REG - 1; i >= 1; i--) {
+				if ((uint32_t)i >= k) continue;
+				const uint32_t m = k - 1 - (uint32_t)i;
+				const uint32_t word = m == 0 ? r.x : m == 1 ? r.y : m == 2 ? r.z : r.w;
+				const uint32_t jj = pp_bounded(word, (uint32_t)i + 1);
+				const uint32_t vi = V[i];
+				uint32_t vj = V[0];
+#pragma unroll
+				for (int t = 1; t < K_REG; t++)
+					if ((uint32_t)t == jj) vj = V[t];
+#pragma unroll
+				for (int t = 0; t < K_REG; t++)
+					if ((uint32_t)t == jj) V[t] = vi;
+				V[i] = vj;
 			}
-		}
-		bool grass = grass_alive_at<GT>(gslow, alive_at, Rmax);
-		bool ate = false;
-		for (uint32_t q = lp; q < end; q++) {
-			const uint32_t v = sv.get(q);
-			if (v & V_KILLED) continue;
-			int32_t e = (int32_t)(v & V_ENERGY);
-			uint32_t thr;
-			if (!(v & V_WOLF)) {
-				if (grass) {
-					grass = false;
-					ate = true;
-					e += (int32_t)ra.sheep_gain;
+			bool grass = grass_alive_at<GT>(gslow, alive_at, Rmax);
+			bool ate = false;
+#pragma unroll
+			for (int q = 0; q < K_REG; q++) {
+				if ((uint32_t)q >= k) continue;
+				const uint32_t v = V[q];
+				if (v & V_KILLED) continue;
+				int32_t e = (int32_t)(v & V_ENERGY);
+				uint32_t thr;
+				if (!(v & V_WOLF)) {
+					if (grass) {
+						grass = false;
+						ate = true;
+						e += (int32_t)ra.sheep_gain;
+					}
+					thr = ra.sheep_thr;
+				} else {
+					bool fed = false;
+#pragma unroll
+					for (int o = 0; o < K_REG; o++) {
+						if (!fed && (uint32_t)o < k && !(V[o] & (V_WOLF | V_KILLED))) {
+							V[o] |= V_KILLED;
+							e += (int32_t)ra.wolves_gain;
+							fed = true;
+						}
+					}
+					thr = ra.wolves_thr;
 				}
-				thr = ra.sheep_thr;
-			} else {
-				for (uint32_t o = lp; o < end; o++) {
-					const uint32_t u = sv.get(o);
-					if (!(u & (V_WOLF | V_KILLED))) {
-						sv.set(o, u | V_KILLED);
-						e += (int32_t)ra.wolves_gain;
-						break;
+				uint32_t nv = v & V_WOLF;
+				if (e > (int32_t)thr && (v & V_REPRO)) {
+					nv |= V_CHILD | ((e & 1) ? V_ODD : 0u);
+					e -= e / 2;
+				}
+				overflow |= e > (int32_t)V_ENERGY;
+				V[q] = nv | ((uint32_t)e & V_ENERGY);
+			}
+			if (ate) grass_at[c] = (GT)(tick + 1 + ra.R);
+			sc[AC_EATEN] = ate ? 1 : 0;
+#pragma unroll
+			for (uint32_t q = 0; q < K_REG; q++) {
+				if (q >= k) continue;
+				const uint32_t v = V[q];
+				sv.set(lp + q, v);
+				const uint32_t e = v & V_ENERGY;
+				const uint32_t t = (v & V_WOLF) ? 1u : 0u;
+				const uint32_t outs = ((v & V_KILLED) == 0) + ((v & V_CHILD) != 0);
+				const uint32_t ea = ((v & V_KILLED) ? 0u : e)
+					+ ((v & V_CHILD) ? ((v & V_ODD) ? e - 1 : e) : 0u);
+				if (t) {
+					sc[AC_WOLVES] += outs;
+					sc[AC_WOLVES_EN] += ea;
+				} else {
+					sc[AC_SHEEP] += outs;
+					sc[AC_SHEEP_EN] += ea;
+				}
+				sc[AC_OUT] += outs;
+				if (lp + q < (uint32_t)A_TILE) nout += outs;
+				else if (outs) atomicAdd(&act_cnt[tile + (lp + q) / A_TILE], outs);
+			}
+		} else {
+			if (k > 1) {
+				uint4 r = make_uint4(0, 0, 0, 0);
+				for (uint32_t i = k - 1, m = 0; i > 0; i--, m++) {
+					if ((m & 3u) == 0)
+						r = pp_philox(c - ra.cb, ra.step, PP_STREAM_SHUFFLE, m >> 2,
+							ra.key0, ra.key1);
+					const uint32_t word = (m & 3u) == 0 ? r.x
+						: (m & 3u) == 1 ? r.y : (m & 3u) == 2 ? r.z : r.w;
+					const uint32_t jj = pp_bounded(word, i + 1);
+					if (jj != i) {
+						const uint32_t a = sv.get(lp + i);
+						sv.set(lp + i, sv.get(lp + jj));
+						sv.set(lp + jj, a);
 					}
 				}
-				thr = ra.wolves_thr;
 			}
-			uint32_t nv = v & V_WOLF;
-			if (e > (int32_t)thr && (v & V_REPRO)) {
-				nv |= V_CHILD | ((e & 1) ? V_ODD : 0u);
-				e -= e / 2;
+			bool grass = grass_alive_at<GT>(gslow, alive_at, Rmax);
+			bool ate = false;
+			for (uint32_t q = lp; q < end; q++) {
+				const uint32_t v = sv.get(q);
+				if (v & V_KILLED) continue;
+				int32_t e = (int32_t)(v & V_ENERGY);
+				uint32_t thr;
+				if (!(v & V_WOLF)) {
+					if (grass) {
+						grass = false;
+						ate = true;
+						e += (int32_t)ra.sheep_gain;
+					}
+					thr = ra.sheep_thr;
+				} else {
+					for (uint32_t o = lp; o < end; o++) {
+						const uint32_t u = sv.get(o);
+						if (!(u & (V_WOLF | V_KILLED))) {
+							sv.set(o, u | V_KILLED);
+							e += (int32_t)ra.wolves_gain;
+							break;
+						}
+					}
+					thr = ra.wolves_thr;
+				}
+				uint32_t nv = v & V_WOLF;
+				if (e > (int32_t)thr && (v & V_REPRO)) {
+					nv |= V_CHILD | ((e & 1) ? V_ODD : 0u);
+					e -= e / 2;
+				}
+				overflow |= e > (int32_t)V_ENERGY;
+				sv.set(q, nv | ((uint32_t)e & V_ENERGY));
 			}
-			overflow |= e > (int32_t)V_ENERGY;
-			sv.set(q, nv | ((uint32_t)e & V_ENERGY));
-		}
-		if (ate) grass_at[c] = (GT)(tick + 1 + ra.R);
-		unsigned long long sc[NACC] = {0, 0, 0, 0, 0, 0, 0};
-		sc[AC_EATEN] = ate ? 1 : 0;
-		sc[AC_OCC] = 1;
-		for (uint32_t q = lp; q < end; q++) {
-			const uint32_t v = sv.get(q);
-			const uint32_t e = v & V_ENERGY;
-			const uint32_t t = (v & V_WOLF) ? 1u : 0u;
-			const uint32_t outs = ((v & V_KILLED) == 0) + ((v & V_CHILD) != 0);
-			const uint32_t ea = ((v & V_KILLED) ? 0u : e)
-				+ ((v & V_CHILD) ? ((v & V_ODD) ? e - 1 : e) : 0u);
-			if (t) {
-				sc[AC_WOLVES] += outs;
-				sc[AC_WOLVES_EN] += ea;
-			} else {
-				sc[AC_SHEEP] += outs;
-				sc[AC_SHEEP_EN] += ea;
+			if (ate) grass_at[c] = (GT)(tick + 1 + ra.R);
+			sc[AC_EATEN] = ate ? 1 : 0;
+			for (uint32_t q = lp; q < end; q++) {
+				const uint32_t v = sv.get(q);
+				const uint32_t e = v & V_ENERGY;
+				const uint32_t t = (v & V_WOLF) ? 1u : 0u;
+				const uint32_t outs = ((v & V_KILLED) == 0) + ((v & V_CHILD) != 0);
+				const uint32_t ea = ((v & V_KILLED) ? 0u : e)
+					+ ((v & V_CHILD) ? ((v & V_ODD) ? e - 1 : e) : 0u);
+				if (t) {
+					sc[AC_WOLVES] += outs;
+					sc[AC_WOLVES_EN] += ea;
+				} else {
+					sc[AC_SHEEP] += outs;
+					sc[AC_SHEEP_EN] += ea;
+				}
+				sc[AC_OUT] += outs;
+				if (q < (uint32_t)A_TILE) nout += outs;
+				else if (outs) atomicAdd(&act_cnt[tile + q / A_TILE], outs);
 			}
-			sc[AC_OUT] += outs;
-			if (q < (uint32_t)A_TILE) nout += outs;
-			else if (outs) atomicAdd(&act_cnt[tile + q / A_TILE], outs);
 		}
 		if (!MIXED) {
 #pragma unroll
@@ -1624,7 +1711,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
  * k_act_mixed (segments never span replicas, so the two kernels touch
  * disjoint cells and values). */
 template <typename GT>
-__global__ __launch_bounds__(WG) void k_act(const uint32_t *__restrict__ keys,
+__global__ __launch_bounds__(WG, 5) void k_act(const uint32_t *__restrict__ keys,
 	uint32_t *vals, GT *grass_at, uint32_t *act_cnt, uint32_t *mixed_list,
 	uint32_t *ocnt, uint32_t *ncnt, Ctl *ctl, Tables T, Params P) {
 	__shared__ ActLds<GT> S;
