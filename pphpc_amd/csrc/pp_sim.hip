@@ -871,6 +871,7 @@ __global__ __launch_bounds__(WG) void k_sort_down(
  * radix sort gives. */
 
 constexpr uint32_t BCAP = 1024;  /* agents of one bucket sorted in LDS */
+constexpr uint32_t NXCD = 8;     /* accelerator dies (XCDs) of the GPU */
 constexpr uint32_t W_MAX = 2048; /* cells of a bucket block: 2^S, S <= 11 */
 static_assert(W_MAX <= 2 * BCAP, "a dense bucket's counters reuse the staging");
 constexpr uint32_t BG_IPT = 8;   /* source positions per thread per round */
@@ -1048,13 +1049,21 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 		}
 	};
 
-	uint32_t b = blockIdx.x;
+	/* Work-groups are dispatched to the XCDs round-robin by index: the
+	 * groups of one XCD take a contiguous share of the buckets, so that a
+	 * bucket and the bucket rows above and below it, whose agents it
+	 * gathers, are read through the same L2. */
+	const uint32_t gpx = gridDim.x / NXCD, xcd = blockIdx.x % NXCD;
+	const uint32_t share = (NB + NXCD - 1) / NXCD;
+	const uint32_t b_end = min(NB, (xcd + 1) * share);
+	uint32_t b = blockIdx.x < gpx * NXCD ? xcd * share + blockIdx.x / NXCD : NB;
+	if (b >= b_end) b = NB;
 	if (b < NB) {
 		issue(geo(b));
 		publish(s_need[0]);
 	}
 	__syncthreads();
-	for (uint32_t it = 0; b < NB; b += gridDim.x, it++) {
+	for (uint32_t it = 0; b < NB; b += gpx, it++) {
 		const uint32_t *nd = s_need[it & 1];
 		const Geo g = geo(b);
 		const uint32_t x0 = g.j << S, x1 = min(x0 + W, g.gx);
@@ -1167,7 +1176,7 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 			break;
 		}
 		/* the next bucket's loads, in flight during this one's sort */
-		const uint32_t bn = b + gridDim.x;
+		const uint32_t bn = b + gpx < b_end ? b + gpx : NB;
 		if (bn < NB) issue(geo(bn));
 		if (cnt <= BCAP) {
 			/* counting sort by cell offset in LDS: counts, exclusive scan,
@@ -1287,6 +1296,7 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 			if (bn < NB) publish(s_need[(it + 1) & 1]);
 		}
 		__syncthreads();
+		if (bn >= NB) break;
 	}
 }
 

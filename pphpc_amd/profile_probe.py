@@ -1,13 +1,16 @@
 """Workload driver for hardware-counter profiles of the step kernels.
 
     python3 pphpc_amd/profile_probe.py [--grid 12800] [--steps 6] [--copy-mib 1024]
+                                       [--sort MODE]
 
 Runs, in one process:
   1. one device-to-device copy of --copy-mib MiB through the HIP runtime
      (a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE give the
      scale that turns those counters into bytes);
   2. init + --steps iterations of the dense v2-style workload on a
-     --grid x --grid torus (capacity 4x the initial agents).
+     --grid x --grid torus (capacity 4x the initial agents), with the agent
+     sort of --sort (0 radix every iteration, 1 automatic, 8..11 row
+     buckets; default automatic).
 It prints the exact event sums of the run, so the per-dispatch counters can
 be divided by the algorithmic bytes of each kernel.
 """
@@ -44,6 +47,7 @@ def main():
     ap.add_argument("--grid", type=int, default=12800)
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--sort", type=int, default=1)
     args = ap.parse_args()
     from pphpc_amd import Params, Simulation
     copied = device_copy(args.copy_mib)
@@ -57,6 +61,7 @@ def main():
     n0 = d["init_sheep"] + d["init_wolves"]
     sim = Simulation(Params.from_values(**d), seed=0, capacity=4 * n0,
                      device=0)
+    sim.set_sort(args.sort)
     sim.init()
     sim.run(args.steps, collect=False)
     out = {"copy_bytes": copied, "grid": L, "steps": args.steps,
