@@ -121,6 +121,18 @@ def kernel_bytes(name, c, passes, radix_all=False):
     return None
 
 
+def counted_traffic(kernel, sort):
+    """HBM bytes per dispatch of a kernel at 12800x12800 from the committed
+    rocprofv3 counter runs (profiles/hbm_traffic_v2s12800.md: FETCH_SIZE and
+    WRITE_SIZE, scaled by a 1 GiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_v2s12800.json")
+    try:
+        d = json.load(open(path))
+        return d["per_dispatch"][sort]["k_" + kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def measure_roofline(params, capacity, steps, seed=0, sort_mode=None):
     """Profiled run: per-kernel event timing on the simulation's stream, and
     the unprofiled time per iteration of the same iterations (graphs)."""
@@ -389,10 +401,14 @@ def main():
         log("profiled 12800x12800 runs for the HBM roofline: radix sort")
         roof_big = measure_roofline(big, 4 * nb, args.big_roofline_steps,
                                     args.seed, sort_mode=0)
+        roof_big["traffic"] = counted_traffic(roof_big["kernel"], "radix")
+        roof_big["traffic_source"] = "profiles/hbm_traffic_v2s12800.md"
         log(f"12800^2 radix: dominant kernel {roof_big['kernel']}: "
             f"{roof_big['achieved']} GB/s, {roof_big['ms_per_iteration']} ms")
         roof_big_b = measure_roofline(big, 4 * nb, args.big_roofline_steps,
                                       args.seed)
+        roof_big_b["traffic"] = counted_traffic(roof_big_b["kernel"], "auto")
+        roof_big_b["traffic_source"] = "profiles/hbm_traffic_v2s12800.md"
         log(f"12800^2 auto (buckets): dominant kernel {roof_big_b['kernel']}: "
             f"{roof_big_b['achieved']} GB/s, "
             f"{roof_big_b['ms_per_iteration']} ms")
