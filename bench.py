@@ -23,8 +23,8 @@ Also reported (one JSON line on stdout, progress on stderr):
                 bytes, latency-bound), and its time per iteration
   cpu_baseline  the OpenMP restatement of the step (row bands, as the
                 reference's CPU simulator) on this host's cores, and the
-                single-threaded C oracle, each on the first iterations of the
-                same workload
+                single-threaded C oracle: the OpenMP one over the whole
+                workload, the oracle over its first iterations
   batched       64 replicas of grid sides drawn from {100 .. 3200}, v1 or v2s
                 parameters 50/50, seeds 1..64, 4000 iterations each: one
                 batch (one set of launches per iteration, sort shape from the
@@ -237,11 +237,14 @@ def cpu_baseline(params, capacity, sample_steps, single_steps, threads):
     one = os.path.join(ROOT, "oracle", "build", "pp_oracle")
     if not (os.path.exists(omp) and os.path.exists(one)):
         return None
+    sample_steps = min(sample_steps, params["iters"])
+    span = (f"all {sample_steps} iterations" if sample_steps == params["iters"]
+            else f"first {sample_steps} iterations")
     kv = _run_cpu(omp, params, capacity, sample_steps, ("-t", str(threads)))
     kv1 = _run_cpu(one, params, capacity, single_steps)
     return {"value": float(kv["agent_steps_per_s"]), "unit": "agent-steps/s",
             "cores": int(kv["threads"]), "kind": "port",
-            "sample": f"first {sample_steps} iterations of the same workload "
+            "sample": f"{span} of the same workload "
                       f"(seed 0), OpenMP row-band restatement of the step "
                       f"(oracle/pp_omp.c) on {kv['threads']} threads: "
                       f"{float(kv['seconds']):.1f} s",
@@ -338,7 +341,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--roofline-steps", type=int, default=400)
     ap.add_argument("--big-roofline-steps", type=int, default=16)
-    ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=4000)
     ap.add_argument("--cpu-single-steps", type=int, default=150)
     ap.add_argument("--cpu-threads", type=int,
                     default=min(16, int(os.environ.get("OMP_NUM_THREADS", 0))
