@@ -193,6 +193,47 @@ struct Tables {
 
 /* ---------------------------------------------------------------- helpers */
 
+/* LDS accesses through explicitly LDS-typed pointers. Where a branch picks
+ * between an LDS array and a global one, plain pointers let the compiler
+ * merge both sides into one flat (generic) access, and every flat access
+ * waits for all outstanding global loads and stores (vmcnt(0) and
+ * lgkmcnt(0)), which stalls prefetches and store streams behind it. */
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+typedef __attribute__((address_space(1))) unsigned long long glb_u64;
+
+/* Waits until every global load and store this wave issued has completed
+ * (s_waitcnt vmcnt(0)). Global accesses retire in issue order, so where
+ * loads are issued conditionally, or stores follow them, the compiler can
+ * only wait on a load with vmcnt(0), which then also waits for stores just
+ * issued. One explicit wait at a point where the loads have had time to
+ * arrive, before any store, leaves the compiler no load to wait for
+ * afterwards. */
+__device__ __forceinline__ void wait_global() {
+	__builtin_amdgcn_s_waitcnt(0x0F70);   /* vmcnt(0), expcnt/lgkmcnt free */
+}
+
+__device__ __forceinline__ uint32_t lds_ld(const uint32_t *p) {
+	return *(const lds_u32 *)p;
+}
+__device__ __forceinline__ void lds_st(uint32_t *p, uint32_t v) {
+	*(lds_u32 *)p = v;
+}
+__device__ __forceinline__ void lds_add(uint32_t *p, uint32_t v) {
+	__hip_atomic_fetch_add((lds_u32 *)p, v, __ATOMIC_RELAXED,
+		__HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add(unsigned long long *p,
+	unsigned long long v) {
+	__hip_atomic_fetch_add((lds_u64 *)p, v, __ATOMIC_RELAXED,
+		__HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void glb_add(unsigned long long *p,
+	unsigned long long v) {
+	__hip_atomic_fetch_add((glb_u64 *)p, v, __ATOMIC_RELAXED,
+		__HIP_MEMORY_SCOPE_AGENT);
+}
+
 /* Work-group exclusive scan of one value per thread (NT threads). */
 template <int NT>
 __device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t *s_tmp,
@@ -350,6 +391,28 @@ __device__ __forceinline__ RM load_rm(const Tables &T, uint32_t k,
 	return m;
 }
 
+/* load_rm for a replica index uniform over the wave, its fields moved to
+ * scalar registers (later uses then wait on no vector load). */
+__device__ __forceinline__ RM load_rm_uniform(const Tables &T, uint32_t k,
+	uint32_t tick, uint32_t S) {
+	RM m = load_rm(T, k, tick, S);
+	m.cb = __builtin_amdgcn_readfirstlane(m.cb);
+	m.C = __builtin_amdgcn_readfirstlane(m.C);
+	m.gx = __builtin_amdgcn_readfirstlane(m.gx);
+	m.gy = __builtin_amdgcn_readfirstlane(m.gy);
+	m.sheep_prob = __builtin_amdgcn_readfirstlane(m.sheep_prob);
+	m.wolves_prob = __builtin_amdgcn_readfirstlane(m.wolves_prob);
+	m.key0 = __builtin_amdgcn_readfirstlane(m.key0);
+	m.key1 = __builtin_amdgcn_readfirstlane(m.key1);
+	m.step = __builtin_amdgcn_readfirstlane(m.step);
+	m.base = __builtin_amdgcn_readfirstlane(m.base);
+	m.bbase = __builtin_amdgcn_readfirstlane(m.bbase);
+	m.nb = __builtin_amdgcn_readfirstlane(m.nb);
+	m.h = __builtin_amdgcn_readfirstlane(m.h);
+	m.active = __builtin_amdgcn_readfirstlane((uint32_t)m.active) != 0;
+	return m;
+}
+
 /* One move of a replica's agent with local id `id` at local cell (x, y):
  * Philox block (id, step, MOVE, 0) under the replica's key; direction U[0,5)
  * from word 0 (0 stay, 1 +x, 2 -x, 3 +y, 4 -y, torus wrap: the table of
@@ -437,12 +500,12 @@ __device__ __forceinline__ void wave_bucket_add(uint32_t *s_win,
 			: (uint32_t)WAVE;
 		const uint32_t run = next - (uint32_t)lane;
 		const uint32_t o = bkt - w0;
-		if (bkt >= w0 && o < WIN) atomicAdd(&s_win[o], run);
+		if (bkt >= w0 && o < WIN) lds_add(&s_win[o], run);
 		else atomicAdd(&g_cnt[bkt], run);
 		if (top >= 1) {
 			const uint32_t q1 = bkt >> 6, o1 = q1 - u0;
 			if (q1 >= u0 && o1 < L1W) {
-				atomicAdd(&s_l1[o1], run);
+				lds_add(&s_l1[o1], run);
 			} else {
 				for (uint32_t k = 1; k <= top; k++)
 					atomicAdd(&g_cnt[cnt_off(nbc, (int)k) + (bkt >> (6 * k))], run);
@@ -1309,10 +1372,11 @@ struct SegVals {
 	uint32_t *s_val;
 	uint32_t *g_val;
 	__device__ __forceinline__ uint32_t get(uint32_t q) const {
-		return q < (uint32_t)A_TILE ? s_val[q] : g_val[q];
+		if (q < (uint32_t)A_TILE) return lds_ld(s_val + q);
+		return g_val[q];
 	}
 	__device__ __forceinline__ void set(uint32_t q, uint32_t v) const {
-		if (q < (uint32_t)A_TILE) s_val[q] = v;
+		if (q < (uint32_t)A_TILE) lds_st(s_val + q, v);
 		else g_val[q] = v;
 	}
 };
@@ -1354,7 +1418,8 @@ struct RA {
 	uint32_t key0, key1;
 };
 
-__device__ __forceinline__ RA load_ra(const Tables &T, uint32_t k,
+/* For a replica index that may differ between lanes. */
+__device__ __forceinline__ RA load_ra_lane(const Tables &T, uint32_t k,
 	uint32_t tick) {
 	const Rep &rp = T.rep[T.order[k]];
 	RA a;
@@ -1370,6 +1435,26 @@ __device__ __forceinline__ RA load_ra(const Tables &T, uint32_t k,
 	return a;
 }
 
+/* For a replica index uniform over the wave. */
+__device__ __forceinline__ RA load_ra(const Tables &T, uint32_t k,
+	uint32_t tick) {
+	/* k is uniform: the fields are moved to scalar registers, so later
+	 * uses do not wait on the vector loads (which retire in order with
+	 * every other global access). */
+	const Rep &rp = T.rep[__builtin_amdgcn_readfirstlane(T.order[k])];
+	RA a;
+	a.cb = __builtin_amdgcn_readfirstlane(rp.cb);
+	a.step = tick - __builtin_amdgcn_readfirstlane(rp.t0);
+	a.R = __builtin_amdgcn_readfirstlane(rp.R);
+	a.sheep_gain = __builtin_amdgcn_readfirstlane(rp.sheep_gain);
+	a.wolves_gain = __builtin_amdgcn_readfirstlane(rp.wolves_gain);
+	a.sheep_thr = __builtin_amdgcn_readfirstlane(rp.sheep_thr);
+	a.wolves_thr = __builtin_amdgcn_readfirstlane(rp.wolves_thr);
+	a.key0 = __builtin_amdgcn_readfirstlane(rp.key0);
+	a.key1 = __builtin_amdgcn_readfirstlane(rp.key1);
+	return a;
+}
+
 /* Adds v to counter idx of the replica with table index k, in an action tile
  * whose positions belong to several replicas (the first with index k_first):
  * LDS slots for the first MIX_SLOTS of them, global atomics beyond. */
@@ -1378,8 +1463,8 @@ __device__ __forceinline__ void mix_add(unsigned long long (*s_mix)[NACC],
 	unsigned long long v) {
 	if (!v) return;
 	const uint32_t d = k - k_first;
-	if (d < (uint32_t)MIX_SLOTS) atomicAdd(&s_mix[d][idx], v);
-	else atomicAdd(&T.rctl[T.order[k]].acc[idx], v);
+	if (d < (uint32_t)MIX_SLOTS) lds_add(&s_mix[d][idx], v);
+	else glb_add(&T.rctl[T.order[k]].acc[idx], v);
 }
 
 /* Adds the per-thread counters of one wave to dst (wave sums, one atomic
@@ -1395,7 +1480,7 @@ __device__ __noinline__ void wave_flush(unsigned long long a0,
 #pragma unroll
 	for (int k = 0; k < NACC; k++) {
 		const unsigned long long t = wave_sum(v[k]);
-		if (lane == 0 && t) atomicAdd(&dst[k], t);
+		if (lane == 0 && t) glb_add(&dst[k], t);
 	}
 }
 
@@ -1433,12 +1518,15 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 	uint32_t cj[A_IPT];
 	GT gj[A_IPT];
 	bool hj[A_IPT];
+	/* every lane loads (the tile's first cell where it holds no head), so
+	 * the loads are issued unconditionally */
+	const uint32_t c_safe = __builtin_amdgcn_readfirstlane(S.key[1]);
 #pragma unroll
 	for (int j = 0; j < A_IPT; j++) {
 		const uint32_t lp = tid * A_IPT + j;
 		cj[j] = S.key[1 + lp];
 		hj[j] = lp < cnt && S.key[lp] != cj[j];
-		gj[j] = hj[j] ? grass_at[cj[j]] : (GT)0;
+		gj[j] = grass_at[hj[j] ? cj[j] : c_safe];
 	}
 	/* Cells with one agent (most of them) are handled in place; heads of
 	 * longer segments are listed and then processed by consecutive lanes, so
@@ -1450,7 +1538,13 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 	if (MIXED)
 		for (int i = tid; i < MIX_SLOTS * NACC; i += WG) (&S.mix[0][0])[i] = 0;
 	__syncthreads();
+	/* the grass values (and the next tile's pairs, issued before them) */
+	wait_global();
 	uint32_t nout = 0;
+	/* sums over this thread's single-agent cells: at most A_IPT outputs
+	 * pairs below 2^25 each, so 32 bits hold them */
+	uint32_t f_sheep = 0, f_wolves = 0, f_se = 0, f_we = 0, f_eaten = 0,
+		f_occ = 0;
 #pragma unroll
 	for (int j = 0; j < A_IPT; j++) {
 		if (!hj[j]) continue;
@@ -1482,23 +1576,31 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 			overflow |= e > (int32_t)V_ENERGY;
 			S.val[lp] = nv | ((uint32_t)e & V_ENERGY);
 			const uint32_t outs = (nv & V_CHILD) ? 2u : 1u;
-			const unsigned long long en = (uint32_t)e + ce;
+			const uint32_t en = (uint32_t)e + ce;
 			if (t) {
-				acc[AC_WOLVES] += outs;
-				acc[AC_WOLVES_EN] += en;
+				f_wolves += outs;
+				f_we += en;
 			} else {
-				acc[AC_SHEEP] += outs;
-				acc[AC_SHEEP_EN] += en;
+				f_sheep += outs;
+				f_se += en;
 			}
-			acc[AC_EATEN] += ate;
-			acc[AC_OCC] += 1;
-			acc[AC_OUT] += outs;
+			f_eaten += ate;
+			f_occ += 1;
 			nout += outs;
 			continue;
 		}
 		const uint32_t slot = atomicAdd(&S.nslow, 1u);
 		S.slow_lp[slot] = lp;
 		S.slow_g[slot] = gj[j];
+	}
+	if (!MIXED) {
+		acc[AC_SHEEP] += f_sheep;
+		acc[AC_WOLVES] += f_wolves;
+		acc[AC_SHEEP_EN] += f_se;
+		acc[AC_WOLVES_EN] += f_we;
+		acc[AC_EATEN] += f_eaten;
+		acc[AC_OCC] += f_occ;
+		acc[AC_OUT] += f_sheep + f_wolves;
 	}
 	__syncthreads();
 	const uint32_t nslow = S.nslow;
@@ -1508,43 +1610,43 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 		const uint32_t c = S.key[1 + lp];
 		const GT gslow = S.slow_g[si];
 		const uint32_t kk = MIXED ? rep_index(T, nrep, c) : k_first;
-		const RA ra = MIXED ? load_ra(T, kk, tick) : ra_u;
+		const RA ra = MIXED ? load_ra_lane(T, kk, tick) : ra_u;
 		const uint32_t end = seg_end(S.key, keys, lp, cnt, base, L, c);
 		const uint32_t k = end - lp;
 		unsigned long long sc[NACC] = {0, 0, 0, 0, 0, 0, 0};
 		sc[AC_OCC] = 1;
 		if (k <= K_REG) {
-			/* the cell's values in registers (one Philox block covers the
-			 * k - 1 <= 4 draws): the same steps as below, without a chain
-			 * of dependent LDS accesses */
-			uint32_t V[K_REG];
-#pragma unroll
-			for (uint32_t q = 0; q < K_REG; q++) V[q] = q < k ? sv.get(lp + q) : 0u;
+			/* The cell in registers (one Philox block covers the k - 1 <= 4
+			 * draws), with the same steps as the general path below. The
+			 * Fisher-Yates swaps act on a permutation of 4-bit indices, and
+			 * the values are read from the tile already in acting order;
+			 * sheep and eaten agents are bit masks over acting positions. */
 			const uint4 r = pp_philox(c - ra.cb, ra.step, PP_STREAM_SHUFFLE, 0,
 				ra.key0, ra.key1);
+			uint32_t perm = 0x43210u;
 #pragma unroll
 			for (int i = K_REG - 1; i >= 1; i--) {
 				if ((uint32_t)i >= k) continue;
 				const uint32_t m = k - 1 - (uint32_t)i;
 				const uint32_t word = m == 0 ? r.x : m == 1 ? r.y : m == 2 ? r.z : r.w;
 				const uint32_t jj = pp_bounded(word, (uint32_t)i + 1);
-				const uint32_t vi = V[i];
-				uint32_t vj = V[0];
+				const uint32_t x = ((perm >> (4 * i)) ^ (perm >> (4 * jj))) & 0xFu;
+				perm ^= (x << (4 * i)) | (x << (4 * jj));
+			}
+			uint32_t V[K_REG];
+			uint32_t sheep = 0;
 #pragma unroll
-				for (int t = 1; t < K_REG; t++)
-					if ((uint32_t)t == jj) vj = V[t];
-#pragma unroll
-				for (int t = 0; t < K_REG; t++)
-					if ((uint32_t)t == jj) V[t] = vi;
-				V[i] = vj;
+			for (uint32_t q = 0; q < K_REG; q++) {
+				V[q] = q < k ? sv.get(lp + ((perm >> (4 * q)) & 0xFu)) : 0u;
+				if (q < k && !(V[q] & V_WOLF)) sheep |= 1u << q;
 			}
 			bool grass = grass_alive_at<GT>(gslow, alive_at, Rmax);
 			bool ate = false;
+			uint32_t killed = 0;
 #pragma unroll
-			for (int q = 0; q < K_REG; q++) {
-				if ((uint32_t)q >= k) continue;
+			for (uint32_t q = 0; q < K_REG; q++) {
+				if (q >= k || ((killed >> q) & 1u)) continue;
 				const uint32_t v = V[q];
-				if (v & V_KILLED) continue;
 				int32_t e = (int32_t)(v & V_ENERGY);
 				uint32_t thr;
 				if (!(v & V_WOLF)) {
@@ -1555,14 +1657,11 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 					}
 					thr = ra.sheep_thr;
 				} else {
-					bool fed = false;
-#pragma unroll
-					for (int o = 0; o < K_REG; o++) {
-						if (!fed && (uint32_t)o < k && !(V[o] & (V_WOLF | V_KILLED))) {
-							V[o] |= V_KILLED;
-							e += (int32_t)ra.wolves_gain;
-							fed = true;
-						}
+					/* the first sheep in acting order not yet eaten */
+					const uint32_t prey = sheep & ~killed;
+					if (prey) {
+						killed |= prey & (0u - prey);
+						e += (int32_t)ra.wolves_gain;
 					}
 					thr = ra.wolves_thr;
 				}
@@ -1575,28 +1674,34 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 				V[q] = nv | ((uint32_t)e & V_ENERGY);
 			}
 			if (ate) grass_at[c] = (GT)(tick + 1 + ra.R);
-			sc[AC_EATEN] = ate ? 1 : 0;
+			/* per-cell sums fit 32 bits: at most 2 K_REG outputs below
+			 * 2^24 each */
+			uint32_t n_sheep = 0, n_wolves = 0, e_sheep = 0, e_wolves = 0;
 #pragma unroll
 			for (uint32_t q = 0; q < K_REG; q++) {
 				if (q >= k) continue;
-				const uint32_t v = V[q];
+				const uint32_t v = V[q] | (((killed >> q) & 1u) ? V_KILLED : 0u);
 				sv.set(lp + q, v);
 				const uint32_t e = v & V_ENERGY;
-				const uint32_t t = (v & V_WOLF) ? 1u : 0u;
 				const uint32_t outs = ((v & V_KILLED) == 0) + ((v & V_CHILD) != 0);
 				const uint32_t ea = ((v & V_KILLED) ? 0u : e)
 					+ ((v & V_CHILD) ? ((v & V_ODD) ? e - 1 : e) : 0u);
-				if (t) {
-					sc[AC_WOLVES] += outs;
-					sc[AC_WOLVES_EN] += ea;
+				if (v & V_WOLF) {
+					n_wolves += outs;
+					e_wolves += ea;
 				} else {
-					sc[AC_SHEEP] += outs;
-					sc[AC_SHEEP_EN] += ea;
+					n_sheep += outs;
+					e_sheep += ea;
 				}
-				sc[AC_OUT] += outs;
 				if (lp + q < (uint32_t)A_TILE) nout += outs;
 				else if (outs) atomicAdd(&act_cnt[tile + (lp + q) / A_TILE], outs);
 			}
+			sc[AC_SHEEP] = n_sheep;
+			sc[AC_WOLVES] = n_wolves;
+			sc[AC_SHEEP_EN] = e_sheep;
+			sc[AC_WOLVES_EN] = e_wolves;
+			sc[AC_OUT] = n_sheep + n_wolves;
+			sc[AC_EATEN] = ate ? 1 : 0;
 		} else {
 			if (k > 1) {
 				uint4 r = make_uint4(0, 0, 0, 0);
@@ -1721,7 +1826,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
  * k_act_mixed (segments never span replicas, so the two kernels touch
  * disjoint cells and values). */
 template <typename GT>
-__global__ __launch_bounds__(WG, 5) void k_act(const uint32_t *__restrict__ keys,
+__global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys,
 	uint32_t *vals, GT *grass_at, uint32_t *act_cnt, uint32_t *mixed_list,
 	uint32_t *ocnt, uint32_t *ncnt, Ctl *ctl, Tables T, Params P) {
 	__shared__ ActLds<GT> S;
@@ -1774,7 +1879,16 @@ __global__ __launch_bounds__(WG, 5) void k_act(const uint32_t *__restrict__ keys
 		}
 		if (tid == 0) pprev = base ? keys[base - 1] : DEAD_KEY;
 	};
-	if (t_beg < t_end) fetch(t_beg);
+	if (t_beg < t_end) {
+		fetch(t_beg);
+		/* A batch of one replica (every plain simulation): its constants
+		 * before the first tile. */
+		if (nrep == 1) {
+			acc_k = 0;
+			ra = load_ra(T, 0, tick);
+		}
+	}
+	wait_global();
 
 	for (uint32_t tile = t_beg; tile < t_end; tile++) {
 		const uint32_t cnt = min((uint32_t)A_TILE, L - tile * A_TILE);
@@ -1785,30 +1899,37 @@ __global__ __launch_bounds__(WG, 5) void k_act(const uint32_t *__restrict__ keys
 		}
 		if (tid == 0) S.key[0] = pprev;
 		__syncthreads();
-		if (tile + 1 < t_end) fetch(tile + 1);
 		/* Replicas of this tile: keys ascend, so they are the table entries
 		 * k_first..k_last. Every branch below is uniform over the group. */
 		const uint32_t k_first = rep_index(T, nrep,
 			__builtin_amdgcn_readfirstlane(S.key[1]));
 		const uint32_t k_last = rep_index(T, nrep,
 			__builtin_amdgcn_readfirstlane(S.key[cnt]));
+		/* The next tile's pairs are requested inside each branch, after
+		 * every load this tile waits on before its grass, and both branches
+		 * end in an explicit wait (act_tile waits on the grass loads issued
+		 * after them): the loop's top then never waits for them behind this
+		 * tile's stores. */
 		if (k_first != k_last) {
+			if (tile + 1 < t_end) fetch(tile + 1);
 			if (tid == 0) mixed_list[atomicAdd(&ctl->nmixed, 1u)] = tile;
 			__syncthreads();
-			continue;
-		}
-		if (k_first != acc_k) {
-			if (acc_k != NO_REP) {
-				wave_flush(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6],
-					&T.rctl[T.order[acc_k]].acc[0]);
+			wait_global();
+		} else {
+			if (k_first != acc_k) {
+				if (acc_k != NO_REP) {
+					wave_flush(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6],
+						&T.rctl[T.order[acc_k]].acc[0]);
 #pragma unroll
-				for (int i = 0; i < NACC; i++) acc[i] = 0;
+					for (int i = 0; i < NACC; i++) acc[i] = 0;
+				}
+				acc_k = k_first;
+				ra = load_ra(T, k_first, tick);
 			}
-			acc_k = k_first;
-			ra = load_ra(T, k_first, tick);
+			if (tile + 1 < t_end) fetch(tile + 1);
+			overflow |= act_tile<GT, false>(S, tile, cnt, keys, vals, grass_at,
+				act_cnt, T, nrep, tick, L, P.R, k_first, k_last, ra, acc);
 		}
-		overflow |= act_tile<GT, false>(S, tile, cnt, keys, vals, grass_at,
-			act_cnt, T, nrep, tick, L, P.R, k_first, k_last, ra, acc);
 	}
 	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
 	if (acc_k != NO_REP) wg_add<NACC>(acc, S.red, &T.rctl[T.order[acc_k]].acc[0]);
@@ -2060,7 +2181,7 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	bool overflow = false;
 	/* A batch of one replica (every plain simulation) needs no lookups. */
 	RM m_one{};
-	if (nrep == 1 && ntiles > blockIdx.x) m_one = load_rm(T, 0, tick, S);
+	if (nrep == 1 && ntiles > blockIdx.x) m_one = load_rm_uniform(T, 0, tick, S);
 
 	for (uint32_t i = tid; i < 2 * BWIN; i += WG) s_win[i] = 0;
 	for (uint32_t i = tid; i < 2 * L1W; i += WG) s_l1[i] = 0;
@@ -2068,6 +2189,9 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 		const uint32_t base = tile * A_TILE;
 		const uint32_t goff = act_off[tile];
+		/* the tile's first cell, requested with its pairs (uniform: a
+		 * scalar load) */
+		const uint32_t c_first = __builtin_amdgcn_readfirstlane(keys[base]);
 		const uint32_t p0 = base + tid * A_IPT;
 		uint32_t c[A_IPT], v[A_IPT];
 		if (p0 + A_IPT <= L) {
@@ -2108,7 +2232,6 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		/* Replicas of this tile's positions (keys ascend), and the first
 		 * bucket of the count window: the bucket of the tile's first cell,
 		 * less BWIN/4 for moves to the row above. Uniform over the group. */
-		const uint32_t c_first = __builtin_amdgcn_readfirstlane(keys[base]);
 		bool mixed = false;
 		RM m_t = m_one;
 		uint32_t k_first = 0;
@@ -2117,7 +2240,7 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 			const uint32_t k_last = rep_index(T, nrep,
 				__builtin_amdgcn_readfirstlane(keys[min(base + A_TILE, L) - 1]));
 			mixed = k_first != k_last;
-			m_t = load_rm(T, k_first, tick, S);
+			m_t = load_rm_uniform(T, k_first, tick, S);
 		}
 		uint32_t w0 = 0;
 		{
