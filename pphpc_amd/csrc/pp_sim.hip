@@ -938,6 +938,7 @@ constexpr uint32_t NXCD = 8;     /* accelerator dies (XCDs) of the GPU */
 constexpr uint32_t W_MAX = 2048; /* cells of a bucket block: 2^S, S <= 11 */
 static_assert(W_MAX <= 2 * BCAP, "a dense bucket's counters reuse the staging");
 constexpr uint32_t BG_IPT = 8;   /* source positions per thread per round */
+static_assert(NWAVE == 4, "the gather reads the waves' counts as one uint4");
 /* Most buckets the automatic choice sorts by buckets in a batch of several
  * replicas (above, the radix sort is faster there: each bucket costs a few
  * dependent round trips to memory, and many small replicas make many
@@ -985,9 +986,11 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	__shared__ uint32_t s_cnt2[W_MAX / 2]; /* per cell, 16 bits: count, start, end */
 	uint32_t *s_ka = s_buf, *s_va = s_buf + BCAP;
 	__shared__ uint32_t s_need[2][14];
-	__shared__ uint32_t s_wc[2][BG_IPT * NWAVE];
+	__shared__ uint4 s_wc[2][BG_IPT];     /* per element row: the waves' counts */
 	__shared__ uint32_t s_rng[10];
 	__shared__ uint32_t s_nrng;
+	/* per range: first sequence index (cum) and position - index (delta) */
+	__shared__ uint32_t s_cum[5], s_delta[5], s_nsrc;
 	__shared__ uint32_t s_tmp[NWAVE];
 	if (ctl->error) return;
 	const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
@@ -1010,13 +1013,16 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	const uint32_t NB = ctl->nbuckets, nrep = ctl->nrep, S = P.bshift;
 	const uint32_t W = 1u << S;
 
-	/* a plain simulation's one replica, loaded once */
-	uint32_t gx1 = 0, gy1 = 0, cb1 = 0;
+	/* a plain simulation's one replica and its bucket layout, once */
+	uint32_t gx1 = 0, gy1 = 0, cb1 = 0, nb1 = 1, h1 = 1, nby1 = 1;
 	if (nrep == 1) {
 		const Rep &rp = T.rep[T.order[0]];
-		gx1 = rp.gx;
-		gy1 = rp.gy;
-		cb1 = rp.cb;
+		gx1 = __builtin_amdgcn_readfirstlane(rp.gx);
+		gy1 = __builtin_amdgcn_readfirstlane(rp.gy);
+		cb1 = __builtin_amdgcn_readfirstlane(rp.cb);
+		nb1 = row_blocks(gx1, S);
+		h1 = rows_per_bucket(gx1, S);
+		nby1 = (gy1 + h1 - 1) / h1;
 	}
 	/* Geometry of bucket b: its replica (largest k with bbase[k] <= b), grid
 	 * rows [y h, y h + h), block j (nb >= 2: h = 1), and the buckets its
@@ -1031,6 +1037,9 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 		g.gx = gx1;
 		g.gy = gy1;
 		g.cb = cb1;
+		g.nb = nb1;
+		g.h = h1;
+		uint32_t nby = nby1;
 		uint32_t bb = 0;
 		if (nrep != 1) {
 			uint32_t lo = 0, hi = nrep;
@@ -1044,20 +1053,22 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 			g.gy = rp.gy;
 			g.cb = rp.cb;
 			bb = T.bbase[lo];
+			g.nb = row_blocks(g.gx, S);
+			g.h = rows_per_bucket(g.gx, S);
+			nby = (g.gy + g.h - 1) / g.h;
 		}
-		g.nb = row_blocks(g.gx, S);
-		g.h = rows_per_bucket(g.gx, S);
-		const uint32_t nby = (g.gy + g.h - 1) / g.h;
 		const uint32_t local = b - bb;
-		g.y = local / g.nb;
+		g.y = g.nb == 1 ? local : local / g.nb;
 		g.j = local - g.y * g.nb;
 		const uint32_t x0 = g.j << S, x1 = min(x0 + W, g.gx);
 		const uint32_t xl = x0 ? x0 - 1 : g.gx - 1;
 		const uint32_t xr = x1 < g.gx ? x1 : 0;
+		const uint32_t ym = g.y ? g.y - 1 : nby - 1;
+		const uint32_t yp = g.y + 1 < nby ? g.y + 1 : 0;
 		g.need[0] = b;
-		g.need[1] = bb + ((g.y + nby - 1) % nby) * g.nb + g.j;
+		g.need[1] = bb + ym * g.nb + g.j;
 		g.need[2] = b;
-		g.need[3] = bb + ((g.y + 1) % nby) * g.nb + g.j;
+		g.need[3] = bb + yp * g.nb + g.j;
 		g.need[4] = bb + g.y * g.nb + (xl >> S);
 		g.need[5] = bb + g.y * g.nb + (xr >> S);
 		return g;
@@ -1158,16 +1169,24 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 			};
 			ce2(0, 3); ce2(1, 4); ce2(0, 2); ce2(1, 3); ce2(0, 1);
 			ce2(2, 4); ce2(1, 2); ce2(3, 4); ce2(2, 3);
-			uint32_t m = 0, last = 0xFFFFFFFFu;
+			uint32_t m = 0, last = 0xFFFFFFFFu, seq = 0;
 #pragma unroll
 			for (int a = 0; a < 5; a++) {
 				if (rs[a] >= re[a] || rs[a] == last) continue;
 				s_rng[2 * m] = rs[a];
 				s_rng[2 * m + 1] = re[a];
+				s_cum[m] = seq;
+				s_delta[m] = rs[a] - seq;
+				seq += re[a] - rs[a];
 				last = rs[a];
 				m++;
 			}
+			for (uint32_t a = m; a < 5; a++) {
+				s_cum[a] = 0xFFFFFFFFu;
+				s_delta[a] = 0;
+			}
 			s_nrng = m;
+			s_nsrc = seq;
 		}
 		for (uint32_t i = tid; i < W / 2; i += WG) s_cnt2[i] = 0;
 		__syncthreads();
@@ -1177,8 +1196,18 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 		 * issued together; the compaction keeps the sequence order */
 		uint32_t cnt = 0;
 		const uint32_t nr = s_nrng;
-		uint32_t nsrc = 0;
-		for (uint32_t r = 0; r < nr; r++) nsrc += s_rng[2 * r + 1] - s_rng[2 * r];
+		const uint32_t nsrc = __builtin_amdgcn_readfirstlane(s_nsrc);
+		/* range r of sequence index e: the last r with cum[r] <= e (uniform
+		 * bounds, so a few compares and selects per element) */
+		const uint32_t cum1 = __builtin_amdgcn_readfirstlane(s_cum[1]);
+		const uint32_t cum2 = __builtin_amdgcn_readfirstlane(s_cum[2]);
+		const uint32_t cum3 = __builtin_amdgcn_readfirstlane(s_cum[3]);
+		const uint32_t cum4 = __builtin_amdgcn_readfirstlane(s_cum[4]);
+		const uint32_t d0 = __builtin_amdgcn_readfirstlane(s_delta[0]);
+		const uint32_t d1 = __builtin_amdgcn_readfirstlane(s_delta[1]);
+		const uint32_t d2 = __builtin_amdgcn_readfirstlane(s_delta[2]);
+		const uint32_t d3 = __builtin_amdgcn_readfirstlane(s_delta[3]);
+		const uint32_t d4 = __builtin_amdgcn_readfirstlane(s_delta[4]);
 		const uint64_t lt_mask = (1ull << lane) - 1ull;
 		for (uint32_t v0 = 0, rnd = 0; v0 < nsrc; v0 += WG * BG_IPT, rnd++) {
 			uint32_t kk[BG_IPT], vv[BG_IPT];
@@ -1188,19 +1217,16 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 				kk[jj] = DEAD_KEY;
 				vv[jj] = 0u;
 				if (e < nsrc) {
-					uint32_t r = 0, c0 = 0;
-					while (r + 1 < nr && e >= c0 + s_rng[2 * r + 1] - s_rng[2 * r]) {
-						c0 += s_rng[2 * r + 1] - s_rng[2 * r];
-						r++;
-					}
-					const uint32_t p = s_rng[2 * r] + (e - c0);
+					const uint32_t d = e >= cum4 ? d4 : e >= cum3 ? d3
+						: e >= cum2 ? d2 : e >= cum1 ? d1 : d0;
+					const uint32_t p = e + d;
 					kk[jj] = keys[p];
 					vv[jj] = vals[p];
 				}
 			}
 			/* per element: its rank among the wave's landing ones; per
 			 * (element row jj, wave): their count, in LDS */
-			uint32_t *wc = s_wc[rnd & 1];
+			uint32_t *wc = reinterpret_cast<uint32_t *>(s_wc[rnd & 1]);
 			uint32_t rank[BG_IPT], in_mask = 0;
 #pragma unroll
 			for (uint32_t jj = 0; jj < BG_IPT; jj++) {
@@ -1214,13 +1240,10 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 			uint32_t run = 0;
 #pragma unroll
 			for (uint32_t jj = 0; jj < BG_IPT; jj++) {
-				uint32_t off = 0;
-#pragma unroll
-				for (int ww = 0; ww < NWAVE; ww++) {
-					const uint32_t c = wc[jj * NWAVE + ww];
-					if (ww == w) off = run;
-					run += c;
-				}
+				const uint4 c4 = s_wc[rnd & 1][jj];
+				const uint32_t off = run + (w > 0 ? c4.x : 0u) + (w > 1 ? c4.y : 0u)
+					+ (w > 2 ? c4.z : 0u);
+				run += c4.x + c4.y + c4.z + c4.w;
 				if (in_mask & (1u << jj)) {
 					const uint32_t slot = cnt + off + rank[jj];
 					if (slot < BCAP) {
