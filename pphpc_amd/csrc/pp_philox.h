@@ -25,10 +25,12 @@ __device__ __forceinline__ uint4 pp_philox(uint32_t c0, uint32_t c1,
 	uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
 #pragma unroll
 	for (int r = 0; r < 10; r++) {
-		const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
-		const uint32_t lo0 = 0xD2511F53u * c0;
-		const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
-		const uint32_t lo1 = 0xCD9E8D57u * c2;
+		/* one 32 x 32 -> 64 product per multiplier (a single
+		 * v_mad_u64_u32 instead of separate high and low multiplies) */
+		const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+		const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+		const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+		const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
 		const uint32_t n0 = hi1 ^ c1 ^ k0;
 		const uint32_t n2 = hi0 ^ c3 ^ k1;
 		c0 = n0;

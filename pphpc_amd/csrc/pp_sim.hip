@@ -1300,29 +1300,20 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 				s_ix[(old >> sh) & 0xFFFFu] = (uint16_t)i;
 			}
 			__syncthreads();
-			/* each counter is now the end of its cell's slots; the thread of
-			 * a cell's first slot orders that cell's slots */
-			for (uint32_t q = tid; q < cnt; q += WG) {
-				const uint32_t c = s_ka[s_ix[q]];
-				const uint32_t e = (s_cnt2[c >> 1] >> ((c & 1u) * 16)) & 0xFFFFu;
-				const uint32_t s0 = c ? (s_cnt2[(c - 1) >> 1] >> (((c - 1) & 1u) * 16)) & 0xFFFFu : 0u;
-				if (q != s0 || e - s0 < 2) continue;
-				for (uint32_t a = s0 + 1; a < e; a++) {
-					const uint16_t x = s_ix[a];
-					uint32_t d = a;
-					while (d > s0 && s_ix[d - 1] > x) {
-						s_ix[d] = s_ix[d - 1];
-						d--;
-					}
-					s_ix[d] = x;
-				}
-			}
-			__syncthreads();
 			if (bn < NB) publish(s_need[(it + 1) & 1]);
+			/* Each counter is now the end of its cell's slots, which hold the
+			 * cell's agents in scatter order: an agent's place among them is
+			 * the number of its cell's agents gathered before it (few: cells
+			 * hold a handful of agents), and it is written there directly. */
 			for (uint32_t q = tid; q < cnt; q += WG) {
 				const uint32_t i = s_ix[q];
-				okeys[out0 + q] = cs + s_ka[i];
-				ovals[out0 + q] = s_va[i];
+				const uint32_t c = s_ka[i];
+				const uint32_t e = (s_cnt2[c >> 1] >> ((c & 1u) * 16)) & 0xFFFFu;
+				const uint32_t s0 = c ? (s_cnt2[(c - 1) >> 1] >> (((c - 1) & 1u) * 16)) & 0xFFFFu : 0u;
+				uint32_t at = s0;
+				for (uint32_t a = s0; a < e; a++) at += s_ix[a] < i;
+				okeys[out0 + at] = cs + c;
+				ovals[out0 + at] = s_va[i];
 			}
 		} else {
 			/* More agents than LDS holds (a dense bucket): count per cell,
