@@ -155,15 +155,43 @@ struct Ctl {
 	uint32_t live[2];  /* survivors of the move, indexed by tick parity */
 	uint32_t tick;     /* iterations run by this batch */
 	int32_t error;     /* 0 or a PP_* code; every kernel is a no-op once set */
+	uint32_t nrep;     /* replicas in the lookup table */
+	uint32_t nbuckets; /* buckets of the replicas in the table */
+	uint32_t nmixed;   /* action tiles listed for k_act_mixed this tick */
 	uint32_t err_step; /* tick at which error was set */
 	int32_t err_rep;   /* slot of the replica involved, -1 for the batch */
-	uint32_t nrep;     /* replicas in the lookup table */
-	uint32_t nmixed;   /* action tiles listed for k_act_mixed this tick */
-	uint32_t nbuckets; /* buckets of the replicas in the table */
 	/* sums over completed ticks of: agents at the start, alive after the
 	 * move, after actions, occupied cells, grass eaten */
 	unsigned long long agent_steps, sum_live, sum_out, sum_occ, sum_eaten;
 };
+
+/* The first eight words of Ctl, which the step kernels start from: read
+ * with one load (a chain of dependent reads - the error flag, then the
+ * tick, then the live count of its parity - costs a round trip each). */
+struct CtlHead {
+	uint32_t n, live0, live1, tick;
+	int32_t error;
+	uint32_t nrep, nbuckets, nmixed;
+	__device__ __forceinline__ uint32_t live(uint32_t t) const {
+		return (t & 1u) ? live1 : live0;
+	}
+};
+static_assert(offsetof(Ctl, nmixed) == 28, "CtlHead covers Ctl's first 8 words");
+
+__device__ __forceinline__ CtlHead ctl_head(const Ctl *ctl) {
+	typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
+	const u32x8 v = *reinterpret_cast<const u32x8 *>(ctl);
+	CtlHead h;
+	h.n = v[0];
+	h.live0 = v[1];
+	h.live1 = v[2];
+	h.tick = v[3];
+	h.error = (int32_t)v[4];
+	h.nrep = v[5];
+	h.nbuckets = v[6];
+	h.nmixed = v[7];
+	return h;
+}
 
 /* Batch-wide launch constants. */
 struct Params {
@@ -178,6 +206,8 @@ struct Params {
 	uint32_t bucket_on;  /* sort by row buckets; the emit counts for them */
 	uint32_t nb_cap;     /* bucket capacity of the count tables */
 	uint32_t cnt_top;    /* highest level of the bucket count tables in use */
+	uint32_t one_slot;   /* slot of the replica when the table holds one */
+	uint32_t act_l1;     /* offset of the per-64-tile sums in the action counts */
 };
 
 /* Replica tables in device memory: constants and control words by slot, and
@@ -263,6 +293,19 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 #pragma unroll
 	for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
 	return v;
+}
+
+/* Sum of a 32-bit value over the 64 lanes of the wave (all lanes active),
+ * returned in every lane: DPP quad permutes, row shifts and row broadcasts
+ * (no LDS round trips, unlike lane shuffles). Lane 63 ends with the total. */
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xb1, 0xf, 0xf, false);  /* quad [1,0,3,2] */
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4e, 0xf, 0xf, false);  /* quad [2,3,0,1] */
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false); /* row_shr:4 */
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false); /* row_shr:8 */
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false); /* row_bcast:15 */
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false); /* row_bcast:31 */
+	return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 /* Work-group sums of K counters (NT threads), added with one atomic each to
@@ -369,8 +412,8 @@ __device__ __forceinline__ uint32_t bucket_of(uint32_t bbase, uint32_t nb,
 
 /* The move constants of the k-th replica of the table at tick `tick`. */
 __device__ __forceinline__ RM load_rm(const Tables &T, uint32_t k,
-	uint32_t tick, uint32_t S) {
-	const uint32_t slot = T.order[k];
+	uint32_t tick, uint32_t S, uint32_t slot_hint = NO_REP) {
+	const uint32_t slot = slot_hint != NO_REP ? slot_hint : T.order[k];
 	const Rep &rp = T.rep[slot];
 	const RepCtl &rc = T.rctl[slot];
 	RM m;
@@ -394,8 +437,8 @@ __device__ __forceinline__ RM load_rm(const Tables &T, uint32_t k,
 /* load_rm for a replica index uniform over the wave, its fields moved to
  * scalar registers (later uses then wait on no vector load). */
 __device__ __forceinline__ RM load_rm_uniform(const Tables &T, uint32_t k,
-	uint32_t tick, uint32_t S) {
-	RM m = load_rm(T, k, tick, S);
+	uint32_t tick, uint32_t S, uint32_t slot_hint = NO_REP) {
+	RM m = load_rm(T, k, tick, S, slot_hint);
 	m.cb = __builtin_amdgcn_readfirstlane(m.cb);
 	m.C = __builtin_amdgcn_readfirstlane(m.C);
 	m.gx = __builtin_amdgcn_readfirstlane(m.gx);
@@ -992,31 +1035,20 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	/* per range: first sequence index (cum) and position - index (delta) */
 	__shared__ uint32_t s_cum[5], s_delta[5], s_nsrc;
 	__shared__ uint32_t s_tmp[NWAVE];
-	if (ctl->error) return;
+	const CtlHead H = ctl_head(ctl);
+	if (H.error) return;
 	const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
-	const uint32_t tick = ctl->tick;
-	/* The tick's first kernel: clear the action tiles' counters and re-stamp
-	 * the alive cells of grass slice (t mod REFRESH) with t (keeps stored
-	 * regrow ticks within the modular window; the state does not change). */
+	const uint32_t tick = H.tick;
+	/* The tick's first kernel: clear the action tiles' counters */
 	for (uint32_t i = blockIdx.x * WG + tid; i < act_words; i += gridDim.x * WG)
 		act_cnt[i] = 0;
-	{
-		const uint32_t per = (P.C + REFRESH - 1) / REFRESH;
-		const uint32_t c0 = (tick % REFRESH) * per;
-		const uint32_t c1 = min(c0 + per, P.C);
-		for (uint32_t c = c0 + blockIdx.x * WG + tid; c < c1; c += gridDim.x * WG) {
-			const GT g = grass_at[c];
-			if (grass_alive_at<GT>(g, tick, P.R) && g != (GT)tick)
-				grass_at[c] = (GT)tick;
-		}
-	}
-	const uint32_t NB = ctl->nbuckets, nrep = ctl->nrep, S = P.bshift;
+	const uint32_t NB = H.nbuckets, nrep = H.nrep, S = P.bshift;
 	const uint32_t W = 1u << S;
 
 	/* a plain simulation's one replica and its bucket layout, once */
 	uint32_t gx1 = 0, gy1 = 0, cb1 = 0, nb1 = 1, h1 = 1, nby1 = 1;
 	if (nrep == 1) {
-		const Rep &rp = T.rep[T.order[0]];
+		const Rep &rp = T.rep[P.one_slot];
 		gx1 = __builtin_amdgcn_readfirstlane(rp.gx);
 		gy1 = __builtin_amdgcn_readfirstlane(rp.gy);
 		cb1 = __builtin_amdgcn_readfirstlane(rp.cb);
@@ -1375,6 +1407,36 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 		__syncthreads();
 		if (bn >= NB) break;
 	}
+	/* Re-stamp the alive cells of grass slice (t mod REFRESH) with t (keeps
+	 * stored regrow ticks within the modular window; the state does not
+	 * change). At the end, off the buckets' path; the slice is dealt out
+	 * from the groups that had no bucket, if any, and a thread issues its
+	 * loads before its stores. */
+	{
+		const uint32_t per = (P.C + REFRESH - 1) / REFRESH;
+		const uint32_t c0 = (tick % REFRESH) * per;
+		const uint32_t c1 = min(c0 + per, P.C);
+		const uint32_t busy = min(NB, gridDim.x);
+		const uint32_t g = (blockIdx.x + gridDim.x - busy) % gridDim.x;
+		const uint32_t stride = gridDim.x * WG;
+		constexpr int RB = 8;   /* cells a thread loads before it stores */
+		for (uint32_t c = c0 + g * WG + tid; c < c1; c += RB * stride) {
+			GT gv[RB];
+#pragma unroll
+			for (int u = 0; u < RB; u++) {
+				const uint64_t i = (uint64_t)c + (uint64_t)u * stride;
+				gv[u] = i < c1 ? grass_at[i] : (GT)tick;
+			}
+#pragma unroll
+			for (int u = 0; u < RB; u++) {
+				const uint64_t i = (uint64_t)c + (uint64_t)u * stride;
+				if (i < c1 && grass_alive_at<GT>(gv[u], tick, P.R) && gv[u] != (GT)tick)
+					grass_at[i] = (GT)tick;
+			}
+			if ((uint64_t)c + (uint64_t)RB * stride >= c1) break;
+		}
+	}
+
 }
 
 /* ----------------------------------------------------------------- action */
@@ -1449,13 +1511,15 @@ __device__ __forceinline__ RA load_ra_lane(const Tables &T, uint32_t k,
 	return a;
 }
 
-/* For a replica index uniform over the wave. */
+/* For a replica index uniform over the wave (slot_hint: its slot, when
+ * known without the table). */
 __device__ __forceinline__ RA load_ra(const Tables &T, uint32_t k,
-	uint32_t tick) {
+	uint32_t tick, uint32_t slot_hint = NO_REP) {
 	/* k is uniform: the fields are moved to scalar registers, so later
 	 * uses do not wait on the vector loads (which retire in order with
 	 * every other global access). */
-	const Rep &rp = T.rep[__builtin_amdgcn_readfirstlane(T.order[k])];
+	const Rep &rp = T.rep[slot_hint != NO_REP ? slot_hint
+		: __builtin_amdgcn_readfirstlane(T.order[k])];
 	RA a;
 	a.cb = __builtin_amdgcn_readfirstlane(rp.cb);
 	a.step = tick - __builtin_amdgcn_readfirstlane(rp.t0);
@@ -1498,6 +1562,56 @@ __device__ __noinline__ void wave_flush(unsigned long long a0,
 	}
 }
 
+/* Adds v outputs to action tile t: its count, and the sum of its block of
+ * 64 tiles (at l1o), which the scan works from. */
+__device__ __forceinline__ void tile_add(uint32_t *act_cnt, uint32_t l1o,
+	uint32_t t, uint32_t v) {
+	atomicAdd(&act_cnt[t], v);
+	atomicAdd(&act_cnt[l1o + (t >> 6)], v);
+}
+
+/* Work-group sums of the action counters, added to dst (a replica's acc):
+ * AC_OUT is the sheep plus the wolves emitted; counts are summed in 32 bits
+ * and each energy sum as two 32-bit pieces split at bit 24 (exact while
+ * every lane's sum is below 2^48; a wave holding a larger one sums it in 64
+ * bits). acc is cleared. */
+__device__ __forceinline__ void act_wg_add(unsigned long long (&acc)[NACC],
+	unsigned long long (*s_red)[NACC], unsigned long long *dst) {
+	const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+	const uint32_t ns = wave_sum32((uint32_t)acc[AC_SHEEP]);
+	const uint32_t nw = wave_sum32((uint32_t)acc[AC_WOLVES]);
+	const uint32_t ne = wave_sum32((uint32_t)acc[AC_EATEN]);
+	const uint32_t no = wave_sum32((uint32_t)acc[AC_OCC]);
+	unsigned long long es, ew;
+	if (__ballot(((acc[AC_SHEEP_EN] | acc[AC_WOLVES_EN]) >> 48) != 0) == 0) {
+		es = wave_sum32((uint32_t)(acc[AC_SHEEP_EN] & 0xFFFFFFu))
+			+ ((unsigned long long)wave_sum32((uint32_t)(acc[AC_SHEEP_EN] >> 24)) << 24);
+		ew = wave_sum32((uint32_t)(acc[AC_WOLVES_EN] & 0xFFFFFFu))
+			+ ((unsigned long long)wave_sum32((uint32_t)(acc[AC_WOLVES_EN] >> 24)) << 24);
+	} else {
+		es = wave_sum(acc[AC_SHEEP_EN]);
+		ew = wave_sum(acc[AC_WOLVES_EN]);
+	}
+	if (lane == 0) {
+		s_red[w][AC_SHEEP] = ns;
+		s_red[w][AC_WOLVES] = nw;
+		s_red[w][AC_SHEEP_EN] = es;
+		s_red[w][AC_WOLVES_EN] = ew;
+		s_red[w][AC_EATEN] = ne;
+		s_red[w][AC_OCC] = no;
+		s_red[w][AC_OUT] = (unsigned long long)ns + nw;
+	}
+#pragma unroll
+	for (int k = 0; k < NACC; k++) acc[k] = 0;
+	__syncthreads();
+	if (threadIdx.x < NACC) {
+		unsigned long long t = 0;
+		for (int i = 0; i < NWAVE; i++) t += s_red[i][threadIdx.x];
+		if (t) atomicAdd(&dst[threadIdx.x], t);
+	}
+	__syncthreads();
+}
+
 /* LDS of one action work-group. */
 template <typename GT>
 struct ActLds {
@@ -1519,7 +1633,7 @@ struct ActLds {
 template <typename GT, bool MIXED>
 __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 	uint32_t cnt, const uint32_t *__restrict__ keys, uint32_t *vals,
-	GT *grass_at, uint32_t *act_cnt, const Tables &T, uint32_t nrep,
+	GT *grass_at, uint32_t *act_cnt, uint32_t l1o, const Tables &T, uint32_t nrep,
 	uint32_t tick, uint32_t L, uint32_t Rmax, uint32_t k_first,
 	uint32_t k_last, const RA &ra_u, unsigned long long (&acc)[NACC]) {
 	const int tid = threadIdx.x;
@@ -1708,7 +1822,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 					e_sheep += ea;
 				}
 				if (lp + q < (uint32_t)A_TILE) nout += outs;
-				else if (outs) atomicAdd(&act_cnt[tile + (lp + q) / A_TILE], outs);
+				else if (outs) tile_add(act_cnt, l1o, tile + (lp + q) / A_TILE, outs);
 			}
 			sc[AC_SHEEP] = n_sheep;
 			sc[AC_WOLVES] = n_wolves;
@@ -1784,7 +1898,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 				}
 				sc[AC_OUT] += outs;
 				if (q < (uint32_t)A_TILE) nout += outs;
-				else if (outs) atomicAdd(&act_cnt[tile + q / A_TILE], outs);
+				else if (outs) tile_add(act_cnt, l1o, tile + q / A_TILE, outs);
 			}
 		}
 		if (!MIXED) {
@@ -1802,7 +1916,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 		if (S.key[1 + i] != S.key[0]) vals[base + i] = S.val[i];
 	uint32_t total;
 	(void)wg_excl_scan<WG>(nout, S.tmp, total);
-	if (tid == 0 && total) atomicAdd(&act_cnt[tile], total);
+	if (tid == 0 && total) tile_add(act_cnt, l1o, tile, total);
 	if (MIXED) {
 		const uint32_t nk = min(k_last - k_first + 1, (uint32_t)MIX_SLOTS);
 		for (uint32_t i = tid; i < nk * NACC; i += WG) {
@@ -1832,8 +1946,9 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
  *      segment, so they neither act nor get eaten (pp_cpu.cl:841-845).
  * The permuted, updated values are written back in place, so the next state
  * is the sorted positions in order, each emitting its survivor then its
- * newborn; act_cnt[t] (zeroed by the pass-0 scan) receives the number of
- * agents the positions of tile t emit. Per replica, the counts and energy
+ * newborn; act_cnt[t] (zeroed by the tick's first kernel) receives the
+ * number of agents the positions of tile t emit, and the sum of its block of
+ * 64 tiles the same. Per replica, the counts and energy
  * sums of the new state, the grass eaten, the occupied cells and the outputs
  * are summed into its control words, in registers over the tiles of one
  * replica. Tiles holding agents of several replicas are listed for
@@ -1844,13 +1959,14 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 	uint32_t *vals, GT *grass_at, uint32_t *act_cnt, uint32_t *mixed_list,
 	uint32_t *ocnt, uint32_t *ncnt, Ctl *ctl, Tables T, Params P) {
 	__shared__ ActLds<GT> S;
-	if (ctl->error) return;
+	const CtlHead H = ctl_head(ctl);
+	if (H.error) return;
 	const int tid = threadIdx.x;
-	const uint32_t tick = ctl->tick, nrep = ctl->nrep;
+	const uint32_t tick = H.tick, nrep = H.nrep;
 	/* The sort has read the bucket counts: clear the entries in use for
 	 * this tick's emit. */
 	if (P.bucket_on) {
-		const uint32_t NB = ctl->nbuckets;
+		const uint32_t NB = H.nbuckets;
 		for (uint32_t k = 0; k <= P.cnt_top; k++) {
 			const uint32_t o = cnt_off(P.nb_cap, (int)k), n = (NB >> (6 * k)) + 1;
 			for (uint32_t i = blockIdx.x * WG + tid; i < n; i += gridDim.x * WG) {
@@ -1863,7 +1979,7 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 			ocnt[edge_last_off(P.nb_cap) + i] = 0;
 		}
 	}
-	const uint32_t L = ctl->live[tick & 1];
+	const uint32_t L = H.live(tick);
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
 	unsigned long long acc[NACC] = {0, 0, 0, 0, 0, 0, 0};
 	uint32_t acc_k = NO_REP;   /* replica the registers sum for */
@@ -1899,7 +2015,7 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 		 * before the first tile. */
 		if (nrep == 1) {
 			acc_k = 0;
-			ra = load_ra(T, 0, tick);
+			ra = load_ra(T, 0, tick, P.one_slot);
 		}
 	}
 	wait_global();
@@ -1942,11 +2058,13 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 			}
 			if (tile + 1 < t_end) fetch(tile + 1);
 			overflow |= act_tile<GT, false>(S, tile, cnt, keys, vals, grass_at,
-				act_cnt, T, nrep, tick, L, P.R, k_first, k_last, ra, acc);
+				act_cnt, P.act_l1, T, nrep, tick, L, P.R, k_first, k_last, ra, acc);
 		}
 	}
 	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
-	if (acc_k != NO_REP) wg_add<NACC>(acc, S.red, &T.rctl[T.order[acc_k]].acc[0]);
+	if (acc_k != NO_REP)
+		act_wg_add(acc, S.red,
+			&T.rctl[nrep == 1 ? P.one_slot : T.order[acc_k]].acc[0]);
 }
 
 /* The tiles k_act listed as holding agents of several replicas (at most one
@@ -1972,13 +2090,15 @@ __global__ __launch_bounds__(WG) void k_act_mixed(
 		const uint32_t k_last = rep_index(T, nrep,
 			__builtin_amdgcn_readfirstlane(S.key[cnt]));
 		overflow |= act_tile<GT, true>(S, tile, cnt, keys, vals, grass_at,
-			act_cnt, T, nrep, tick, L, P.R, k_first, k_last, none, acc);
+			act_cnt, P.act_l1, T, nrep, tick, L, P.R, k_first, k_last, none, acc);
 	}
 	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
 }
 
-/* One work-group: exclusive scan of the action tiles' output counts (the
- * output offsets), then the end of tick t for every replica of the batch
+/* One work-group: exclusive scan of the output counts of the action tiles'
+ * blocks of 64 (the block output offsets; a tile's offset adds the counts of
+ * the tiles before it in its block), then the end of tick t for every
+ * replica of the batch
  * (in table order, so that its agents' batch positions are the prefix of the
  * outputs of the replicas before it). Grass statistics come from the
  * regrow-tick encoding without touching the grid: with D dead cells and
@@ -1995,18 +2115,23 @@ __global__ __launch_bounds__(WG) void k_act_mixed(
 __global__ __launch_bounds__(SCAN_WG) void k_act_scan(
 	uint32_t *__restrict__ act_cnt, Ctl *ctl, Tables T, Params P) {
 	__shared__ uint32_t s_tmp[SCAN_WG / WAVE];
-	if (ctl->error) return;
-	const uint32_t tick = ctl->tick;
-	const uint32_t L = ctl->live[tick & 1];
+	const CtlHead H = ctl_head(ctl);
+	if (H.error) return;
+	const uint32_t tick = H.tick;
+	const uint32_t L = H.live(tick);
 	const uint32_t nt = (L + A_TILE - 1) / A_TILE;
-	const uint32_t per = (nt + SCAN_WG - 1) / SCAN_WG;
-	const uint32_t t0 = threadIdx.x * per, t1 = min(t0 + per, nt);
+	/* the sums of blocks of 64 tiles, scanned in place: the emit adds the
+	 * counts of the tiles before its own within the block */
+	uint32_t *blk = act_cnt + P.act_l1;
+	const uint32_t nbk = (nt + 63) / 64;
+	const uint32_t per = (nbk + SCAN_WG - 1) / SCAN_WG;
+	const uint32_t t0 = threadIdx.x * per, t1 = min(t0 + per, nbk);
 	uint32_t sum = 0;
 	for (uint32_t t = t0; t < t1; t += SCAN_UNROLL) {
 		uint32_t v[SCAN_UNROLL];
 #pragma unroll
 		for (int u = 0; u < SCAN_UNROLL; u++)
-			v[u] = t + u < t1 ? act_cnt[t + u] : 0u;
+			v[u] = t + u < t1 ? blk[t + u] : 0u;
 #pragma unroll
 		for (int u = 0; u < SCAN_UNROLL; u++) sum += v[u];
 	}
@@ -2020,17 +2145,17 @@ __global__ __launch_bounds__(SCAN_WG) void k_act_scan(
 		uint32_t v[SCAN_UNROLL];
 #pragma unroll
 		for (int u = 0; u < SCAN_UNROLL; u++)
-			v[u] = t + u < t1 ? act_cnt[t + u] : 0u;
+			v[u] = t + u < t1 ? blk[t + u] : 0u;
 #pragma unroll
 		for (int u = 0; u < SCAN_UNROLL; u++) {
-			if (t + u < t1) act_cnt[t + u] = run;
+			if (t + u < t1) blk[t + u] = run;
 			run += v[u];
 		}
 	}
 	/* The replicas, in table order, by the first wave (no barriers). */
 	if (threadIdx.x >= WAVE) return;
 	const int lane = threadIdx.x;
-	const uint32_t nrep = ctl->nrep;
+	const uint32_t nrep = H.nrep;
 	uint32_t carry = 0;
 	unsigned long long bs[3] = {0, 0, 0};  /* agent-steps, occupied, eaten */
 	for (uint32_t k0 = 0; k0 < nrep; k0 += WAVE) {
@@ -2184,25 +2309,33 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	__shared__ uint32_t s_tmp[NWAVE];
 	__shared__ unsigned long long s_red[NWAVE];
 
-	if (ctl->error) return;
+	const CtlHead H = ctl_head(ctl);
+	if (H.error) return;
 	const int tid = threadIdx.x;
-	const uint32_t tick = ctl->tick;            /* tick being moved for */
-	const uint32_t L = ctl->live[(tick - 1) & 1];
-	const uint32_t nrep = ctl->nrep;
+	const uint32_t tick = H.tick;               /* tick being moved for */
+	const uint32_t L = H.live(tick - 1);
+	const uint32_t nrep = H.nrep;
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
 	const uint32_t S = P.bshift;
 	unsigned long long live = 0;
 	bool overflow = false;
 	/* A batch of one replica (every plain simulation) needs no lookups. */
 	RM m_one{};
-	if (nrep == 1 && ntiles > blockIdx.x) m_one = load_rm_uniform(T, 0, tick, S);
+	if (nrep == 1 && ntiles > blockIdx.x)
+		m_one = load_rm_uniform(T, 0, tick, S, P.one_slot);
 
 	for (uint32_t i = tid; i < 2 * BWIN; i += WG) s_win[i] = 0;
 	for (uint32_t i = tid; i < 2 * L1W; i += WG) s_l1[i] = 0;
 	if (tid < 2 * L2W) s_l2[tid] = 0;
 	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 		const uint32_t base = tile * A_TILE;
-		const uint32_t goff = act_off[tile];
+		/* the tile's output offset: its block's scanned sum plus the counts
+		 * of the block's tiles before it (a wave-wide load and sum, issued
+		 * with the tile's pairs) */
+		const uint32_t tile_lane = tid & (WAVE - 1);
+		const uint32_t before = tile_lane < (tile & 63u)
+			? act_off[(tile & ~63u) + tile_lane] : 0u;
+		const uint32_t goff = act_off[P.act_l1 + (tile >> 6)] + wave_sum32(before);
 		/* the tile's first cell, requested with its pairs (uniform: a
 		 * scalar load) */
 		const uint32_t c_first = __builtin_amdgcn_readfirstlane(keys[base]);
@@ -2310,7 +2443,8 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	}
 	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
 	const int lane = tid & (WAVE - 1), w = tid / WAVE;
-	const unsigned long long tsum = wave_sum(live);
+	/* a thread's survivors stay far below 2^32 (its share of one tick) */
+	const unsigned long long tsum = wave_sum32((uint32_t)live);
 	if (lane == 0) s_red[w] = tsum;
 	__syncthreads();
 	if (tid == 0) {
@@ -2829,10 +2963,13 @@ static int upload_table(pp_batch_t *b) {
 		(b->sort_mode != 1 || nrep <= 1 || nbk <= BUCKET_AUTO_MAX)) ? 1u : 0u;
 	uint32_t top = 0;
 	while (top + 1 < (uint32_t)CNT_LEVELS && (nbk >> (6 * (top + 1))) > 0) top++;
-	if (on != b->P.bucket_on || shift != b->P.bshift || top != b->P.cnt_top) {
+	const uint32_t one = nrep == 1 ? order[0] : 0u;
+	if (on != b->P.bucket_on || shift != b->P.bshift || top != b->P.cnt_top ||
+		one != b->P.one_slot) {
 		b->P.bucket_on = on;
 		b->P.bshift = shift;
 		b->P.cnt_top = top;
+		b->P.one_slot = one;
 		drop_graph(b);     /* Params are launch arguments of the graphs */
 	}
 	const uint32_t nb32 = on ? (uint32_t)nbk : 0u;
@@ -3026,7 +3163,11 @@ static uint64_t batch_bytes(uint64_t A, uint64_t C, uint32_t max_rep,
 	t += C * grass_bytes_for(R);
 	t += (uint64_t)max_rep * (R + 1) * 8;
 	t += hist_words * 4 + RADIX_MAX * 4;
-	t += ((uint64_t)ceil_div(A, A_TILE) + 1) * 8;
+	{
+		/* tile counts and their sums per 64 tiles; the mixed-tile list */
+		const uint64_t tiles = (uint64_t)ceil_div(A, A_TILE) + 1;
+		t += (tiles + (tiles + 63) / 64 + 1) * 4 + tiles * 4;
+	}
 	t += sizeof(Ctl) + (uint64_t)max_rep * (sizeof(Rep) + sizeof(RepCtl) + 8);
 	t += (uint64_t)max_rep * stats_rows * sizeof(pp_stats64_t);
 	return t;
@@ -3060,9 +3201,12 @@ static int batch_alloc(pp_batch_t *b) {
 	b->hist_alloc_words = max_hist_words(b->key_bits, b->P.cap);
 	HIP_OK(hipMalloc(&b->hist, (size_t)b->hist_alloc_words * 4));
 	HIP_OK(hipMalloc(&b->totals, RADIX_MAX * 4));
-	b->act_words = ceil_div(A, A_TILE) + 1;
+	/* tile counts, then the sums of blocks of 64 tiles */
+	const uint32_t act_tiles = ceil_div(A, A_TILE) + 1;
+	b->P.act_l1 = act_tiles;
+	b->act_words = act_tiles + ceil_div(act_tiles, 64) + 1;
 	HIP_OK(hipMalloc(&b->act_cnt, (size_t)b->act_words * 4));
-	HIP_OK(hipMalloc(&b->mixed_list, (size_t)b->act_words * 4));
+	HIP_OK(hipMalloc(&b->mixed_list, (size_t)act_tiles * 4));
 	const size_t nbw = cnt_words(b->P.nb_cap);
 	HIP_OK(hipMalloc(&b->ocnt, nbw * 4));
 	HIP_OK(hipMalloc(&b->ncnt, nbw * 4));
