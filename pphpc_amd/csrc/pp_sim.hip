@@ -130,6 +130,7 @@ struct Rep {
 	uint32_t iters;            /* done after this many iterations */
 	uint32_t cap;              /* agents allowed after births */
 	uint32_t stats_rows;       /* length of its stats ring */
+	uint32_t gx_magic, gx_shift; /* division by gx: see div_by */
 	uint32_t pad;
 	unsigned long long *ring;  /* R + 1 counters of dead cells per regrow tick */
 	pp_stats64_t *stats;       /* stats ring */
@@ -384,6 +385,7 @@ __device__ __forceinline__ void wave_hist_add(uint32_t *s_hist, uint32_t d) {
 /* What the move needs of a replica. */
 struct RM {
 	uint32_t cb, C, gx, gy, sheep_prob, wolves_prob, key0, key1;
+	uint32_t gxm, gxs;  /* division by gx (div_by) */
 	uint32_t step;   /* local iteration the moved agents act in */
 	uint32_t base;   /* batch position of its local agent id 0 */
 	uint32_t bbase;  /* its first bucket */
@@ -404,6 +406,24 @@ __host__ __device__ __forceinline__ uint32_t rows_per_bucket(uint32_t gx,
 	return gx > (1u << S) ? 1u : (1u << S) / gx;
 }
 
+/* Division of any 32-bit n by a fixed d with one high multiply, a subtract
+ * and two shifts (the branch-free unsigned method of the libdivide library:
+ * l = ceil(log2 d), magic = floor(2^32 (2^l - d) / d) + 1, shifts min(l, 1)
+ * and max(l - 1, 0) packed as shift | shift2 << 8). Exact for all n. */
+__host__ __device__ __forceinline__ void div_magic(uint32_t d, uint32_t *magic,
+	uint32_t *shift) {
+	uint32_t l = 0;
+	while (l < 32 && (1ull << l) < d) l++;
+	*magic = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+	*shift = (l ? 1u : 0u) | ((l ? l - 1 : 0u) << 8);
+}
+
+__device__ __forceinline__ uint32_t div_by(uint32_t n, uint32_t magic,
+	uint32_t shift) {
+	const uint32_t t = __umulhi(magic, n);
+	return (t + ((n - t) >> (shift & 0xFFu))) >> (shift >> 8);
+}
+
 /* Bucket of local cell (x, y). */
 __device__ __forceinline__ uint32_t bucket_of(uint32_t bbase, uint32_t nb,
 	uint32_t h, uint32_t x, uint32_t y, uint32_t S) {
@@ -421,6 +441,8 @@ __device__ __forceinline__ RM load_rm(const Tables &T, uint32_t k,
 	m.C = rp.C;
 	m.gx = rp.gx;
 	m.gy = rp.gy;
+	m.gxm = rp.gx_magic;
+	m.gxs = rp.gx_shift;
 	m.bbase = T.bbase[k];
 	m.nb = row_blocks(rp.gx, S);
 	m.h = rows_per_bucket(rp.gx, S);
@@ -443,6 +465,8 @@ __device__ __forceinline__ RM load_rm_uniform(const Tables &T, uint32_t k,
 	m.C = __builtin_amdgcn_readfirstlane(m.C);
 	m.gx = __builtin_amdgcn_readfirstlane(m.gx);
 	m.gy = __builtin_amdgcn_readfirstlane(m.gy);
+	m.gxm = __builtin_amdgcn_readfirstlane(m.gxm);
+	m.gxs = __builtin_amdgcn_readfirstlane(m.gxs);
 	m.sheep_prob = __builtin_amdgcn_readfirstlane(m.sheep_prob);
 	m.wolves_prob = __builtin_amdgcn_readfirstlane(m.wolves_prob);
 	m.key0 = __builtin_amdgcn_readfirstlane(m.key0);
@@ -2262,7 +2286,7 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 			uint32_t mv = 0, mk = DEAD_KEY;
 			const RM m = MIXED ? load_rm(T, rep_index(T, nrep, cc), tick, S) : m_u;
 			const uint32_t l = cc - m.cb;
-			const uint32_t y = l / m.gx, x = l - y * m.gx;
+			const uint32_t y = div_by(l, m.gxm, m.gxs), x = l - y * m.gx;
 			if (m.active)
 				mk = move_agent(id - m.base, x, y, (int32_t)en, t, m, S, &mv, &nbk,
 					overflow);
@@ -3372,6 +3396,7 @@ static int admit(pp_batch_t *b, const pp_params_t *p, uint64_t seed,
 	rp.cb = (uint32_t)cb;
 	rp.C = C;
 	rp.gx = p->grid_x;
+	div_magic(rp.gx, &rp.gx_magic, &rp.gx_shift);
 	rp.gy = p->grid_y;
 	rp.R = p->grass_restart;
 	rp.sheep_gain = p->sheep_gain_from_food;
