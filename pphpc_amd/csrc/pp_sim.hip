@@ -265,17 +265,25 @@ __device__ __forceinline__ void glb_add(unsigned long long *p,
 		__HIP_MEMORY_SCOPE_AGENT);
 }
 
+/* Inclusive prefix sum of a 32-bit value over the lanes of the wave (all
+ * lanes active), with DPP: row shifts by 1, 2, 4 and 8 scan each row of 16
+ * lanes, then row broadcasts of lanes 15 and 31 carry the rows' totals. */
+__device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v) {
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false); /* row_shr:1 */
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false); /* row_shr:2 */
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false); /* row_shr:4 */
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false); /* row_shr:8 */
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false); /* row_bcast:15 */
+	v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false); /* row_bcast:31 */
+	return v;
+}
+
 /* Work-group exclusive scan of one value per thread (NT threads). */
 template <int NT>
 __device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t *s_tmp,
 	uint32_t &total) {
 	const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-	uint32_t x = v;
-#pragma unroll
-	for (int o = 1; o < WAVE; o <<= 1) {
-		const uint32_t t = __shfl_up(x, o);
-		if (lane >= o) x += t;
-	}
+	const uint32_t x = wave_incl_scan32(v);
 	if (lane == WAVE - 1) s_tmp[w] = x;
 	__syncthreads();
 	uint32_t pre = 0, tot = 0;
@@ -1169,8 +1177,7 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 			uint32_t v = 0;
 #pragma unroll
 			for (int k = 0; k < CNT_LEVELS; k++) v += pl[i][k];
-#pragma unroll
-			for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+			v = wave_sum32(v);
 			if (lane == 0) {
 				dst[2 * s] = v;
 				dst[2 * s + 1] = v + pc[i];
@@ -1460,7 +1467,6 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 			if ((uint64_t)c + (uint64_t)RB * stride >= c1) break;
 		}
 	}
-
 }
 
 /* ----------------------------------------------------------------- action */
@@ -2189,14 +2195,9 @@ __global__ __launch_bounds__(SCAN_WG) void k_act_scan(
 			slot = T.order[k];
 			out = (uint32_t)T.rctl[slot].acc[AC_OUT];
 		}
-		uint32_t x = out;
-#pragma unroll
-		for (int o = 1; o < WAVE; o <<= 1) {
-			const uint32_t t = __shfl_up(x, o);
-			if (lane >= o) x += t;
-		}
+		const uint32_t x = wave_incl_scan32(out);
 		const uint32_t pre = x - out;
-		const uint32_t tot = __shfl(x, WAVE - 1);
+		const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)x, WAVE - 1);
 		if (k < nrep) {
 			RepCtl &rc = T.rctl[slot];
 			const Rep &rp = T.rep[slot];
