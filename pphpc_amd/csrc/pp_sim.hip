@@ -278,6 +278,13 @@ __device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v) {
 	return v;
 }
 
+/* The value of the previous lane (lane 0: its own), with DPP wave_shr:1 -
+ * a register move across lanes, no LDS round trip. */
+__device__ __forceinline__ uint32_t prev_lane(uint32_t v) {
+	return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x138, 0xf, 0xf,
+		false);
+}
+
 /* Work-group exclusive scan of one value per thread (NT threads). */
 template <int NT>
 __device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t *s_tmp,
@@ -379,7 +386,7 @@ __device__ __forceinline__ bool grass_alive_at(GT g, uint32_t t, uint32_t Rmax) 
  * digits form long runs). NO_DIGIT lanes count nothing. */
 __device__ __forceinline__ void wave_hist_add(uint32_t *s_hist, uint32_t d) {
 	const int lane = threadIdx.x & (WAVE - 1);
-	const uint32_t prev = __shfl_up(d, 1);
+	const uint32_t prev = prev_lane(d);
 	const bool head = lane == 0 || d != prev;
 	const uint64_t heads = __ballot(head);
 	if (head && d != NO_DIGIT) {
@@ -566,7 +573,7 @@ __device__ __forceinline__ void wave_bucket_add(uint32_t *s_win,
 	uint32_t *s_l1, uint32_t *g_cnt, uint32_t nbc, uint32_t top, uint32_t w0,
 	uint32_t u0, uint32_t bkt) {
 	const int lane = threadIdx.x & (WAVE - 1);
-	const uint32_t prev = __shfl_up(bkt, 1);
+	const uint32_t prev = prev_lane(bkt);
 	const bool head = lane == 0 || bkt != prev;
 	const uint64_t heads = __ballot(head);
 	if (head && bkt != NO_DIGIT) {
