@@ -2149,9 +2149,10 @@ __global__ __launch_bounds__(WG) void k_act_mixed(
  * and the emit drops the agents of both. The batch fails when its outputs
  * exceed the agent buffers. */
 
-__global__ __launch_bounds__(SCAN_WG) void k_act_scan(
+template <int NT>
+__global__ __launch_bounds__(NT) void k_act_scan(
 	uint32_t *__restrict__ act_cnt, Ctl *ctl, Tables T, Params P) {
-	__shared__ uint32_t s_tmp[SCAN_WG / WAVE];
+	__shared__ uint32_t s_tmp[NT / WAVE];
 	const CtlHead H = ctl_head(ctl);
 	if (H.error) return;
 	const uint32_t tick = H.tick;
@@ -2161,33 +2162,49 @@ __global__ __launch_bounds__(SCAN_WG) void k_act_scan(
 	 * counts of the tiles before its own within the block */
 	uint32_t *blk = act_cnt + P.act_l1;
 	const uint32_t nbk = (nt + 63) / 64;
-	const uint32_t per = (nbk + SCAN_WG - 1) / SCAN_WG;
-	const uint32_t t0 = threadIdx.x * per, t1 = min(t0 + per, nbk);
-	uint32_t sum = 0;
-	for (uint32_t t = t0; t < t1; t += SCAN_UNROLL) {
-		uint32_t v[SCAN_UNROLL];
-#pragma unroll
-		for (int u = 0; u < SCAN_UNROLL; u++)
-			v[u] = t + u < t1 ? blk[t + u] : 0u;
-#pragma unroll
-		for (int u = 0; u < SCAN_UNROLL; u++) sum += v[u];
-	}
 	uint32_t total;
-	uint32_t run = wg_excl_scan<SCAN_WG>(sum, s_tmp, total);
+	if (NT == WAVE) {
+		/* few blocks: one wave, 64 at a time, no barriers */
+		const uint32_t lane = threadIdx.x;
+		uint32_t carry = 0;
+		for (uint32_t c0 = 0; c0 < nbk; c0 += WAVE) {
+			const uint32_t q = c0 + lane;
+			const uint32_t v = q < nbk ? blk[q] : 0u;
+			const uint32_t x = wave_incl_scan32(v);
+			if (q < nbk) blk[q] = carry + x - v;
+			carry += (uint32_t)__builtin_amdgcn_readlane((int)x, WAVE - 1);
+		}
+		total = carry;
+	} else {
+		const uint32_t per = (nbk + NT - 1) / NT;
+		const uint32_t t0 = threadIdx.x * per, t1 = min(t0 + per, nbk);
+		uint32_t sum = 0;
+		for (uint32_t t = t0; t < t1; t += SCAN_UNROLL) {
+			uint32_t v[SCAN_UNROLL];
+#pragma unroll
+			for (int u = 0; u < SCAN_UNROLL; u++)
+				v[u] = t + u < t1 ? blk[t + u] : 0u;
+#pragma unroll
+			for (int u = 0; u < SCAN_UNROLL; u++) sum += v[u];
+		}
+		uint32_t run = wg_excl_scan<NT>(sum, s_tmp, total);
+		if (total <= P.cap) {
+			for (uint32_t t = t0; t < t1; t += SCAN_UNROLL) {
+				uint32_t v[SCAN_UNROLL];
+#pragma unroll
+				for (int u = 0; u < SCAN_UNROLL; u++)
+					v[u] = t + u < t1 ? blk[t + u] : 0u;
+#pragma unroll
+				for (int u = 0; u < SCAN_UNROLL; u++) {
+					if (t + u < t1) blk[t + u] = run;
+					run += v[u];
+				}
+			}
+		}
+	}
 	if (total > P.cap) {
 		if (threadIdx.x == 0) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
 		return;
-	}
-	for (uint32_t t = t0; t < t1; t += SCAN_UNROLL) {
-		uint32_t v[SCAN_UNROLL];
-#pragma unroll
-		for (int u = 0; u < SCAN_UNROLL; u++)
-			v[u] = t + u < t1 ? blk[t + u] : 0u;
-#pragma unroll
-		for (int u = 0; u < SCAN_UNROLL; u++) {
-			if (t + u < t1) blk[t + u] = run;
-			run += v[u];
-		}
 	}
 	/* The replicas, in table order, by the first wave (no barriers). */
 	if (threadIdx.x >= WAVE) return;
@@ -2199,7 +2216,7 @@ __global__ __launch_bounds__(SCAN_WG) void k_act_scan(
 		const uint32_t k = k0 + lane;
 		uint32_t slot = 0, out = 0;
 		if (k < nrep) {
-			slot = T.order[k];
+			slot = nrep == 1 ? P.one_slot : T.order[k];
 			out = (uint32_t)T.rctl[slot].acc[AC_OUT];
 		}
 		const uint32_t x = wave_incl_scan32(out);
@@ -2837,8 +2854,13 @@ static int enqueue_step(pp_batch_t *b, int radix) {
 	else launch_act<uint32_t>(b, sorted);
 	prof_end(b, pe);
 	prof_begin(b, KI_ACT_SCAN, pe);
-	k_act_scan<<<1, SCAN_WG, 0, b->stream>>>(b->act_cnt, b->ctl, b->tables(),
-		b->P);
+	/* block sums that one wave scans in at most 8 steps: one wave */
+	if (b->P.act_l1 <= 64u * 64u * 8u)
+		k_act_scan<WAVE><<<1, WAVE, 0, b->stream>>>(b->act_cnt, b->ctl,
+			b->tables(), b->P);
+	else
+		k_act_scan<SCAN_WG><<<1, SCAN_WG, 0, b->stream>>>(b->act_cnt, b->ctl,
+			b->tables(), b->P);
 	prof_end(b, pe);
 	prof_begin(b, KI_EMIT_MOVE, pe);
 	k_emit_move<<<b->grid_emit, WG, 0, b->stream>>>(b->keys[sorted],
