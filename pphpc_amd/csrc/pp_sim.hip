@@ -2376,7 +2376,49 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	for (uint32_t i = tid; i < 2 * BWIN; i += WG) s_win[i] = 0;
 	for (uint32_t i = tid; i < 2 * L1W; i += WG) s_l1[i] = 0;
 	if (tid < 2 * L2W) s_l2[tid] = 0;
-	for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+	/* Flushes the count windows (bases w0, u0): level 0; level 1, summed
+	 * into the level-2 window; levels 2 and 3. The windows persist over a
+	 * group's run of tiles while its buckets stay inside them. */
+	uint32_t w0 = 0, u0 = 0;
+	bool have_win = false;
+	auto flush = [&]() {
+		for (uint32_t i = tid; i < 2 * BWIN; i += WG) {
+			const uint32_t h = s_win[i];
+			if (h) {
+				atomicAdd(&(i < BWIN ? ocnt : ncnt)[w0 + (i % BWIN)], h);
+				s_win[i] = 0;
+			}
+		}
+		if (P.cnt_top >= 1) {
+			for (uint32_t i = tid; i < 2 * L1W; i += WG) {
+				const uint32_t h = s_l1[i];
+				if (h) {
+					const uint32_t k = i / L1W, q1 = u0 + (i % L1W);
+					atomicAdd(&(k ? ncnt : ocnt)[cnt_off(P.nb_cap, 1) + q1], h);
+					if (P.cnt_top >= 2)
+						atomicAdd(&s_l2[k * L2W + (q1 >> 6) - (u0 >> 6)], h);
+					s_l1[i] = 0;
+				}
+			}
+			__syncthreads();
+			if (P.cnt_top >= 2 && tid < 2 * L2W) {
+				const uint32_t h = s_l2[tid];
+				if (h) {
+					const uint32_t k = tid / L2W, q2 = (u0 >> 6) + tid % L2W;
+					uint32_t *cnt = k ? ncnt : ocnt;
+					atomicAdd(&cnt[cnt_off(P.nb_cap, 2) + q2], h);
+					s_l2[tid] = 0;
+				}
+			}
+		}
+		__syncthreads();
+	};
+	/* Each group takes a contiguous run of tiles: their cells, and so their
+	 * buckets, ascend, and the windows are flushed rarely. */
+	const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+	const uint32_t t_beg = blockIdx.x * per;
+	const uint32_t t_end = min(ntiles, t_beg + per);
+	for (uint32_t tile = t_beg; tile < t_end; tile++) {
 		const uint32_t base = tile * A_TILE;
 		/* the tile's output offset: its block's scanned sum plus the counts
 		 * of the block's tiles before it (a wave-wide load and sum, issued
@@ -2438,14 +2480,24 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 			mixed = k_first != k_last;
 			m_t = load_rm_uniform(T, k_first, tick, S);
 		}
-		uint32_t w0 = 0;
+		uint32_t nw0;
 		{
 			const uint32_t l = c_first - m_t.cb;
 			const uint32_t y = l / m_t.gx, x = l - y * m_t.gx;
 			const uint32_t b0 = bucket_of(m_t.bbase, m_t.nb, m_t.h, x, y, S);
-			w0 = b0 > BWIN / 4 ? b0 - BWIN / 4 : 0u;
+			nw0 = b0 > BWIN / 4 ? b0 - BWIN / 4 : 0u;
 		}
-		const uint32_t u0 = (w0 >> 6) > L1W / 2 ? (w0 >> 6) - L1W / 2 : 0u;
+		/* re-centre the windows once the tiles' buckets have moved a
+		 * quarter window up (buckets ascend over the run) */
+		if (P.bucket_on && have_win && nw0 > w0 + BWIN / 4) {
+			flush();
+			have_win = false;
+		}
+		if (!have_win) {
+			w0 = nw0;
+			u0 = (w0 >> 6) > L1W / 2 ? (w0 >> 6) - L1W / 2 : 0u;
+			have_win = true;
+		}
 		if (!mixed)
 			live += emit_outputs<false>(s_oc, s_ov, s_win, s_l1, w0, u0, total, goff, ocell,
 				oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
@@ -2455,41 +2507,8 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 				oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
 				overflow);
 		__syncthreads();
-		if (P.bucket_on) {
-			/* flush the windows: level 0; level 1, summed into the level-2
-			 * window; levels 2 and 3 */
-			for (uint32_t i = tid; i < 2 * BWIN; i += WG) {
-				const uint32_t h = s_win[i];
-				if (h) {
-					atomicAdd(&(i < BWIN ? ocnt : ncnt)[w0 + (i % BWIN)], h);
-					s_win[i] = 0;
-				}
-			}
-			if (P.cnt_top >= 1) {
-				for (uint32_t i = tid; i < 2 * L1W; i += WG) {
-					const uint32_t h = s_l1[i];
-					if (h) {
-						const uint32_t k = i / L1W, q1 = u0 + (i % L1W);
-						atomicAdd(&(k ? ncnt : ocnt)[cnt_off(P.nb_cap, 1) + q1], h);
-						if (P.cnt_top >= 2)
-							atomicAdd(&s_l2[k * L2W + (q1 >> 6) - (u0 >> 6)], h);
-						s_l1[i] = 0;
-					}
-				}
-				__syncthreads();
-				if (P.cnt_top >= 2 && tid < 2 * L2W) {
-					const uint32_t h = s_l2[tid];
-					if (h) {
-						const uint32_t k = tid / L2W, q2 = (u0 >> 6) + tid % L2W;
-						uint32_t *cnt = k ? ncnt : ocnt;
-						atomicAdd(&cnt[cnt_off(P.nb_cap, 2) + q2], h);
-						s_l2[tid] = 0;
-					}
-				}
-			}
-			__syncthreads();
-		}
 	}
+	if (P.bucket_on && have_win) flush();
 	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
 	const int lane = tid & (WAVE - 1), w = tid / WAVE;
 	/* a thread's survivors stay far below 2^32 (its share of one tick) */
