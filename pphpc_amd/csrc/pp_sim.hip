@@ -1019,7 +1019,7 @@ constexpr uint32_t BCAP = 1024;  /* agents of one bucket sorted in LDS */
 constexpr uint32_t NXCD = 8;     /* accelerator dies (XCDs) of the GPU */
 constexpr uint32_t W_MAX = 2048; /* cells of a bucket block: 2^S, S <= 11 */
 static_assert(W_MAX <= 2 * BCAP, "a dense bucket's counters reuse the staging");
-constexpr uint32_t BG_IPT = 8;   /* source positions per thread per round */
+constexpr uint32_t BG_IPT = 6;   /* source positions per thread per round (1536: the three bucket rows of 12800^2) */
 static_assert(NWAVE == 4, "the gather reads the waves' counts as one uint4");
 /* Most buckets the automatic choice sorts by buckets in a batch of several
  * replicas (above, the radix sort is faster there: each bucket costs a few
