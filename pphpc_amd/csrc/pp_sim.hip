@@ -233,6 +233,21 @@ typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 typedef __attribute__((address_space(1))) unsigned long long glb_u64;
 
+/* p[i] through a 32-bit byte offset: agent capacities stay below 2^29, so
+ * the offsets of agent arrays fit 32 bits, and the compiler can use the
+ * scalar base + 32-bit offset form of a global access instead of 64-bit
+ * address arithmetic in every lane. */
+template <typename T>
+__device__ __forceinline__ T &at32(T *p, uint32_t i) {
+	return *reinterpret_cast<T *>(reinterpret_cast<char *>(p) +
+		(uint32_t)(i * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ const T &at32(const T *p, uint32_t i) {
+	return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) +
+		(uint32_t)(i * (uint32_t)sizeof(T)));
+}
+
 /* Waits until every global load and store this wave issued has completed
  * (s_waitcnt vmcnt(0)). Global accesses retire in issue order, so where
  * loads are issued conditionally, or stores follow them, the compiler can
@@ -1290,8 +1305,8 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 					const uint32_t d = e >= cum4 ? d4 : e >= cum3 ? d3
 						: e >= cum2 ? d2 : e >= cum1 ? d1 : d0;
 					const uint32_t p = e + d;
-					kk[jj] = keys[p];
-					vv[jj] = vals[p];
+					kk[jj] = at32(keys, p);
+					vv[jj] = at32(vals, p);
 				}
 			}
 			/* per element: its rank among the wave's landing ones; per
@@ -1382,8 +1397,8 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 				const uint32_t s0 = c ? (s_cnt2[(c - 1) >> 1] >> (((c - 1) & 1u) * 16)) & 0xFFFFu : 0u;
 				uint32_t at = s0;
 				for (uint32_t a = s0; a < e; a++) at += s_ix[a] < i;
-				okeys[out0 + at] = cs + c;
-				ovals[out0 + at] = s_va[i];
+				at32(okeys, out0 + at) = cs + c;
+				at32(ovals, out0 + at) = s_va[i];
 			}
 		} else {
 			/* More agents than LDS holds (a dense bucket): count per cell,
@@ -1950,7 +1965,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 	/* Write back the positions this tile owns (not those of the previous
 	 * tile's last segment, which its lane wrote). */
 	for (uint32_t i = tid; i < cnt; i += WG)
-		if (S.key[1 + i] != S.key[0]) vals[base + i] = S.val[i];
+		if (S.key[1 + i] != S.key[0]) at32(vals, base + i) = S.val[i];
 	uint32_t total;
 	(void)wg_excl_scan<WG>(nout, S.tmp, total);
 	if (tid == 0 && total) tile_add(act_cnt, l1o, tile, total);
@@ -2040,8 +2055,8 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 			pk[j] = DEAD_KEY;
 			pv[j] = 0u;
 			if (i < L) {
-				pk[j] = keys[i];
-				pv[j] = vals[i];
+				pk[j] = at32(keys, i);
+				pv[j] = at32(vals, i);
 			}
 		}
 		if (tid == 0) pprev = base ? keys[base - 1] : DEAD_KEY;
@@ -2305,9 +2320,9 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 			const uint32_t id = goff + i;
 			const uint32_t cc = s_oc[i], ov = s_ov[i];
 			const uint32_t en = ov & V_ENERGY, t = ov >> 31;
-			ocell[id] = cc;
-			oenergy[id] = (int32_t)en;
-			otype[id] = (uint8_t)t;
+			at32(ocell, id) = cc;
+			at32(oenergy, id) = (int32_t)en;
+			at32(otype, id) = (uint8_t)t;
 			uint32_t mv = 0, mk = DEAD_KEY;
 			const RM m = MIXED ? load_rm(T, rep_index(T, nrep, cc), tick, S) : m_u;
 			const uint32_t l = cc - m.cb;
@@ -2322,8 +2337,8 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 				if (x + 1 == min(((x >> S) + 1) << S, m.gx))
 					atomicAdd(&ocnt[edge_last_off(P.nb_cap) + ob], 1u);
 			}
-			mkeys[id] = mk;
-			mvals[id] = mv;
+			at32(mkeys, id) = mk;
+			at32(mvals, id) = mv;
 			live += mk != DEAD_KEY;
 		}
 		if (P.bucket_on) {
@@ -2433,8 +2448,8 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		const uint32_t p0 = base + tid * A_IPT;
 		uint32_t c[A_IPT], v[A_IPT];
 		if (p0 + A_IPT <= L) {
-			const uint4 kv = *reinterpret_cast<const uint4 *>(keys + p0);
-			const uint4 vv = *reinterpret_cast<const uint4 *>(vals + p0);
+			const uint4 kv = at32(reinterpret_cast<const uint4 *>(keys), p0 / 4);
+			const uint4 vv = at32(reinterpret_cast<const uint4 *>(vals), p0 / 4);
 			c[0] = kv.x; c[1] = kv.y; c[2] = kv.z; c[3] = kv.w;
 			v[0] = vv.x; v[1] = vv.y; v[2] = vv.z; v[3] = vv.w;
 		} else {
