@@ -248,6 +248,14 @@ __device__ __forceinline__ const T &at32(const T *p, uint32_t i) {
 		(uint32_t)(i * (uint32_t)sizeof(T)));
 }
 
+/* Grass of global cell c: one-byte grass through a 32-bit byte offset (any
+ * cell index fits), wider grass by plain indexing. */
+template <typename GT>
+__device__ __forceinline__ GT &grass_cell(GT *g, uint32_t c) {
+	if constexpr (sizeof(GT) == 1) return at32(g, c);
+	else return g[c];
+}
+
 /* Waits until every global load and store this wave issued has completed
  * (s_waitcnt vmcnt(0)). Global accesses retire in issue order, so where
  * loads are issued conditionally, or stores follow them, the compiler can
@@ -1706,7 +1714,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 		const uint32_t lp = tid * A_IPT + j;
 		cj[j] = S.key[1 + lp];
 		hj[j] = lp < cnt && S.key[lp] != cj[j];
-		gj[j] = grass_at[hj[j] ? cj[j] : c_safe];
+		gj[j] = grass_cell(grass_at, hj[j] ? cj[j] : c_safe);
 	}
 	/* Cells with one agent (most of them) are handled in place; heads of
 	 * longer segments are listed and then processed by consecutive lanes, so
@@ -1742,7 +1750,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 				thr = ra_u.sheep_thr;
 				if (grass_alive_at<GT>(gj[j], alive_at, Rmax)) {
 					e += (int32_t)ra_u.sheep_gain;
-					grass_at[c] = (GT)(tick + 1 + ra_u.R);
+					grass_cell(grass_at, c) = (GT)(tick + 1 + ra_u.R);
 					ate = 1;
 				}
 			}
@@ -1853,7 +1861,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 				overflow |= e > (int32_t)V_ENERGY;
 				V[q] = nv | ((uint32_t)e & V_ENERGY);
 			}
-			if (ate) grass_at[c] = (GT)(tick + 1 + ra.R);
+			if (ate) grass_cell(grass_at, c) = (GT)(tick + 1 + ra.R);
 			/* per-cell sums fit 32 bits: at most 2 K_REG outputs below
 			 * 2^24 each */
 			uint32_t n_sheep = 0, n_wolves = 0, e_sheep = 0, e_wolves = 0;
@@ -1932,7 +1940,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 				overflow |= e > (int32_t)V_ENERGY;
 				sv.set(q, nv | ((uint32_t)e & V_ENERGY));
 			}
-			if (ate) grass_at[c] = (GT)(tick + 1 + ra.R);
+			if (ate) grass_cell(grass_at, c) = (GT)(tick + 1 + ra.R);
 			sc[AC_EATEN] = ate ? 1 : 0;
 			for (uint32_t q = lp; q < end; q++) {
 				const uint32_t v = sv.get(q);
