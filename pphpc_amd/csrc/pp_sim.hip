@@ -2957,7 +2957,9 @@ static void apply_shape(pp_batch_t *b, const SortShape &sh) {
 	 * tiles of a full-capacity population; all loops are grid-strided. */
 	const uint32_t cu = (uint32_t)b->num_cu;
 	b->grid_up = min_u32(P.hist_cols, 8 * cu);
-	b->grid_down = min_u32(P.hist_cols, (b->rbits == 11 ? 1 : 2) * cu);
+	/* (8-bit digits: three groups a CU, as many as its LDS holds; 11-bit
+	 * digits: one) */
+	b->grid_down = min_u32(P.hist_cols, (b->rbits == 11 ? 1 : 3) * cu);
 	b->grid_emit = min_u32(ceil_div(P.cap, A_TILE),
 		TILE_ROUNDS * resident(b, (const void *)k_emit_move));
 	if (b->grass_bytes == 1) set_grids_gt<uint8_t>(b);
