@@ -89,6 +89,7 @@ constexpr uint32_t STATS_RING = 4096; /* stats rows kept per plain simulation */
 constexpr uint32_t ITERS_UNLIMITED = 0xFFFFFFFFu; /* iterations of a pp_sim */
 constexpr int A_IPT = 4;            /* sorted positions per thread (action) */
 constexpr int A_TILE = WG * A_IPT;
+static_assert(A_IPT == 2 || A_IPT == 4, "the emit loads a thread's pairs as one vector");
 constexpr int A_STAGE = 2 * A_TILE; /* staged outputs per action tile */
 constexpr int MIX_SLOTS = 16;       /* replicas per action tile summed in LDS */
 constexpr uint32_t K_REG = 5;       /* agents of a cell acted on in registers */
@@ -2456,10 +2457,17 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		const uint32_t p0 = base + tid * A_IPT;
 		uint32_t c[A_IPT], v[A_IPT];
 		if (p0 + A_IPT <= L) {
-			const uint4 kv = at32(reinterpret_cast<const uint4 *>(keys), p0 / 4);
-			const uint4 vv = at32(reinterpret_cast<const uint4 *>(vals), p0 / 4);
-			c[0] = kv.x; c[1] = kv.y; c[2] = kv.z; c[3] = kv.w;
-			v[0] = vv.x; v[1] = vv.y; v[2] = vv.z; v[3] = vv.w;
+			if constexpr (A_IPT == 4) {
+				const uint4 kv = at32(reinterpret_cast<const uint4 *>(keys), p0 / 4);
+				const uint4 vv = at32(reinterpret_cast<const uint4 *>(vals), p0 / 4);
+				c[0] = kv.x; c[1] = kv.y; c[2] = kv.z; c[3] = kv.w;
+				v[0] = vv.x; v[1] = vv.y; v[2] = vv.z; v[3] = vv.w;
+			} else {
+				const uint2 kv = at32(reinterpret_cast<const uint2 *>(keys), p0 / 2);
+				const uint2 vv = at32(reinterpret_cast<const uint2 *>(vals), p0 / 2);
+				c[0] = kv.x; c[1] = kv.y;
+				v[0] = vv.x; v[1] = vv.y;
+			}
 		} else {
 #pragma unroll
 			for (int j = 0; j < A_IPT; j++) {
