@@ -210,7 +210,21 @@ struct Params {
 	uint32_t cnt_top;    /* highest level of the bucket count tables in use */
 	uint32_t one_slot;   /* slot of the replica when the table holds one */
 	uint32_t act_l1;     /* offset of the per-64-tile sums in the action counts */
+	unsigned long long *part;  /* partial-sum slots (PART_*) */
 };
+
+/* Partial sums that every work-group of a kernel adds to: PART_SLOTS slots
+ * 128 bytes apart, a group adding to slot (group index mod PART_SLOTS).
+ * Atomic adds of many work-groups to one address serialise (about 11 ns each
+ * on MI355X: 600 groups adding to one counter lengthen a kernel by about
+ * 6 us); spread over 64 slots they cost nothing measurable. Words
+ * [0, NACC) of a slot: the action counters of a one-replica batch, summed by
+ * k_act_scan; word PART_LIVE: the move's survivors, summed into Ctl::live by
+ * the next tick's first kernel. Both consumers clear what they sum. */
+constexpr uint32_t PART_SLOTS = 64, PART_STRIDE = 16, PART_LIVE = 8;
+static_assert(PART_SLOTS == (uint32_t)WAVE, "one wave sums the slots");
+static_assert(NACC <= (int)PART_LIVE && PART_LIVE < PART_STRIDE,
+	"slot layout");
 
 /* Replica tables in device memory: constants and control words by slot, and
  * the replicas of the batch sorted by first cell (cbs ascending, order[k] =
@@ -333,6 +347,19 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 #pragma unroll
 	for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
 	return v;
+}
+
+/* Adds the survivors of the last move, which the emit's work-groups left in
+ * the partial-sum slots, to live[tick & 1], and clears the slots. Called by
+ * the first wave of one work-group of a tick's first kernel (no other
+ * work-group writes Ctl::live then, and the kernels that read it follow). */
+__device__ __forceinline__ void fold_live(Ctl *ctl, unsigned long long *part,
+	uint32_t tick) {
+	const int lane = threadIdx.x;
+	unsigned long long &p = part[lane * PART_STRIDE + PART_LIVE];
+	const unsigned long long s = wave_sum(p);
+	p = 0;      /* issued after the load returned: s depends on it */
+	if (lane == 0 && s) ctl->live[tick & 1] += (uint32_t)s;
 }
 
 /* Sum of a 32-bit value over the 64 lanes of the wave (all lanes active),
@@ -623,11 +650,13 @@ __device__ __forceinline__ void wave_bucket_add(uint32_t *s_win,
 /* ------------------------------------------------------------------- init */
 
 /* Empties the batch: no agents, no replicas, tick 0, the pass-0 histograms
- * zeroed. */
+ * and the partial-sum slots zeroed. */
 __global__ __launch_bounds__(WG) void k_batch_reset(Ctl *ctl, uint32_t *hist,
-	uint32_t hist_words) {
+	uint32_t hist_words, unsigned long long *part) {
 	const uint32_t gtid = blockIdx.x * WG + threadIdx.x;
 	for (uint32_t i = gtid; i < hist_words; i += gridDim.x * WG) hist[i] = 0;
+	for (uint32_t i = gtid; i < PART_SLOTS * PART_STRIDE; i += gridDim.x * WG)
+		part[i] = 0;
 	if (gtid == 0) memset(ctl, 0, sizeof(Ctl));
 }
 
@@ -702,12 +731,14 @@ __global__ __launch_bounds__(WG) void k_init_agents(uint32_t *cell,
 	wg_add<4>(v, s_red, &T.rctl[slot].acc[0]);
 }
 
-__device__ void write_row(const RepCtl &rc, const Rep &rp, pp_stats64_t *row) {
-	row->sheep = rc.acc[AC_SHEEP];
-	row->wolves = rc.acc[AC_WOLVES];
+/* One statistics row from the counters acc (AC_*) and the grass state. */
+__device__ void write_row(const unsigned long long *acc, const RepCtl &rc,
+	const Rep &rp, pp_stats64_t *row) {
+	row->sheep = acc[AC_SHEEP];
+	row->wolves = acc[AC_WOLVES];
 	row->grass = (unsigned long long)rp.C - rc.dead_cells;
-	row->sheep_en = rc.acc[AC_SHEEP_EN];
-	row->wolves_en = rc.acc[AC_WOLVES_EN];
+	row->sheep_en = acc[AC_SHEEP_EN];
+	row->wolves_en = acc[AC_WOLVES_EN];
 	row->grass_en = rc.cd_sum;
 	row->errors = 0;
 }
@@ -719,7 +750,7 @@ __global__ void k_rep_init_done(Ctl *ctl, Tables T, uint32_t slot,
 	if (threadIdx.x != 0 || blockIdx.x != 0) return;
 	const Rep &rp = T.rep[slot];
 	RepCtl &rc = T.rctl[slot];
-	write_row(rc, rp, &rp.stats[0]);
+	write_row(rc.acc, rc, rp, &rp.stats[0]);
 	for (int i = 0; i < NACC; i++) rc.acc[i] = 0;
 	rc.n = rp.iters ? rp.n0 : 0;
 	rc.base = pos;
@@ -802,10 +833,14 @@ __device__ __forceinline__ uint32_t pass_len(const Ctl *ctl, uint32_t pass) {
 
 template <int KPT>
 __global__ __launch_bounds__(WG) void k_sort_up(const uint32_t *__restrict__ kin,
-	uint32_t *__restrict__ hist, const Ctl *ctl, uint32_t pass, Params P) {
+	uint32_t *__restrict__ hist, Ctl *ctl, uint32_t pass, Params P) {
 	constexpr uint32_t TILE = WG * KPT;
 	__shared__ uint32_t s_hist[RADIX_MAX];
 	if (ctl->error) return;
+	/* the tick's first kernel: the last move's survivors into live, which
+	 * the later passes read (pass 0 sorts all n positions) */
+	if (pass == 0 && blockIdx.x == 0 && threadIdx.x < WAVE)
+		fold_live(ctl, P.part, ctl->tick);
 	const uint32_t n = pass_len(ctl, pass);
 	const uint32_t ntiles = (n + TILE - 1) / TILE;
 	const uint32_t shift = pass * P.digit_bits;
@@ -1102,7 +1137,10 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	if (H.error) return;
 	const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
 	const uint32_t tick = H.tick;
-	/* The tick's first kernel: clear the action tiles' counters */
+	/* The tick's first kernel: the last move's survivors into live (by the
+	 * last work-group, which at small sizes has no bucket), and clear the
+	 * action tiles' counters */
+	if (blockIdx.x == gridDim.x - 1 && tid < WAVE) fold_live(ctl, P.part, tick);
 	for (uint32_t i = blockIdx.x * WG + tid; i < act_words; i += gridDim.x * WG)
 		act_cnt[i] = 0;
 	const uint32_t NB = H.nbuckets, nrep = H.nrep, S = P.bshift;
@@ -1631,7 +1669,8 @@ __device__ __forceinline__ void tile_add(uint32_t *act_cnt, uint32_t l1o,
 	atomicAdd(&act_cnt[l1o + (t >> 6)], v);
 }
 
-/* Work-group sums of the action counters, added to dst (a replica's acc):
+/* Work-group sums of the action counters, added to dst (a replica's acc, or
+ * a partial-sum slot):
  * AC_OUT is the sheep plus the wolves emitted; counts are summed in 32 bits
  * and each energy sum as two 32-bit pieces split at bit 24 (exact while
  * every lane's sum is below 2^48; a wave holding a larger one sums it in 64
@@ -2123,9 +2162,11 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 		}
 	}
 	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
+	/* one replica: a partial-sum slot (k_act_scan adds them up) */
 	if (acc_k != NO_REP)
-		act_wg_add(acc, S.red,
-			&T.rctl[nrep == 1 ? P.one_slot : T.order[acc_k]].acc[0]);
+		act_wg_add(acc, S.red, nrep == 1
+			? P.part + (blockIdx.x % PART_SLOTS) * PART_STRIDE
+			: &T.rctl[T.order[acc_k]].acc[0]);
 }
 
 /* The tiles k_act listed as holding agents of several replicas (at most one
@@ -2177,6 +2218,14 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_act_scan(
 	uint32_t *__restrict__ act_cnt, Ctl *ctl, Tables T, Params P) {
 	__shared__ uint32_t s_tmp[NT / WAVE];
+	/* the action counters a one-replica batch left in the partial-sum
+	 * slots, requested with the control words */
+	unsigned long long pv[NACC];
+	if (threadIdx.x < WAVE) {
+		const unsigned long long *q = P.part + threadIdx.x * PART_STRIDE;
+#pragma unroll
+		for (int i = 0; i < NACC; i++) pv[i] = q[i];
+	}
 	const CtlHead H = ctl_head(ctl);
 	if (H.error) return;
 	const uint32_t tick = H.tick;
@@ -2234,14 +2283,28 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 	if (threadIdx.x >= WAVE) return;
 	const int lane = threadIdx.x;
 	const uint32_t nrep = H.nrep;
+	unsigned long long psum[NACC];
+#pragma unroll
+	for (int i = 0; i < NACC; i++) psum[i] = 0;
+	if (nrep == 1) {
+#pragma unroll
+		for (int i = 0; i < NACC; i++) psum[i] = wave_sum(pv[i]);
+		/* cleared after the loads returned (the sums depend on them) */
+		unsigned long long *q = P.part + lane * PART_STRIDE;
+#pragma unroll
+		for (int i = 0; i < NACC; i++) q[i] = 0;
+	}
 	uint32_t carry = 0;
 	unsigned long long bs[3] = {0, 0, 0};  /* agent-steps, occupied, eaten */
 	for (uint32_t k0 = 0; k0 < nrep; k0 += WAVE) {
 		const uint32_t k = k0 + lane;
 		uint32_t slot = 0, out = 0;
+		unsigned long long a[NACC];
 		if (k < nrep) {
 			slot = nrep == 1 ? P.one_slot : T.order[k];
-			out = (uint32_t)T.rctl[slot].acc[AC_OUT];
+#pragma unroll
+			for (int i = 0; i < NACC; i++) a[i] = T.rctl[slot].acc[i] + psum[i];
+			out = (uint32_t)a[AC_OUT];
 		}
 		const uint32_t x = wave_incl_scan32(out);
 		const uint32_t pre = x - out;
@@ -2264,15 +2327,15 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 					const uint32_t expire = (tick + 1) % M;
 					D -= rp.ring[expire];
 					rp.ring[expire] = 0;
-					const unsigned long long E = rc.acc[AC_EATEN];
+					const unsigned long long E = a[AC_EATEN];
 					rp.ring[tick % M] += E;
 					D += E;
 					S += E * (unsigned long long)rp.R;
 					rc.dead_cells = D;
 					rc.cd_sum = S;
-					write_row(rc, rp, &rp.stats[(s + 1) % rp.stats_rows]);
+					write_row(a, rc, rp, &rp.stats[(s + 1) % rp.stats_rows]);
 					bs[0] += rc.n;
-					bs[1] += rc.acc[AC_OCC];
+					bs[1] += a[AC_OCC];
 					bs[2] += E;
 					rc.agent_steps += rc.n;
 					rc.step = s + 1;
@@ -2546,10 +2609,13 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	const unsigned long long tsum = wave_sum32((uint32_t)live);
 	if (lane == 0) s_red[w] = tsum;
 	__syncthreads();
+	/* the next tick's first kernel adds up the slots into live[tick & 1] */
 	if (tid == 0) {
 		unsigned long long sum = 0;
 		for (int i = 0; i < NWAVE; i++) sum += s_red[i];
-		if (sum) atomicAdd(&ctl->live[tick & 1], (uint32_t)sum);
+		if (sum)
+			atomicAdd(&P.part[(blockIdx.x % PART_SLOTS) * PART_STRIDE + PART_LIVE],
+				sum);
 	}
 }
 
@@ -3241,6 +3307,7 @@ static void batch_free(pp_batch_t *b) {
 	(void)hipFree(b->ncnt);
 	(void)hipFree(b->bbase_dev);
 	(void)hipFree(b->ctl);
+	(void)hipFree(b->P.part);
 	(void)hipFree(b->rep_dev);
 	(void)hipFree(b->rctl_dev);
 	(void)hipFree(b->cbs_dev);
@@ -3274,7 +3341,7 @@ static uint64_t batch_bytes(uint64_t A, uint64_t C, uint32_t max_rep,
 		const uint64_t tiles = (uint64_t)ceil_div(A, A_TILE) + 1;
 		t += (tiles + (tiles + 63) / 64 + 1) * 4 + tiles * 4;
 	}
-	t += sizeof(Ctl) + (uint64_t)max_rep * (sizeof(Rep) + sizeof(RepCtl) + 8);
+	t += sizeof(Ctl) + PART_SLOTS * PART_STRIDE * 8 + (uint64_t)max_rep * (sizeof(Rep) + sizeof(RepCtl) + 8);
 	t += (uint64_t)max_rep * stats_rows * sizeof(pp_stats64_t);
 	return t;
 }
@@ -3320,6 +3387,8 @@ static int batch_alloc(pp_batch_t *b) {
 	HIP_OK(hipMemset(b->ncnt, 0, nbw * 4));
 	HIP_OK(hipMalloc(&b->bbase_dev, ((size_t)b->max_rep + 1) * 4));
 	HIP_OK(hipMalloc(&b->ctl, sizeof(Ctl)));
+	HIP_OK(hipMalloc(&b->P.part, PART_SLOTS * PART_STRIDE * 8));
+	HIP_OK(hipMemset(b->P.part, 0, PART_SLOTS * PART_STRIDE * 8));
 	HIP_OK(hipMalloc(&b->rep_dev, (size_t)b->max_rep * sizeof(Rep)));
 	HIP_OK(hipMalloc(&b->rctl_dev, (size_t)b->max_rep * sizeof(RepCtl)));
 	HIP_OK(hipMalloc(&b->cbs_dev, (size_t)b->max_rep * 4));
@@ -3412,7 +3481,7 @@ static int batch_reset(pp_batch_t *b) {
 	PendingEvent pe;
 	prof_begin(b, KI_INIT_RESET, pe);
 	k_batch_reset<<<min_u32(ceil_div(b->hist_alloc_words, WG), 2048), WG, 0,
-		b->stream>>>(b->ctl, b->hist, (uint32_t)b->hist_alloc_words);
+		b->stream>>>(b->ctl, b->hist, (uint32_t)b->hist_alloc_words, b->P.part);
 	prof_end(b, pe);
 	HIP_OK(hipGetLastError());
 	HIP_OK(hipMemsetAsync(b->rctl_dev, 0, (size_t)b->max_rep * sizeof(RepCtl),
