@@ -100,8 +100,9 @@ def kernel_bytes(name, c, passes, radix_all=False):
       act          read L sorted pairs, write the L updated values back, read
                    the grass of each occupied cell, write each eaten one:
                    12L + 4O + 4E
-      emit_move    read L sorted pairs, write the new state (9 B) and its
-                   moved pair (8 B) per agent: 8L + 17N'
+      emit_move    read L sorted pairs, write each output's moved pair
+                   (8 B): 8L + 8N' (the per-agent state, 9 B an output, is
+                   written only in the last iteration of a run)
     The radix kernels get a count only when every iteration of the run
     sorted by radix (radix_all); otherwise they ran only in the first
     iteration after a change of the replicas.
@@ -117,7 +118,7 @@ def kernel_bytes(name, c, passes, radix_all=False):
     if name == "act":
         return 12 * L + 4 * O + 4 * E
     if name == "emit_move":
-        return 8 * L + 17 * Np
+        return 8 * L + 8 * Np
     return None
 
 

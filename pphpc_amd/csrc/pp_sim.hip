@@ -162,6 +162,9 @@ struct Ctl {
 	uint32_t nmixed;   /* action tiles listed for k_act_mixed this tick */
 	uint32_t err_step; /* tick at which error was set */
 	int32_t err_rep;   /* slot of the replica involved, -1 for the batch */
+	/* the emit that runs with this tick (the last of a run of ticks) writes
+	 * the SoA state; only the host reads it */
+	uint32_t state_tick;
 	/* sums over completed ticks of: agents at the start, alive after the
 	 * move, after actions, occupied cells, grass eaten */
 	unsigned long long agent_steps, sum_live, sum_out, sum_occ, sum_eaten;
@@ -658,6 +661,13 @@ __global__ __launch_bounds__(WG) void k_batch_reset(Ctl *ctl, uint32_t *hist,
 	for (uint32_t i = gtid; i < PART_SLOTS * PART_STRIDE; i += gridDim.x * WG)
 		part[i] = 0;
 	if (gtid == 0) memset(ctl, 0, sizeof(Ctl));
+}
+
+/* Marks the last of the next `ticks` ticks: its emit writes the SoA state
+ * (the other emits skip it; no kernel reads it, the host does after a run). */
+__global__ void k_state_tick(Ctl *ctl, uint32_t ticks) {
+	if (threadIdx.x == 0 && blockIdx.x == 0 && !ctl->error)
+		ctl->state_tick = ctl->tick + ticks;
 }
 
 /* Clears one replica slot's control words and grass ring. */
@@ -2368,7 +2378,8 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 constexpr uint32_t BWIN = 128;   /* buckets per LDS count window */
 
 /* Writes the staged outputs [0, total) of an action tile (batch positions
- * goff + i) to the SoA state, moves each for the next iteration, and counts
+ * goff + i) to the SoA state (when keep: at the last tick of a run, since
+ * no kernel reads it), moves each for the next iteration, and counts
  * each output in the bucket of its cell (ocnt: the order the next sort reads
  * them in) and each survivor of the move in the bucket of its new cell
  * (ncnt), through LDS windows of BWIN buckets from w0. MIXED = false: every
@@ -2382,7 +2393,7 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 	int32_t *__restrict__ oenergy, uint8_t *__restrict__ otype,
 	uint32_t *__restrict__ mkeys, uint32_t *__restrict__ mvals,
 	uint32_t *ocnt, uint32_t *ncnt, const Tables &T, uint32_t nrep,
-	uint32_t tick, const RM &m_u, const Params &P, bool &overflow) {
+	uint32_t tick, const RM &m_u, const Params &P, bool keep, bool &overflow) {
 	unsigned long long live = 0;
 	const uint32_t S = P.bshift;
 	for (uint32_t i0 = 0; i0 < total; i0 += WG) {
@@ -2392,9 +2403,11 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 			const uint32_t id = goff + i;
 			const uint32_t cc = s_oc[i], ov = s_ov[i];
 			const uint32_t en = ov & V_ENERGY, t = ov >> 31;
-			at32(ocell, id) = cc;
-			at32(oenergy, id) = (int32_t)en;
-			at32(otype, id) = (uint8_t)t;
+			if (keep) {
+				at32(ocell, id) = cc;
+				at32(oenergy, id) = (int32_t)en;
+				at32(otype, id) = (uint8_t)t;
+			}
 			uint32_t mv = 0, mk = DEAD_KEY;
 			const RM m = MIXED ? load_rm(T, rep_index(T, nrep, cc), tick, S) : m_u;
 			const uint32_t l = cc - m.cb;
@@ -2446,9 +2459,11 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	__shared__ unsigned long long s_red[NWAVE];
 
 	const CtlHead H = ctl_head(ctl);
+	const uint32_t state_tick = ctl->state_tick;
 	if (H.error) return;
 	const int tid = threadIdx.x;
 	const uint32_t tick = H.tick;               /* tick being moved for */
+	const bool keep = tick == state_tick;
 	const uint32_t L = H.live(tick - 1);
 	const uint32_t nrep = H.nrep;
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
@@ -2595,11 +2610,11 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		if (!mixed)
 			live += emit_outputs<false>(s_oc, s_ov, s_win, s_l1, w0, u0, total, goff, ocell,
 				oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
-				overflow);
+				keep, overflow);
 		else
 			live += emit_outputs<true>(s_oc, s_ov, s_win, s_l1, w0, u0, total, goff, ocell,
 				oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
-				overflow);
+				keep, overflow);
 		__syncthreads();
 	}
 	if (P.bucket_on && have_win) flush();
@@ -3064,12 +3079,17 @@ static int build_graph(pp_batch_t *b, int steps, int radix,
 
 /* Enqueue `ticks` ticks: after a change of the replica table the first one
  * sorts with the radix sort (the moved order is not the cell order yet, or
- * the bucket layout changed); then whole graphs of 8, then single launches. */
+ * the bucket layout changed); then whole graphs of 8, then single launches.
+ * The last tick's emit writes the SoA state that state_of reads. */
 static int enqueue_ticks(pp_batch_t *b, uint32_t ticks) {
 	b->ctl_fresh = 0;
 	const int use_graph = !b->profiling && getenv("PPHPC_NO_GRAPH") == nullptr;
 	const int radix = !b->P.bucket_on;
 	uint32_t i = 0;
+	if (ticks) {
+		k_state_tick<<<1, WAVE, 0, b->stream>>>(b->ctl, ticks);
+		HIP_OK(hipGetLastError());
+	}
 	if (ticks && b->need_radix && !radix) {
 		PP_TRY(enqueue_step(b, 1));
 		i = 1;

@@ -219,13 +219,25 @@ def test_sort_variants_agree(shift):
 
 
 def test_step_by_step_equals_run():
+    """Rows and final agent state agree whether 50 iterations run as one
+    call (graphs of 8 and single ticks; only the last tick writes the
+    per-agent state) or one at a time (every tick writes it)."""
     _case("step vs run")
     d = v2s(128)
     sim = _sim(d, seed=8)
     a = _rows(sim, 50)
+    sa = sim.state()
     sim.init()
     b = [r.as_tuple() for r in [sim.step() for _ in range(50)]]
     assert np.array_equal(a[1:], np.array(b, np.uint64))
+    assert_state_equal(sim.state(), sa, "step vs run 50")
+    sim.run(13)
+    from oracle.pp_oracle import Oracle
+    o = Oracle(d, seed=8, capacity=sim.capacity)
+    rc, _ = o.run(63)
+    assert rc == 0
+    assert_state_equal(sim.state(), o.state(), "run 13 after 50 steps")
+    o.close()
     sim.close()
 
 
