@@ -741,16 +741,21 @@ __global__ __launch_bounds__(WG) void k_init_agents(uint32_t *cell,
 	wg_add<4>(v, s_red, &T.rctl[slot].acc[0]);
 }
 
-/* One statistics row from the counters acc (AC_*) and the grass state. */
-__device__ void write_row(const unsigned long long *acc, const RepCtl &rc,
-	const Rep &rp, pp_stats64_t *row) {
+/* One statistics row from the counters acc (AC_*), the live grass cells
+ * and the countdown sum. */
+__device__ __forceinline__ void write_row(const unsigned long long *acc,
+	unsigned long long grass, unsigned long long grass_en, pp_stats64_t *row) {
 	row->sheep = acc[AC_SHEEP];
 	row->wolves = acc[AC_WOLVES];
-	row->grass = (unsigned long long)rp.C - rc.dead_cells;
+	row->grass = grass;
 	row->sheep_en = acc[AC_SHEEP_EN];
 	row->wolves_en = acc[AC_WOLVES_EN];
-	row->grass_en = rc.cd_sum;
+	row->grass_en = grass_en;
 	row->errors = 0;
+}
+__device__ void write_row(const unsigned long long *acc, const RepCtl &rc,
+	const Rep &rp, pp_stats64_t *row) {
+	write_row(acc, (unsigned long long)rp.C - rc.dead_cells, rc.cd_sum, row);
 }
 
 /* Row 0 of one replica placed at batch position pos = ctl->n; its agents join
@@ -2228,19 +2233,33 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_act_scan(
 	uint32_t *__restrict__ act_cnt, Ctl *ctl, Tables T, Params P) {
 	__shared__ uint32_t s_tmp[NT / WAVE];
-	/* the action counters a one-replica batch left in the partial-sum
-	 * slots, requested with the control words */
+	/* A batch of one replica (every plain simulation; its slot is a launch
+	 * parameter): the action counters it left in the partial-sum slots and
+	 * its control words and constants are requested with the batch's
+	 * control words, its grass ring entries as soon as the tick is known,
+	 * so that its end of tick waits on no further round trip. */
 	unsigned long long pv[NACC];
 	if (threadIdx.x < WAVE) {
 		const unsigned long long *q = P.part + threadIdx.x * PART_STRIDE;
 #pragma unroll
 		for (int i = 0; i < NACC; i++) pv[i] = q[i];
 	}
+	const RepCtl rc1 = T.rctl[P.one_slot];
+	const Rep rp1 = T.rep[P.one_slot];
 	const CtlHead H = ctl_head(ctl);
 	if (H.error) return;
 	const uint32_t tick = H.tick;
 	const uint32_t L = H.live(tick);
 	const uint32_t nt = (L + A_TILE - 1) / A_TILE;
+	uint32_t r_exp = 0, r_cur = 0;
+	unsigned long long ring_exp = 0, ring_cur = 0;
+	if (H.nrep == 1) {
+		const uint32_t M = rp1.R + 1;
+		r_exp = (tick + 1) % M;
+		r_cur = tick % M;
+		ring_exp = rp1.ring[r_exp];
+		ring_cur = rp1.ring[r_cur];
+	}
 	/* the sums of blocks of 64 tiles, scanned in place: the emit adds the
 	 * counts of the tiles before its own within the block */
 	uint32_t *blk = act_cnt + P.act_l1;
@@ -2293,27 +2312,65 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 	if (threadIdx.x >= WAVE) return;
 	const int lane = threadIdx.x;
 	const uint32_t nrep = H.nrep;
-	unsigned long long psum[NACC];
-#pragma unroll
-	for (int i = 0; i < NACC; i++) psum[i] = 0;
+	unsigned long long bs[3] = {0, 0, 0};  /* agent-steps, occupied, eaten */
 	if (nrep == 1) {
+		/* the general loop below for k = 0, from the requested values */
+		unsigned long long a[NACC];
 #pragma unroll
-		for (int i = 0; i < NACC; i++) psum[i] = wave_sum(pv[i]);
+		for (int i = 0; i < NACC; i++) a[i] = rc1.acc[i] + wave_sum(pv[i]);
 		/* cleared after the loads returned (the sums depend on them) */
 		unsigned long long *q = P.part + lane * PART_STRIDE;
 #pragma unroll
 		for (int i = 0; i < NACC; i++) q[i] = 0;
+		const uint32_t out = (uint32_t)a[AC_OUT];
+		if (lane == 0) {
+			RepCtl &rc = T.rctl[P.one_slot];
+			rc.base = 0;
+			if (rc1.status == RS_ACTIVE) {
+				const uint32_t s = tick - rp1.t0;
+				if (out > rp1.cap) {
+					rc.status = RS_FAILED;
+					rc.error = PP_OUT_OF_RESOURCES;
+					rc.err_step = s;
+					rc.n = 0;
+				} else {
+					unsigned long long D = rc1.dead_cells, S = rc1.cd_sum;
+					S -= D;
+					D -= ring_exp;
+					rp1.ring[r_exp] = 0;
+					const unsigned long long E = a[AC_EATEN];
+					rp1.ring[r_cur] = ring_cur + E;
+					D += E;
+					S += E * (unsigned long long)rp1.R;
+					rc.dead_cells = D;
+					rc.cd_sum = S;
+					write_row(a, (unsigned long long)rp1.C - D, S,
+						&rp1.stats[(s + 1) % rp1.stats_rows]);
+					bs[0] = rc1.n;
+					bs[1] = a[AC_OCC];
+					bs[2] = E;
+					rc.agent_steps = rc1.agent_steps + rc1.n;
+					rc.step = s + 1;
+					rc.n = out;
+					if (s + 1 >= rp1.iters) {
+						rc.status = RS_DONE;
+						rc.n = 0;
+					}
+				}
+			}
+#pragma unroll
+			for (int i = 0; i < NACC; i++) rc.acc[i] = 0;
+		}
 	}
 	uint32_t carry = 0;
-	unsigned long long bs[3] = {0, 0, 0};  /* agent-steps, occupied, eaten */
-	for (uint32_t k0 = 0; k0 < nrep; k0 += WAVE) {
+	for (uint32_t k0 = 0; nrep > 1 && k0 < nrep; k0 += WAVE) {
 		const uint32_t k = k0 + lane;
 		uint32_t slot = 0, out = 0;
 		unsigned long long a[NACC];
 		if (k < nrep) {
-			slot = nrep == 1 ? P.one_slot : T.order[k];
+			slot = T.order[k];
 #pragma unroll
-			for (int i = 0; i < NACC; i++) a[i] = T.rctl[slot].acc[i] + psum[i];
+			for (int i = 0; i < NACC; i++) a[i] = T.rctl[slot].acc[i];
 			out = (uint32_t)a[AC_OUT];
 		}
 		const uint32_t x = wave_incl_scan32(out);
