@@ -2246,6 +2246,12 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 	}
 	const RepCtl rc1 = T.rctl[P.one_slot];
 	const Rep rp1 = T.rep[P.one_slot];
+	/* the one-wave form: the first 64 block sums (of those allocated),
+	 * requested with the control words too */
+	uint32_t *blk = act_cnt + P.act_l1;
+	uint32_t v_first = 0;
+	if (NT == WAVE && threadIdx.x < (P.act_l1 + 63) / 64 + 1)
+		v_first = blk[threadIdx.x];
 	const CtlHead H = ctl_head(ctl);
 	if (H.error) return;
 	const uint32_t tick = H.tick;
@@ -2262,7 +2268,6 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 	}
 	/* the sums of blocks of 64 tiles, scanned in place: the emit adds the
 	 * counts of the tiles before its own within the block */
-	uint32_t *blk = act_cnt + P.act_l1;
 	const uint32_t nbk = (nt + 63) / 64;
 	uint32_t total;
 	if (NT == WAVE) {
@@ -2271,7 +2276,7 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 		uint32_t carry = 0;
 		for (uint32_t c0 = 0; c0 < nbk; c0 += WAVE) {
 			const uint32_t q = c0 + lane;
-			const uint32_t v = q < nbk ? blk[q] : 0u;
+			const uint32_t v = q < nbk ? (c0 == 0 ? v_first : blk[q]) : 0u;
 			const uint32_t x = wave_incl_scan32(v);
 			if (q < nbk) blk[q] = carry + x - v;
 			carry += (uint32_t)__builtin_amdgcn_readlane((int)x, WAVE - 1);
