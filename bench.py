@@ -25,6 +25,9 @@ Also reported (one JSON line on stdout, progress on stderr):
                 reference's CPU simulator) on this host's cores, and the
                 single-threaded C oracle: the OpenMP one over the whole
                 workload, the oracle over its first iterations
+  batched_1600  N (8 and 30) replicas of the 1600x1600 workload (seeds 1..N)
+                as one batch against the same N run one at a time as plain
+                simulations: agent-steps/s and replicas/hour both ways
   batched       64 replicas of grid sides drawn from {100 .. 3200}, v1 or v2s
                 parameters 50/50, seeds 1..64, 4000 iterations each: one
                 batch (one set of launches per iteration, sort shape from the
@@ -87,28 +90,32 @@ WORKLOADS = {
 
 
 def kernel_bytes(name, c, passes, radix_all=False):
-    """Algorithmic (compulsory) HBM bytes of all launches of one kernel over
-    a run of iterations, from the exact event sums c of those iterations
-    (agent-steps N, live after move L, agents after actions N', occupied
-    cells O, grass eaten E):
-      bucket_sort  read the N moved pairs (8 B), write the L survivors in
-                   cell order: 8N + 8L
+    """Algorithmic bytes of all launches of one kernel over a run of
+    iterations, from the exact event sums c of those iterations (N agents at
+    the start of each iteration, L alive after the move, N' after actions, O
+    occupied cells, E grass cells eaten, B births), with the per-unit figures
+    of the step's byte model (a 9-byte agent record: cell, energy, type):
+      act          the actions: read each agent and write its acted half, a
+                   cell-index and sheep-list read per agent (22 B per agent),
+                   read and write each occupied cell's grass (8 B), write each
+                   child (9 B): 22N + 8O + 9B
+      emit_move    compaction and spawn of this iteration (19 B per agent, 9
+                   per child) and the move of the next (read and write cell,
+                   energy and flags of each agent of the new state, 18 B):
+                   19N + 9B + 18N'
+      bucket_sort  one read/write pass of the 8-byte (cell, value) records:
+                   read the N moved records, write the L survivors in cell
+                   order: 8N + 8L
       sort_up      radix pass 0 reads the N moved keys, later passes the L
                    survivors' keys: 4N + 4L(P-1)
       sort_down    pass 0 reads the N moved pairs and writes the L survivors,
                    later passes read and write L pairs: 8N + 8L + 16L(P-1)
-      act          read L sorted pairs, write the L updated values back, read
-                   the grass of each occupied cell, write each eaten one:
-                   12L + 4O + 4E
-      emit_move    read L sorted pairs, write each output's moved pair
-                   (8 B): 8L + 8N' (the per-agent state, 9 B an output, is
-                   written only in the last iteration of a run)
     The radix kernels get a count only when every iteration of the run
     sorted by radix (radix_all); otherwise they ran only in the first
     iteration after a change of the replicas.
     """
-    N, L, Np, O, E = (c["agent_steps"], c["live"], c["out"], c["occupied"],
-                      c["eaten"])
+    N, L, Np, O, B = (c["agent_steps"], c["live"], c["out"], c["occupied"],
+                      c["births"])
     if name == "bucket_sort":
         return 8 * N + 8 * L
     if name == "sort_up" and radix_all:
@@ -116,10 +123,24 @@ def kernel_bytes(name, c, passes, radix_all=False):
     if name == "sort_down" and radix_all:
         return 8 * N + 8 * L + 16 * L * (passes - 1)
     if name == "act":
-        return 12 * L + 4 * O + 4 * E
+        return 22 * N + 8 * O + 9 * B
     if name == "emit_move":
-        return 8 * L + 8 * Np
+        return 19 * N + 9 * B + 18 * Np
     return None
+
+
+def compulsory_bytes(name, c):
+    """The bytes this implementation cannot avoid moving (its own packed
+    layout: 8-byte records, 1-byte grass as for grass restarts up to 191),
+    for comparison with kernel_bytes:
+      act        read L records, write L values, read each occupied cell's
+                 grass, write each eaten one: 12L + O + E
+      emit_move  read L records, write each output's moved record: 8L + 8N'
+      bucket_sort  8N + 8L"""
+    N, L, Np, O, E = (c["agent_steps"], c["live"], c["out"], c["occupied"],
+                      c["eaten"])
+    return {"act": 12 * L + O + E, "emit_move": 8 * L + 8 * Np,
+            "bucket_sort": 8 * N + 8 * L}.get(name)
 
 
 def counted_traffic(kernel, sort):
@@ -173,9 +194,12 @@ def measure_roofline(params, capacity, steps, seed=0, sort_mode=None):
     per_kernel = {}
     for k, (nl, t) in step_kernels.items():
         b = kernel_bytes(k, c, passes, radix_all)
+        bc = compulsory_bytes(k, c)
         per_kernel[k] = {
             "launches": nl, "avg_us": round(t / max(nl, 1) / 1e3, 3),
             "GBps": round(b / t, 1) if b and t else None,
+            "frac": round(b / t / HBM_PEAK_GBS, 4) if b and t else None,
+            "compulsory_GBps": round(bc / t, 1) if bc and t else None,
             "share": round(t / sum(v[1] for v in step_kernels.values()), 3)}
     achieved = total_bytes / ns  # bytes/ns == GB/s
     return {
@@ -192,6 +216,7 @@ def measure_roofline(params, capacity, steps, seed=0, sort_mode=None):
         "ms_per_iteration": round(ms_tick, 4),
         "sort": "radix" if radix_all else "row buckets",
         "per_kernel": per_kernel,
+        "events_per_iteration": {k: round(v / steps, 1) for k, v in c.items()},
     }
 
 
@@ -229,6 +254,38 @@ def _run_cpu(exe, params, capacity, sample_steps, extra=()):
     return dict(tok.split("=") for tok in out.split())
 
 
+def probe_reference_cpu_path():
+    """Whether the reference's OpenCL CPU simulator (pp_cpu) could be built
+    and run on this host: an OpenCL CPU device (clinfo) and the cf4ocl2,
+    cl_ops and glib-2.0 headers its build needs
+    (opencl/CMakeLists.txt:22-27). Returns what was found, verbatim where
+    short."""
+    out = {}
+    try:
+        r = subprocess.run(["clinfo", "-l"], capture_output=True, text=True,
+                           timeout=60)
+        out["clinfo_l"] = (r.stdout + r.stderr).strip()[-600:]
+        r = subprocess.run(["clinfo"], capture_output=True, text=True,
+                           timeout=60)
+        cpu_dev = [ln.strip() for ln in r.stdout.splitlines()
+                   if "Device Type" in ln and "CPU" in ln]
+        out["opencl_cpu_devices"] = len(cpu_dev)
+    except (OSError, subprocess.SubprocessError) as e:
+        out["clinfo_l"] = f"clinfo not runnable: {e}"
+        out["opencl_cpu_devices"] = 0
+    incs = ["/usr/include", "/usr/local/include", "/opt/rocm/include"]
+    for name, rel in (("cf4ocl2", "cf4ocl2.h"), ("cl_ops", "cl_ops.h"),
+                      ("glib-2.0", "glib-2.0/glib.h")):
+        hits = [d for d in incs if os.path.exists(os.path.join(d, rel))
+                or os.path.exists(os.path.join(d, name, rel))]
+        out[name + "_headers"] = hits or None
+    out["pkg-config"] = bool(subprocess.run(
+        ["sh", "-c", "command -v pkg-config"], capture_output=True).stdout)
+    out["buildable"] = bool(out["opencl_cpu_devices"] and out["cf4ocl2_headers"]
+                            and out["cl_ops_headers"] and out["glib-2.0_headers"])
+    return out
+
+
 def cpu_baseline(params, capacity, sample_steps, single_steps, threads):
     """CPU restatements of the step on this host, on the first iterations of
     the same workload: the OpenMP row-band restatement (oracle/pp_omp.c) on
@@ -255,7 +312,55 @@ def cpu_baseline(params, capacity, sample_steps, single_steps, threads):
                 "value": float(kv1["agent_steps_per_s"]),
                 "unit": "agent-steps/s", "cores": 1, "kind": "port",
                 "sample": f"first {single_steps} iterations, single-threaded "
-                          f"C oracle: {float(kv1['seconds']):.1f} s"}}
+                          f"C oracle: {float(kv1['seconds']):.1f} s"},
+            "reference_cpu_path": probe_reference_cpu_path()}
+
+
+def batched_same_leg(counts, iters, side=1600):
+    """N replicas of the same workload (side x side, dense v2-style, seeds
+    1..N) as one batch against the same N run one at a time as plain
+    simulations: agent-steps/s and replicas/hour both ways, rows compared."""
+    import numpy as np
+    from pphpc_amd import Batch, Params, Simulation, stats_rows_to_array
+    d = v2s_params(side, iters)
+    n0 = d["init_sheep"] + d["init_wolves"]
+    res = 4 * n0
+    out = []
+    for n in counts:
+        t0 = time.perf_counter()
+        b = Batch(n * res, n * side * side, n, d["grass_restart"], device=0)
+        ids = [b.add(Params.from_values(**d), seed, res)
+               for seed in range(1, n + 1)]
+        while b.resources()["active"]:
+            b.run(512)
+        b.sync()
+        batched_s = time.perf_counter() - t0
+        steps_b = b.counters()["agent_steps"]
+        rows_b = [b.rows_array(r) for r in ids]
+        b.close()
+        t1 = time.perf_counter()
+        steps_p = 0
+        equal = True
+        for seed, rb in zip(range(1, n + 1), rows_b):
+            sim = Simulation(Params.from_values(**d), seed=seed, capacity=res,
+                             device=0)
+            rows = sim.simulate()
+            steps_p += sim.agent_steps
+            sim.close()
+            equal &= bool(np.array_equal(stats_rows_to_array(rows), rb))
+        plain_s = time.perf_counter() - t1
+        log(f"{n} x {side}^2 v2s: batched {batched_s:.2f} s, plain "
+            f"{plain_s:.2f} s, rows equal {equal}")
+        out.append({
+            "replicas": n, "side": side, "iters": iters, "seeds": f"1..{n}",
+            "batched_s": round(batched_s, 3), "plain_s": round(plain_s, 3),
+            "agent_steps_per_s_batched": steps_b / batched_s,
+            "agent_steps_per_s_plain": steps_p / plain_s,
+            "replicas_per_hour_batched": round(n * 3600 / batched_s, 1),
+            "replicas_per_hour_plain": round(n * 3600 / plain_s, 1),
+            "speedup": round(plain_s / batched_s, 3),
+            "rows_equal_plain": equal})
+    return out
 
 
 def batched_leg(n, iters, draw_key=4, tune_ticks=16):
@@ -352,6 +457,9 @@ def main():
     ap.add_argument("--batch-replicas", type=int, default=64)
     ap.add_argument("--batch-iters", type=int, default=4000)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-same", default="8,30",
+                    help="replica counts of the 1600x1600 batched-vs-plain "
+                         "leg (empty: skip)")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-hbm-gb", type=float, default=48.0)
@@ -421,6 +529,11 @@ def main():
     if not args.no_batch:
         batched = batched_leg(args.batch_replicas, args.batch_iters)
 
+    batched_1600 = None
+    if not args.no_batch and args.batch_same:
+        batched_1600 = batched_same_leg(
+            [int(x) for x in args.batch_same.split(",")], args.batch_iters)
+
     serving = None
     if not args.no_serve:
         serving = serving_leg(args.serve_seconds, args.serve_rate,
@@ -464,6 +577,7 @@ def main():
         "roofline_12800": roof_big,
         "roofline_12800_auto": roof_big_b,
         "batched": batched,
+        "batched_1600": batched_1600,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -29,6 +29,14 @@ extern "C" {
 #define PP_INVALID_PARAMS_FILE (-5)
 #define PP_UNABLE_SAVE_STATS (-6)
 #define PP_OUT_OF_RESOURCES (-8)
+/* No counterpart in the reference, whose 16-bit energy field wraps silently
+ * (pp_gpu.cl:166-173): an agent's energy after its action exceeded
+ * PP_ENERGY_MAX. The simulation (or, in a batch, only that replica) stops at
+ * that iteration, as the oracle does. */
+#define PP_ENERGY_OVERFLOW (-9)
+
+/* Largest agent energy (the 24-bit energy field of the packed agent). */
+#define PP_ENERGY_MAX 16777215u
 
 /* Agent type codes: SHEEP_ID / WOLF_ID of pp_common.in.h:48-52. */
 #define PP_SHEEP 0
@@ -104,13 +112,15 @@ int pp_sim_init(pp_sim_t *s, pp_stats64_t *stats0);
 /* One iteration of the loop body of ppg_simulate (pp_gpu.c:671-935): grass,
  * move, sort, cell index, actions, statistics. Writes the statistics of the
  * new state to *out (may be NULL). Returns PP_OUT_OF_RESOURCES when the agents
- * after births exceed the capacity (pp_gpu.c:847-852, :895-900). */
+ * after births exceed the capacity (pp_gpu.c:847-852, :895-900), and
+ * PP_ENERGY_OVERFLOW when an agent's energy after its action exceeds
+ * PP_ENERGY_MAX (checked first). */
 int pp_sim_step(pp_sim_t *s, pp_stats64_t *out);
 
 /* iters iterations with no host synchronisation between them; rows 1..iters
  * of the statistics go to out[0..iters-1] (may be NULL). On
- * PP_OUT_OF_RESOURCES the simulation stops at the failing iteration (see
- * pp_sim_error_step). */
+ * PP_OUT_OF_RESOURCES or PP_ENERGY_OVERFLOW the simulation stops at the
+ * failing iteration (see pp_sim_error_step). */
 int pp_sim_run(pp_sim_t *s, uint32_t iters, pp_stats64_t *out);
 
 /* Replaces the ppg_simulate + pp_stats_save sequence of main
@@ -130,8 +140,9 @@ uint64_t pp_sim_capacity(const pp_sim_t *s);
 /* Exact event sums over the completed iterations since the last init:
  * [0] agents at the start of each iteration (agent-steps), [1] agents alive
  * after the move, [2] agents after actions and births, [3] occupied cells,
- * [4] grass cells eaten. Used to state each kernel's algorithmic bytes. */
-int pp_sim_counters(const pp_sim_t *s, uint64_t out[5]);
+ * [4] grass cells eaten, [5] births. Used to state each kernel's algorithmic
+ * bytes. */
+int pp_sim_counters(const pp_sim_t *s, uint64_t out[6]);
 
 /* State dump, the counterpart of PPG_DUMP (pp_gpu.c:438-555): agents in slot
  * order (cell = y * grid_x + x, energy, type) and the grass countdown of every
@@ -208,7 +219,8 @@ int pp_sim_sync(pp_sim_t *s);
  * (pp_batch_footprint bytes); a replica needs a free slot, grid_xy
  * contiguous free cells and an agent reservation (the most agents it may
  * hold after births; above it the replica fails with PP_OUT_OF_RESOURCES,
- * alone). pp_batch_add returns PP_OUT_OF_RESOURCES when these are not free
+ * alone; an agent above PP_ENERGY_MAX fails it alone with
+ * PP_ENERGY_OVERFLOW). pp_batch_add returns PP_OUT_OF_RESOURCES when these are not free
  * now; slots, cells and reservations come back when replicas finish.
  * ---------------------------------------------------------------------- */
 
@@ -268,7 +280,7 @@ int pp_batch_resources(const pp_batch_t *b, uint64_t *agents_free,
 uint32_t pp_batch_tick(const pp_batch_t *b);
 
 /* Batch-wide event sums, as pp_sim_counters. */
-int pp_batch_counters(const pp_batch_t *b, uint64_t out[5]);
+int pp_batch_counters(const pp_batch_t *b, uint64_t out[6]);
 
 /* Sort passes, sort tile, action tile and emit grid in use. */
 int pp_batch_info(const pp_batch_t *b, uint32_t *sort_passes,
@@ -289,7 +301,11 @@ int pp_batch_set_shape(pp_batch_t *b, uint32_t keys_per_thread,
 /* Autotuner: runs trial_ticks iterations of the batch under each candidate
  * sort (bucket widths 2^8..2^11, radix shapes; the iterations count as
  * normal progress), keeps the fastest and reports it (bucket_shift 0 = radix
- * sort) with its seconds per tick. */
+ * sort) with its seconds per tick. When the fastest is what the automatic
+ * mode (pp_batch_set_sort 1) picks for the current replicas, the batch stays
+ * in the automatic mode; otherwise the winner is pinned as if set with
+ * pp_batch_set_sort / pp_batch_set_shape. A trial during which replicas
+ * finish is timed again. */
 int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks, uint32_t *bucket_shift,
 	uint32_t *keys_per_thread, uint32_t *max_digit_bits, double *sec_per_tick);
 

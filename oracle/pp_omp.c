@@ -25,9 +25,12 @@
  *                order, the oracle's order) and the statistics summed
  *
  *   pp_omp -p FILE [-s FILE] [-r SEED] [-m CAP] [-n ITERS] [-t THREADS] [-q]
+ *          [-b FILE]
  *
  * prints "agent_steps=N seconds=S agent_steps_per_s=R threads=T" like the
- * oracle CLI, and writes the statistics file unless -q.
+ * oracle CLI, and writes the statistics file unless -q. -b FILE also writes
+ * the exact rows computed (rows 0 .. last completed iteration, each seven
+ * little-endian uint64 in ppo_stats order), also after a failure.
  */
 #define _POSIX_C_SOURCE 200809L
 #include "pp_oracle.h"
@@ -66,6 +69,7 @@ typedef struct {
 	int32_t *energy;
 	uint8_t *type;
 	uint64_t n, cap;
+	int over;        /* an agent's energy exceeded the limit */
 } Part;
 
 typedef struct {
@@ -283,6 +287,7 @@ static int sim_step(Sim *s) {
 		}
 		Part *o = &s->out[t];
 		part_reserve(o, 2 * (b1 - b0));
+		o->over = 0;
 		uint64_t no = 0;
 		for (uint64_t q0 = 0; q0 < b1 - b0;) {
 			const uint32_t c = s->mcell[srt[q0]];
@@ -333,6 +338,8 @@ static int sim_step(Sim *s) {
 					e -= ce;
 					s->child_e[id] = ce;
 				}
+				/* the oracle's energy limit (2^24 - 1 after the action) */
+				o->over |= e > 0x00FFFFFF;
 				s->menergy[id] = e;
 			}
 			for (uint32_t a = 0; a < k; a++) {
@@ -362,7 +369,10 @@ static int sim_step(Sim *s) {
 				run += s->out[b].n;
 			}
 			s->ostart[T] = run;
-			if (run > s->cap) rc = -8;
+			int over = 0;
+			for (int b = 0; b < T; b++) over |= s->out[b].over;
+			if (over) rc = -9;
+			else if (run > s->cap) rc = -8;
 		}
 		/* 4. concatenate the bands' parts in band (= cell) order */
 		if (!rc) {
@@ -386,11 +396,11 @@ static double now_s(void) {
 }
 
 int main(int argc, char **argv) {
-	const char *params = "config.txt", *stats = "stats.txt";
+	const char *params = "config.txt", *stats = "stats.txt", *raw = NULL;
 	uint64_t seed = 0, cap = 0;
 	long iters_override = -1;
 	int quiet = 0, threads = 0, opt;
-	while ((opt = getopt(argc, argv, "p:s:r:m:n:t:q")) != -1) {
+	while ((opt = getopt(argc, argv, "p:s:r:m:n:t:qb:")) != -1) {
 		switch (opt) {
 		case 'p': params = optarg; break;
 		case 's': stats = optarg; break;
@@ -399,6 +409,7 @@ int main(int argc, char **argv) {
 		case 'n': iters_override = strtol(optarg, NULL, 0); break;
 		case 't': threads = atoi(optarg); break;
 		case 'q': quiet = 1; break;
+		case 'b': raw = optarg; break;
 		default: return -1;
 		}
 	}
@@ -421,10 +432,17 @@ int main(int argc, char **argv) {
 		sim_stats(s, &rows[t + 1]);
 	}
 	const double t1 = now_s();
-	if (rc) fprintf(stderr, "Error: out of resources at step %u\n", t);
+	if (rc) fprintf(stderr, "Error: %s at step %u\n",
+		rc == -9 ? "energy above its limit" : "out of resources", t);
 	printf("agent_steps=%llu seconds=%.6f agent_steps_per_s=%.6e threads=%d\n",
 		(unsigned long long)s->agent_steps, t1 - t0,
 		(double)s->agent_steps / (t1 - t0), s->T);
+	if (raw) {
+		FILE *fp = fopen(raw, "wb");
+		if (!fp || fwrite(rows, sizeof(ppo_stats), (size_t)t + 1, fp) != (size_t)t + 1)
+			rc = rc ? rc : -6;
+		if (fp) fclose(fp);
+	}
 	if (!quiet && !rc) rc = ppo_stats_save(stats, rows, p.iters + 1, p.grid_xy);
 	free(rows);
 	sim_destroy(s);

@@ -55,6 +55,15 @@
 #define PPO_SHEEP 0
 #define PPO_WOLF 1
 
+/* Largest energy an agent may hold: 2^24 - 1, the energy field of the GPU's
+ * packed agent value. An agent whose energy after its action (eating, then
+ * reproduction) exceeds it ends the simulation with PPO_ENERGY_OVERFLOW at
+ * that iteration, before the capacity check. The reference stores energy in
+ * a 16-bit field that wraps silently (pp_gpu.cl:166-173, :251-258); here the
+ * limit is explicit and its failure is a distinct code. */
+#define PPO_ENERGY_MAX 0x00FFFFFF
+#define PPO_ENERGY_OVERFLOW (-9)
+
 /* ------------------------------------------------------------------ RNG */
 
 static inline uint32_t mulhilo32(uint32_t a, uint32_t b, uint32_t *hi) {
@@ -330,6 +339,7 @@ int ppo_step(ppo_sim *s) {
 	/* 4. Per-cell actions in shuffled order, emitting the next state. */
 	uint64_t ev[5] = {n - live, 0, 0, 0, 0};
 	uint64_t out = 0;
+	int over = 0;
 	uint64_t q0 = 0;
 	while (q0 < live) {
 		uint32_t c = s->mcell[s->sorted[q0]];
@@ -396,6 +406,7 @@ int ppo_step(ppo_sim *s) {
 				s->child_e[id] = ce;
 				ev[3 + s->type[id]]++;
 			}
+			over |= e > PPO_ENERGY_MAX;
 			s->menergy[id] = e;
 		}
 		for (uint32_t a = 0; a < k; a++) {
@@ -416,6 +427,7 @@ int ppo_step(ppo_sim *s) {
 		q0 = q1;
 	}
 
+	if (over) return PPO_ENERGY_OVERFLOW;
 	if (out > s->cap) {
 		/* PP_OUT_OF_RESOURCES (pp_gpu.c:847-852). The grass pass already
 		 * ran; the error ends the simulation, as in the reference. */
@@ -587,7 +599,8 @@ int main(int argc, char **argv) {
 		rc = ppo_run(s, p.iters, rows + 1);
 	}
 	double t1 = now_s();
-	if (rc) fprintf(stderr, "Error: out of resources at step %u\n",
+	if (rc) fprintf(stderr, "Error: %s at step %u\n",
+		rc == PPO_ENERGY_OVERFLOW ? "energy above its limit" : "out of resources",
 		ppo_step_count(s));
 	printf("agent_steps=%llu seconds=%.6f agent_steps_per_s=%.6e\n",
 		(unsigned long long)ppo_agent_steps(s), t1 - t0,

@@ -70,13 +70,15 @@ void ppo_destroy(ppo_sim *s);
 /* Stats of the current state (row 0 right after ppo_create). */
 void ppo_stats_get(const ppo_sim *s, ppo_stats *out);
 
-/* Advance one iteration. Returns 0, or -8 (PP_OUT_OF_RESOURCES) when the
- * agents after births exceed capacity; the simulation cannot continue after
- * that (the agent arrays keep the previous state, the grass does not). */
+/* Advance one iteration. Returns 0; -9 (PP_ENERGY_OVERFLOW) when an agent's
+ * energy after its action exceeds 2^24 - 1; or -8 (PP_OUT_OF_RESOURCES) when
+ * the agents after births exceed capacity. The simulation cannot continue
+ * after either (the agent arrays keep the previous state, the grass does
+ * not). */
 int ppo_step(ppo_sim *s);
 
 /* Run iters iterations from the current state, writing rows 1..iters of the
- * stats into out[0..iters-1]. Returns 0 or -8 (and stops at that step). */
+ * stats into out[0..iters-1]. Returns 0, -9 or -8 (and stops at that step). */
 int ppo_run(ppo_sim *s, uint32_t iters, ppo_stats *out);
 
 /* Number of completed steps and live agents. */

@@ -30,6 +30,7 @@ from .sim import (  # noqa: F401
     PP_INVALID_PARAMS_FILE,
     PP_UNABLE_SAVE_STATS,
     PP_OUT_OF_RESOURCES,
+    PP_ENERGY_OVERFLOW,
     PARAM_KEYS,
     STAT_FIELDS,
 )

@@ -109,6 +109,7 @@ const char *pp_strerror(int code) {
 	case PP_INVALID_PARAMS_FILE: return "invalid parameters file";
 	case PP_UNABLE_SAVE_STATS: return "unable to save statistics";
 	case PP_OUT_OF_RESOURCES: return "agents above capacity (out of resources)";
+	case PP_ENERGY_OVERFLOW: return "agent energy above its 24-bit limit";
 	default: return "unknown error code";
 	}
 }

@@ -165,6 +165,9 @@ int main(int argc, char **argv) {
 			fprintf(stderr, "Error: agents above capacity at iteration %u "
 				"(capacity %llu)\n", pp_sim_error_step(sim),
 				(unsigned long long)pp_sim_capacity(sim));
+		else if (rc == PP_ENERGY_OVERFLOW)
+			fprintf(stderr, "Error: agent energy above %u at iteration %u\n",
+				PP_ENERGY_MAX, pp_sim_error_step(sim));
 		else
 			fprintf(stderr, "Error: %s\n", pp_strerror(rc));
 	} else {

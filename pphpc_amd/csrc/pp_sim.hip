@@ -71,6 +71,7 @@
 
 #include <algorithm>
 #include <map>
+#include <new>
 #include <vector>
 
 #include "pphpc.h"
@@ -100,7 +101,10 @@ constexpr int SCAN_UNROLL = 8;
  * every stored value is within R + REFRESH of the tick. */
 constexpr uint32_t REFRESH = 64;
 
-/* Packed agent value carried through the sort and the action kernels. */
+/* Packed agent value carried through the sort and the action kernels. An
+ * agent whose energy after its action exceeds V_ENERGY fails its replica with
+ * PP_ENERGY_OVERFLOW at that iteration (the oracle applies the same limit);
+ * energy never grows in the move, so the move needs no check. */
 constexpr uint32_t V_ENERGY = 0x00FFFFFFu; /* energy, 24 bits */
 constexpr uint32_t V_WOLF = 1u << 24;      /* type (0 sheep, 1 wolf) */
 constexpr uint32_t V_REPRO = 1u << 25;     /* reproduction draw < probability */
@@ -114,9 +118,10 @@ constexpr uint32_t RS_ACTIVE = 1;
 constexpr uint32_t RS_DONE = 2;
 constexpr uint32_t RS_FAILED = 3;
 
-/* Counters k_act accumulates per replica for the statistics row. */
+/* Counters k_act accumulates per replica for the statistics row; AC_OVF
+ * counts agents above the energy limit, AC_BIRTH the newborns. */
 enum { AC_SHEEP = 0, AC_WOLVES, AC_SHEEP_EN, AC_WOLVES_EN, AC_EATEN, AC_OCC,
-	AC_OUT, NACC };
+	AC_OUT, AC_OVF, AC_BIRTH, NACC };
 
 /* Per-replica constants, written by the host at admission. */
 struct Rep {
@@ -166,8 +171,9 @@ struct Ctl {
 	 * the SoA state; only the host reads it */
 	uint32_t state_tick;
 	/* sums over completed ticks of: agents at the start, alive after the
-	 * move, after actions, occupied cells, grass eaten */
-	unsigned long long agent_steps, sum_live, sum_out, sum_occ, sum_eaten;
+	 * move, after actions, occupied cells, grass eaten, births */
+	unsigned long long agent_steps, sum_live, sum_out, sum_occ, sum_eaten,
+		sum_births;
 };
 
 /* The first eight words of Ctl, which the step kernels start from: read
@@ -224,7 +230,7 @@ struct Params {
  * [0, NACC) of a slot: the action counters of a one-replica batch, summed by
  * k_act_scan; word PART_LIVE: the move's survivors, summed into Ctl::live by
  * the next tick's first kernel. Both consumers clear what they sum. */
-constexpr uint32_t PART_SLOTS = 64, PART_STRIDE = 16, PART_LIVE = 8;
+constexpr uint32_t PART_SLOTS = 64, PART_STRIDE = 16, PART_LIVE = 12;
 static_assert(PART_SLOTS == (uint32_t)WAVE, "one wave sums the slots");
 static_assert(NACC <= (int)PART_LIVE && PART_LIVE < PART_STRIDE,
 	"slot layout");
@@ -560,7 +566,7 @@ __device__ __forceinline__ RM load_rm_uniform(const Tables &T, uint32_t k,
  * of the new cell (blocks of 2^S cells per row), NO_DIGIT for the dead. */
 __device__ __forceinline__ uint32_t move_agent(uint32_t id, uint32_t x,
 	uint32_t y, int32_t energy, uint32_t wolf, const RM &m, uint32_t S,
-	uint32_t *val, uint32_t *bkt, bool &overflow) {
+	uint32_t *val, uint32_t *bkt) {
 	const uint4 r = pp_philox(id, m.step, PP_STREAM_MOVE, 0, m.key0, m.key1);
 	const uint32_t dir = pp_bounded(r.x, 5);
 	const uint32_t dice = pp_bounded(r.y, 100);
@@ -574,7 +580,6 @@ __device__ __forceinline__ uint32_t move_agent(uint32_t id, uint32_t x,
 	else if (dir == 3) ny = (y == m.gy - 1) ? 0u : y + 1;
 	else if (dir == 4) ny = (y == 0) ? m.gy - 1 : y - 1;
 	const uint32_t prob = wolf ? m.wolves_prob : m.sheep_prob;
-	overflow |= e > (int32_t)V_ENERGY;
 	*val = ((uint32_t)e & V_ENERGY) | (wolf ? V_WOLF : 0u)
 		| (dice < prob ? V_REPRO : 0u);
 	*bkt = bucket_of(m.bbase, m.nb, m.h, nx, ny, S);
@@ -804,21 +809,18 @@ __global__ __launch_bounds__(WG) void k_move0(const uint32_t *__restrict__ cell,
 	m.active = rc.status == RS_ACTIVE;
 	const uint32_t n = rp.n0;
 	unsigned long long live = 0;
-	bool overflow = false;
 	for (uint32_t l = blockIdx.x * WG + threadIdx.x; l < n; l += gridDim.x * WG) {
 		const uint32_t g = pos + l;
 		uint32_t k = DEAD_KEY, v = 0, bkt;
 		if (m.active) {
 			const uint32_t c = cell[g] - m.cb;
 			const uint32_t y = c / m.gx, x = c - y * m.gx;
-			k = move_agent(l, x, y, energy[g], type[g], m, P.bshift, &v, &bkt,
-				overflow);
+			k = move_agent(l, x, y, energy[g], type[g], m, P.bshift, &v, &bkt);
 		}
 		keys[g] = k;
 		vals[g] = v;
 		live += k != DEAD_KEY;
 	}
-	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, (int32_t)slot);
 	const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
 	const unsigned long long tsum = wave_sum(live);
 	if (lane == 0) s_red[w] = tsum;
@@ -1666,8 +1668,8 @@ __device__ __forceinline__ void mix_add(unsigned long long (*s_mix)[NACC],
 __device__ __noinline__ void wave_flush(unsigned long long a0,
 	unsigned long long a1, unsigned long long a2, unsigned long long a3,
 	unsigned long long a4, unsigned long long a5, unsigned long long a6,
-	unsigned long long *dst) {
-	unsigned long long v[NACC] = {a0, a1, a2, a3, a4, a5, a6};
+	unsigned long long a7, unsigned long long a8, unsigned long long *dst) {
+	unsigned long long v[NACC] = {a0, a1, a2, a3, a4, a5, a6, a7, a8};
 	const int lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
 	for (int k = 0; k < NACC; k++) {
@@ -1697,6 +1699,8 @@ __device__ __forceinline__ void act_wg_add(unsigned long long (&acc)[NACC],
 	const uint32_t nw = wave_sum32((uint32_t)acc[AC_WOLVES]);
 	const uint32_t ne = wave_sum32((uint32_t)acc[AC_EATEN]);
 	const uint32_t no = wave_sum32((uint32_t)acc[AC_OCC]);
+	const uint32_t nv = wave_sum32((uint32_t)acc[AC_OVF]);
+	const uint32_t nb = wave_sum32((uint32_t)acc[AC_BIRTH]);
 	unsigned long long es, ew;
 	if (__ballot(((acc[AC_SHEEP_EN] | acc[AC_WOLVES_EN]) >> 48) != 0) == 0) {
 		es = wave_sum32((uint32_t)(acc[AC_SHEEP_EN] & 0xFFFFFFu))
@@ -1715,6 +1719,8 @@ __device__ __forceinline__ void act_wg_add(unsigned long long (&acc)[NACC],
 		s_red[w][AC_EATEN] = ne;
 		s_red[w][AC_OCC] = no;
 		s_red[w][AC_OUT] = (unsigned long long)ns + nw;
+		s_red[w][AC_OVF] = nv;
+		s_red[w][AC_BIRTH] = nb;
 	}
 #pragma unroll
 	for (int k = 0; k < NACC; k++) acc[k] = 0;
@@ -1744,9 +1750,10 @@ struct ActLds {
  * S.val, cnt valid), whose agents belong to the replicas k_first..k_last of
  * the table. MIXED = false: one replica (ra), counters summed into acc;
  * MIXED = true: several, each cell looks up its replica and counters go
- * through LDS slots. See k_act for the rules. Returns the overflow flag. */
+ * through LDS slots. See k_act for the rules. Agents whose energy after
+ * their action exceeds V_ENERGY are counted in AC_OVF. */
 template <typename GT, bool MIXED>
-__device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
+__device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 	uint32_t cnt, const uint32_t *__restrict__ keys, uint32_t *vals,
 	GT *grass_at, uint32_t *act_cnt, uint32_t l1o, const Tables &T, uint32_t nrep,
 	uint32_t tick, uint32_t L, uint32_t Rmax, uint32_t k_first,
@@ -1755,7 +1762,6 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 	const uint32_t base = tile * A_TILE;
 	const uint32_t alive_at = tick + 1;
 	const SegVals sv{S.val, vals + base};
-	bool overflow = false;
 	/* Heads of this thread's positions; their grass loads are issued
 	 * together before any segment is processed. */
 	uint32_t cj[A_IPT];
@@ -1787,7 +1793,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 	/* sums over this thread's single-agent cells: at most A_IPT outputs
 	 * pairs below 2^25 each, so 32 bits hold them */
 	uint32_t f_sheep = 0, f_wolves = 0, f_se = 0, f_we = 0, f_eaten = 0,
-		f_occ = 0;
+		f_occ = 0, f_ovf = 0, f_birth = 0;
 #pragma unroll
 	for (int j = 0; j < A_IPT; j++) {
 		if (!hj[j]) continue;
@@ -1816,7 +1822,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 				ce = (uint32_t)(e / 2);
 				e -= e / 2;
 			}
-			overflow |= e > (int32_t)V_ENERGY;
+			f_ovf += e > (int32_t)V_ENERGY;
 			S.val[lp] = nv | ((uint32_t)e & V_ENERGY);
 			const uint32_t outs = (nv & V_CHILD) ? 2u : 1u;
 			const uint32_t en = (uint32_t)e + ce;
@@ -1829,6 +1835,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 			}
 			f_eaten += ate;
 			f_occ += 1;
+			f_birth += outs - 1;
 			nout += outs;
 			continue;
 		}
@@ -1844,6 +1851,8 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 		acc[AC_EATEN] += f_eaten;
 		acc[AC_OCC] += f_occ;
 		acc[AC_OUT] += f_sheep + f_wolves;
+		acc[AC_OVF] += f_ovf;
+		acc[AC_BIRTH] += f_birth;
 	}
 	__syncthreads();
 	const uint32_t nslow = S.nslow;
@@ -1885,7 +1894,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 			}
 			bool grass = grass_alive_at<GT>(gslow, alive_at, Rmax);
 			bool ate = false;
-			uint32_t killed = 0;
+			uint32_t killed = 0, ovf = 0;
 #pragma unroll
 			for (uint32_t q = 0; q < K_REG; q++) {
 				if (q >= k || ((killed >> q) & 1u)) continue;
@@ -1913,13 +1922,14 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 					nv |= V_CHILD | ((e & 1) ? V_ODD : 0u);
 					e -= e / 2;
 				}
-				overflow |= e > (int32_t)V_ENERGY;
+				ovf += e > (int32_t)V_ENERGY;
 				V[q] = nv | ((uint32_t)e & V_ENERGY);
 			}
 			if (ate) grass_cell(grass_at, c) = (GT)(tick + 1 + ra.R);
 			/* per-cell sums fit 32 bits: at most 2 K_REG outputs below
 			 * 2^24 each */
-			uint32_t n_sheep = 0, n_wolves = 0, e_sheep = 0, e_wolves = 0;
+			uint32_t n_sheep = 0, n_wolves = 0, e_sheep = 0, e_wolves = 0,
+				births = 0;
 #pragma unroll
 			for (uint32_t q = 0; q < K_REG; q++) {
 				if (q >= k) continue;
@@ -1936,6 +1946,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 					n_sheep += outs;
 					e_sheep += ea;
 				}
+				births += (v & V_CHILD) != 0;
 				if (lp + q < (uint32_t)A_TILE) nout += outs;
 				else if (outs) tile_add(act_cnt, l1o, tile + (lp + q) / A_TILE, outs);
 			}
@@ -1945,6 +1956,8 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 			sc[AC_WOLVES_EN] = e_wolves;
 			sc[AC_OUT] = n_sheep + n_wolves;
 			sc[AC_EATEN] = ate ? 1 : 0;
+			sc[AC_OVF] = ovf;
+			sc[AC_BIRTH] = births;
 		} else {
 			if (k > 1) {
 				uint4 r = make_uint4(0, 0, 0, 0);
@@ -1992,7 +2005,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 					nv |= V_CHILD | ((e & 1) ? V_ODD : 0u);
 					e -= e / 2;
 				}
-				overflow |= e > (int32_t)V_ENERGY;
+				sc[AC_OVF] += e > (int32_t)V_ENERGY;
 				sv.set(q, nv | ((uint32_t)e & V_ENERGY));
 			}
 			if (ate) grass_cell(grass_at, c) = (GT)(tick + 1 + ra.R);
@@ -2012,6 +2025,7 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 					sc[AC_SHEEP_EN] += ea;
 				}
 				sc[AC_OUT] += outs;
+				sc[AC_BIRTH] += (v & V_CHILD) != 0;
 				if (q < (uint32_t)A_TILE) nout += outs;
 				else if (outs) tile_add(act_cnt, l1o, tile + q / A_TILE, outs);
 			}
@@ -2041,7 +2055,6 @@ __device__ __forceinline__ bool act_tile(ActLds<GT> &S, uint32_t tile,
 		}
 		__syncthreads();
 	}
-	return overflow;
 }
 
 /* Agent actions for the agents sorted by cell (ascending agent id within a
@@ -2099,7 +2112,6 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 	unsigned long long acc[NACC] = {0, 0, 0, 0, 0, 0, 0};
 	uint32_t acc_k = NO_REP;   /* replica the registers sum for */
 	RA ra{};
-	bool overflow = false;
 	/* Each group takes a contiguous run of tiles, so it crosses few replica
 	 * boundaries. The next tile's pairs are loaded into registers before
 	 * this tile's stores are issued: the loads then do not wait behind
@@ -2164,7 +2176,7 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 			if (k_first != acc_k) {
 				if (acc_k != NO_REP) {
 					wave_flush(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6],
-						&T.rctl[T.order[acc_k]].acc[0]);
+						acc[7], acc[8], &T.rctl[T.order[acc_k]].acc[0]);
 #pragma unroll
 					for (int i = 0; i < NACC; i++) acc[i] = 0;
 				}
@@ -2172,11 +2184,10 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 				ra = load_ra(T, k_first, tick);
 			}
 			if (tile + 1 < t_end) fetch(tile + 1);
-			overflow |= act_tile<GT, false>(S, tile, cnt, keys, vals, grass_at,
+			act_tile<GT, false>(S, tile, cnt, keys, vals, grass_at,
 				act_cnt, P.act_l1, T, nrep, tick, L, P.R, k_first, k_last, ra, acc);
 		}
 	}
-	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
 	/* one replica: a partial-sum slot (k_act_scan adds them up) */
 	if (acc_k != NO_REP)
 		act_wg_add(acc, S.red, nrep == 1
@@ -2198,7 +2209,6 @@ __global__ __launch_bounds__(WG) void k_act_mixed(
 	const uint32_t nm = ctl->nmixed;
 	unsigned long long acc[NACC] = {0, 0, 0, 0, 0, 0, 0};
 	const RA none{};
-	bool overflow = false;
 	for (uint32_t i = blockIdx.x; i < nm; i += gridDim.x) {
 		const uint32_t tile = mixed_list[i];
 		const uint32_t cnt = load_tile(keys, vals, S.key, S.val, tile * A_TILE, L);
@@ -2206,10 +2216,9 @@ __global__ __launch_bounds__(WG) void k_act_mixed(
 			__builtin_amdgcn_readfirstlane(S.key[1]));
 		const uint32_t k_last = rep_index(T, nrep,
 			__builtin_amdgcn_readfirstlane(S.key[cnt]));
-		overflow |= act_tile<GT, true>(S, tile, cnt, keys, vals, grass_at,
+		act_tile<GT, true>(S, tile, cnt, keys, vals, grass_at,
 			act_cnt, P.act_l1, T, nrep, tick, L, P.R, k_first, k_last, none, acc);
 	}
-	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
 }
 
 /* One work-group: exclusive scan of the output counts of the action tiles'
@@ -2224,9 +2233,10 @@ __global__ __launch_bounds__(WG) void k_act_mixed(
  * eaten at this tick restart at R (slot t mod (R+1)): exactly the grass
  * kernel (pp_gpu.cl:448-476) plus the sums of reduce_grass1/2
  * (pp_gpu.cl:485-583). Then the replica's statistics row s+1 is written, its
- * capacity checked (pp_gpu.c:847-852) and its counters cleared; a replica
- * that reached its iteration count is done, one above its capacity failed,
- * and the emit drops the agents of both. The batch fails when its outputs
+ * energy limit and capacity checked (pp_gpu.c:847-852) and its counters
+ * cleared; a replica that reached its iteration count is done, one with an
+ * agent above the energy limit or above its capacity failed (the energy
+ * limit first, as the oracle checks), and the emit drops the agents of both. The batch fails when its outputs
  * exceed the agent buffers. */
 
 template <int NT>
@@ -2317,7 +2327,8 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 	if (threadIdx.x >= WAVE) return;
 	const int lane = threadIdx.x;
 	const uint32_t nrep = H.nrep;
-	unsigned long long bs[3] = {0, 0, 0};  /* agent-steps, occupied, eaten */
+	/* agent-steps, occupied, eaten, births */
+	unsigned long long bs[4] = {0, 0, 0, 0};
 	if (nrep == 1) {
 		/* the general loop below for k = 0, from the requested values */
 		unsigned long long a[NACC];
@@ -2333,9 +2344,9 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 			rc.base = 0;
 			if (rc1.status == RS_ACTIVE) {
 				const uint32_t s = tick - rp1.t0;
-				if (out > rp1.cap) {
+				if (a[AC_OVF] || out > rp1.cap) {
 					rc.status = RS_FAILED;
-					rc.error = PP_OUT_OF_RESOURCES;
+					rc.error = a[AC_OVF] ? PP_ENERGY_OVERFLOW : PP_OUT_OF_RESOURCES;
 					rc.err_step = s;
 					rc.n = 0;
 				} else {
@@ -2354,6 +2365,7 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 					bs[0] = rc1.n;
 					bs[1] = a[AC_OCC];
 					bs[2] = E;
+					bs[3] = a[AC_BIRTH];
 					rc.agent_steps = rc1.agent_steps + rc1.n;
 					rc.step = s + 1;
 					rc.n = out;
@@ -2387,9 +2399,9 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 			rc.base = carry + pre;
 			if (rc.status == RS_ACTIVE) {
 				const uint32_t s = tick - rp.t0;
-				if (out > rp.cap) {
+				if (a[AC_OVF] || out > rp.cap) {
 					rc.status = RS_FAILED;
-					rc.error = PP_OUT_OF_RESOURCES;
+					rc.error = a[AC_OVF] ? PP_ENERGY_OVERFLOW : PP_OUT_OF_RESOURCES;
 					rc.err_step = s;
 					rc.n = 0;
 				} else {
@@ -2409,6 +2421,7 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 					bs[0] += rc.n;
 					bs[1] += a[AC_OCC];
 					bs[2] += E;
+					bs[3] += a[AC_BIRTH];
 					rc.agent_steps += rc.n;
 					rc.step = s + 1;
 					rc.n = out;
@@ -2424,13 +2437,14 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 		carry += tot;
 	}
 #pragma unroll
-	for (int i = 0; i < 3; i++) bs[i] = wave_sum(bs[i]);
+	for (int i = 0; i < 4; i++) bs[i] = wave_sum(bs[i]);
 	if (lane != 0) return;
 	ctl->agent_steps += bs[0];
 	ctl->sum_live += L;
 	ctl->sum_out += total;
 	ctl->sum_occ += bs[1];
 	ctl->sum_eaten += bs[2];
+	ctl->sum_births += bs[3];
 	ctl->n = total;
 	ctl->nmixed = 0;
 	ctl->live[(tick + 1) & 1] = 0;
@@ -2455,7 +2469,7 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 	int32_t *__restrict__ oenergy, uint8_t *__restrict__ otype,
 	uint32_t *__restrict__ mkeys, uint32_t *__restrict__ mvals,
 	uint32_t *ocnt, uint32_t *ncnt, const Tables &T, uint32_t nrep,
-	uint32_t tick, const RM &m_u, const Params &P, bool keep, bool &overflow) {
+	uint32_t tick, const RM &m_u, const Params &P, bool keep) {
 	unsigned long long live = 0;
 	const uint32_t S = P.bshift;
 	for (uint32_t i0 = 0; i0 < total; i0 += WG) {
@@ -2475,8 +2489,7 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 			const uint32_t l = cc - m.cb;
 			const uint32_t y = div_by(l, m.gxm, m.gxs), x = l - y * m.gx;
 			if (m.active)
-				mk = move_agent(id - m.base, x, y, (int32_t)en, t, m, S, &mv, &nbk,
-					overflow);
+				mk = move_agent(id - m.base, x, y, (int32_t)en, t, m, S, &mv, &nbk);
 			ob = bucket_of(m.bbase, m.nb, m.h, x, y, S);
 			if (P.bucket_on && m.nb >= 2) {
 				const uint32_t xb = x & ((1u << S) - 1u);
@@ -2531,7 +2544,6 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
 	const uint32_t S = P.bshift;
 	unsigned long long live = 0;
-	bool overflow = false;
 	/* A batch of one replica (every plain simulation) needs no lookups. */
 	RM m_one{};
 	if (nrep == 1 && ntiles > blockIdx.x)
@@ -2672,15 +2684,14 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		if (!mixed)
 			live += emit_outputs<false>(s_oc, s_ov, s_win, s_l1, w0, u0, total, goff, ocell,
 				oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
-				keep, overflow);
+				keep);
 		else
 			live += emit_outputs<true>(s_oc, s_ov, s_win, s_l1, w0, u0, total, goff, ocell,
 				oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
-				keep, overflow);
+				keep);
 		__syncthreads();
 	}
 	if (P.bucket_on && have_win) flush();
-	if (overflow) set_error(ctl, PP_OUT_OF_RESOURCES, tick, -1);
 	const int lane = tid & (WAVE - 1), w = tid / WAVE;
 	/* a thread's survivors stay far below 2^32 (its share of one tick) */
 	const unsigned long long tsum = wave_sum32((uint32_t)live);
@@ -2873,6 +2884,7 @@ struct pp_batch {
 	int graph_steps = 0;
 
 	int profiling = 0;
+	int no_graph = 0;   /* never capture graphs (allocation-free runs) */
 	std::vector<PendingEvent> pending;
 	std::vector<hipEvent_t> free_events;
 	uint64_t prof_launches[KI_COUNT] = {};
@@ -2883,6 +2895,7 @@ struct pp_batch {
 	uint64_t reserved = 0;
 	std::map<uint32_t, Replica> reps;
 	uint32_t next_id = 0;
+	uint64_t table_version = 0;   /* replica table uploads */
 
 	Tables tables() const {
 		return Tables{rep_dev, rctl_dev, cbs_dev, order_dev, bbase_dev};
@@ -3145,7 +3158,8 @@ static int build_graph(pp_batch_t *b, int steps, int radix,
  * The last tick's emit writes the SoA state that state_of reads. */
 static int enqueue_ticks(pp_batch_t *b, uint32_t ticks) {
 	b->ctl_fresh = 0;
-	const int use_graph = !b->profiling && getenv("PPHPC_NO_GRAPH") == nullptr;
+	const int use_graph = !b->profiling && !b->no_graph &&
+		getenv("PPHPC_NO_GRAPH") == nullptr;
 	const int radix = !b->P.bucket_on;
 	uint32_t i = 0;
 	if (ticks) {
@@ -3232,6 +3246,7 @@ static int upload_table(pp_batch_t *b) {
 	b->ctl_host->nrep = nrep;
 	b->ctl_host->nbuckets = nb32;
 	b->need_radix = 1;
+	b->table_version++;
 	return PP_SUCCESS;
 }
 
@@ -3439,44 +3454,57 @@ static uint64_t max_hist_words(uint32_t key_bits, uint32_t cap) {
 	return w;
 }
 
-static int batch_alloc(pp_batch_t *b) {
+/* The device buffers of a batch, each taken from take(&ptr, bytes): device
+ * allocations (batch_alloc), or slices of a fixed region.
+ * b->hist_alloc_words is set by the caller. */
+template <typename Take>
+static int batch_buffers(pp_batch_t *b, Take &&take) {
 	const uint64_t A = b->P.cap, C = b->P.C;
 	/* Vector accesses read/write 16 bytes at multiples of 4 agents. */
 	const uint64_t A4 = (A + 3) & ~3ull;
-	HIP_OK(hipMalloc(&b->cell, A4 * 4));
-	HIP_OK(hipMalloc(&b->energy, A4 * 4));
-	HIP_OK(hipMalloc(&b->type, A4));
+	HIP_OK(take((void **)&b->cell, A4 * 4));
+	HIP_OK(take((void **)&b->energy, A4 * 4));
+	HIP_OK(take((void **)&b->type, A4));
 	for (int i = 0; i < 3; i++) {
-		HIP_OK(hipMalloc(&b->keys[i], A4 * 4));
-		HIP_OK(hipMalloc(&b->vals[i], A4 * 4));
+		HIP_OK(take((void **)&b->keys[i], A4 * 4));
+		HIP_OK(take((void **)&b->vals[i], A4 * 4));
 	}
-	HIP_OK(hipMalloc(&b->grass_at, C * b->grass_bytes));
+	HIP_OK(take((void **)&b->grass_at, C * b->grass_bytes));
 	b->ring_stride = b->P.R + 1;
-	HIP_OK(hipMalloc(&b->ring, (size_t)b->max_rep * b->ring_stride * 8));
-	b->hist_alloc_words = max_hist_words(b->key_bits, b->P.cap);
-	HIP_OK(hipMalloc(&b->hist, (size_t)b->hist_alloc_words * 4));
-	HIP_OK(hipMalloc(&b->totals, RADIX_MAX * 4));
+	HIP_OK(take((void **)&b->ring, (size_t)b->max_rep * b->ring_stride * 8));
+	HIP_OK(take((void **)&b->hist, (size_t)b->hist_alloc_words * 4));
+	HIP_OK(take((void **)&b->totals, RADIX_MAX * 4));
 	/* tile counts, then the sums of blocks of 64 tiles */
 	const uint32_t act_tiles = ceil_div(A, A_TILE) + 1;
 	b->P.act_l1 = act_tiles;
 	b->act_words = act_tiles + ceil_div(act_tiles, 64) + 1;
-	HIP_OK(hipMalloc(&b->act_cnt, (size_t)b->act_words * 4));
-	HIP_OK(hipMalloc(&b->mixed_list, (size_t)act_tiles * 4));
+	HIP_OK(take((void **)&b->act_cnt, (size_t)b->act_words * 4));
+	HIP_OK(take((void **)&b->mixed_list, (size_t)act_tiles * 4));
 	const size_t nbw = cnt_words(b->P.nb_cap);
-	HIP_OK(hipMalloc(&b->ocnt, nbw * 4));
-	HIP_OK(hipMalloc(&b->ncnt, nbw * 4));
+	HIP_OK(take((void **)&b->ocnt, nbw * 4));
+	HIP_OK(take((void **)&b->ncnt, nbw * 4));
+	HIP_OK(take((void **)&b->bbase_dev, ((size_t)b->max_rep + 1) * 4));
+	HIP_OK(take((void **)&b->ctl, sizeof(Ctl)));
+	HIP_OK(take((void **)&b->P.part, PART_SLOTS * PART_STRIDE * 8));
+	HIP_OK(take((void **)&b->rep_dev, (size_t)b->max_rep * sizeof(Rep)));
+	HIP_OK(take((void **)&b->rctl_dev, (size_t)b->max_rep * sizeof(RepCtl)));
+	HIP_OK(take((void **)&b->cbs_dev, (size_t)b->max_rep * 4));
+	HIP_OK(take((void **)&b->order_dev, (size_t)b->max_rep * 4));
+	const size_t srows = (size_t)b->max_rep * b->stats_rows;
+	HIP_OK(take((void **)&b->stats_dev, srows * sizeof(pp_stats64_t)));
+	return PP_SUCCESS;
+}
+
+static int batch_alloc(pp_batch_t *b) {
+	b->hist_alloc_words = max_hist_words(b->key_bits, b->P.cap);
+	PP_TRY(batch_buffers(b, [](void **p, size_t bytes) {
+		return hipMalloc(p, bytes);
+	}));
+	const size_t nbw = cnt_words(b->P.nb_cap);
 	HIP_OK(hipMemset(b->ocnt, 0, nbw * 4));
 	HIP_OK(hipMemset(b->ncnt, 0, nbw * 4));
-	HIP_OK(hipMalloc(&b->bbase_dev, ((size_t)b->max_rep + 1) * 4));
-	HIP_OK(hipMalloc(&b->ctl, sizeof(Ctl)));
-	HIP_OK(hipMalloc(&b->P.part, PART_SLOTS * PART_STRIDE * 8));
 	HIP_OK(hipMemset(b->P.part, 0, PART_SLOTS * PART_STRIDE * 8));
-	HIP_OK(hipMalloc(&b->rep_dev, (size_t)b->max_rep * sizeof(Rep)));
-	HIP_OK(hipMalloc(&b->rctl_dev, (size_t)b->max_rep * sizeof(RepCtl)));
-	HIP_OK(hipMalloc(&b->cbs_dev, (size_t)b->max_rep * 4));
-	HIP_OK(hipMalloc(&b->order_dev, (size_t)b->max_rep * 4));
 	const size_t srows = (size_t)b->max_rep * b->stats_rows;
-	HIP_OK(hipMalloc(&b->stats_dev, srows * sizeof(pp_stats64_t)));
 	HIP_OK(hipHostMalloc(&b->stats_host, srows * sizeof(pp_stats64_t),
 		hipHostMallocDefault));
 	HIP_OK(hipHostMalloc(&b->ctl_host, sizeof(Ctl), hipHostMallocDefault));
@@ -3680,7 +3708,6 @@ static int admit(pp_batch_t *b, const pp_params_t *p, uint64_t seed,
 	r.keep = keep;
 	r.sink = nullptr;
 	r.sink_from = 0;
-	if (keep) r.rows.reserve((size_t)iters + 1);
 	PP_TRY(upload_table(b));
 
 	const Tables T = b->tables();
@@ -3794,34 +3821,50 @@ static int time_ticks(pp_batch_t *b, uint32_t ticks, double *sec) {
 	return collect(b);
 }
 
+/* Runs the body of an entry point: no C++ exception crosses the C ABI (an
+ * allocation that fails becomes PP_OUT_OF_RESOURCES, anything else
+ * PP_LIBRARY_ERROR). */
+template <typename F>
+static int guarded(F &&f) {
+	try {
+		return f();
+	} catch (const std::bad_alloc &) {
+		return PP_OUT_OF_RESOURCES;
+	} catch (...) {
+		return PP_LIBRARY_ERROR;
+	}
+}
+
 extern "C" {
 
 /* ------------------------------------------------------ single simulation */
 
 int pp_sim_create(const pp_params_t *p, uint64_t seed, uint64_t capacity,
 	int device, pp_sim_t **out) {
-	if (!p || !out) return PP_INVALID_ARGS;
-	*out = nullptr;
-	PP_TRY(check_params(p));
-	const uint64_t C = (uint64_t)p->grid_x * p->grid_y;
-	const uint64_t n0 = (uint64_t)p->init_sheep + p->init_wolves;
-	if (capacity == 0) {
-		capacity = 4 * n0;
-		if (capacity < PP_DEFAULT_MAX_AGENTS) capacity = PP_DEFAULT_MAX_AGENTS;
-	}
-	if (capacity < n0 || capacity >= (1ull << 29)) return PP_INVALID_ARGS;
-	pp_sim_t *s = new pp_sim_t();
-	int rc = batch_create(capacity, C, 1, p->grass_restart, device, &s->b);
-	if (rc) {
-		delete s;
-		return rc;
-	}
-	s->prm = *p;
-	s->prm.grid_xy = (uint32_t)C;
-	s->seed = seed;
-	s->cap = (uint32_t)capacity;
-	*out = s;
-	return PP_SUCCESS;
+	return guarded([&]() -> int {
+		if (!p || !out) return PP_INVALID_ARGS;
+		*out = nullptr;
+		PP_TRY(check_params(p));
+		const uint64_t C = (uint64_t)p->grid_x * p->grid_y;
+		const uint64_t n0 = (uint64_t)p->init_sheep + p->init_wolves;
+		if (capacity == 0) {
+			capacity = 4 * n0;
+			if (capacity < PP_DEFAULT_MAX_AGENTS) capacity = PP_DEFAULT_MAX_AGENTS;
+		}
+		if (capacity < n0 || capacity >= (1ull << 29)) return PP_INVALID_ARGS;
+		pp_sim_t *s = new pp_sim_t();
+		int rc = batch_create(capacity, C, 1, p->grass_restart, device, &s->b);
+		if (rc) {
+			delete s;
+			return rc;
+		}
+		s->prm = *p;
+		s->prm.grid_xy = (uint32_t)C;
+		s->seed = seed;
+		s->cap = (uint32_t)capacity;
+		*out = s;
+		return PP_SUCCESS;
+	});
 }
 
 void pp_sim_destroy(pp_sim_t *s) {
@@ -3837,49 +3880,53 @@ static Replica *sim_rep(const pp_sim_t *s) {
 }
 
 int pp_sim_init(pp_sim_t *s, pp_stats64_t *stats0) {
-	if (!s) return PP_INVALID_ARGS;
-	s->initialised = 0;
-	PP_TRY(batch_reset(s->b));
-	uint32_t id = 0;
-	PP_TRY(admit(s->b, &s->prm, s->seed, s->cap, ITERS_UNLIMITED, 0, &id));
-	s->rid = id;
-	s->initialised = 1;
-	Replica *r = sim_rep(s);
-	r->sink = stats0;
-	r->sink_from = 0;
-	const int rc = collect(s->b);
-	r->sink = nullptr;
-	if (rc) return rc;
-	if (s->b->ctl_host->error) return s->b->ctl_host->error;
-	return PP_SUCCESS;
+	return guarded([&]() -> int {
+		if (!s) return PP_INVALID_ARGS;
+		s->initialised = 0;
+		PP_TRY(batch_reset(s->b));
+		uint32_t id = 0;
+		PP_TRY(admit(s->b, &s->prm, s->seed, s->cap, ITERS_UNLIMITED, 0, &id));
+		s->rid = id;
+		s->initialised = 1;
+		Replica *r = sim_rep(s);
+		r->sink = stats0;
+		r->sink_from = 0;
+		const int rc = collect(s->b);
+		r->sink = nullptr;
+		if (rc) return rc;
+		if (s->b->ctl_host->error) return s->b->ctl_host->error;
+		return PP_SUCCESS;
+	});
 }
 
 int pp_sim_run(pp_sim_t *s, uint32_t iters, pp_stats64_t *out) {
-	Replica *r = sim_rep(s);
-	if (!r) return PP_INVALID_ARGS;
-	pp_batch_t *b = s->b;
-	HIP_OK(hipSetDevice(b->device));
-	if (b->ctl_host->error) return b->ctl_host->error;
-	if (r->error) return r->error;
-	const uint64_t last = (uint64_t)b->ctl_host->tick + iters + b->P.R + 1;
-	if (last >= 0xFFFFFFFFull) return PP_INVALID_ARGS;
-	uint32_t done = 0;
-	while (done < iters) {
-		const uint32_t chunk = min_u32(iters - done, b->stats_rows / 2);
-		const uint32_t step0 = r->rows_have - 1;
-		r->sink = out ? out + done : nullptr;
-		r->sink_from = step0 + 1;
-		int rc = enqueue_ticks(b, chunk);
-		if (!rc) rc = collect(b);
-		r->sink = nullptr;
-		if (rc) return rc;
-		const uint32_t got = r->rows_have - 1 - step0;
-		done += got;
+	return guarded([&]() -> int {
+		Replica *r = sim_rep(s);
+		if (!r) return PP_INVALID_ARGS;
+		pp_batch_t *b = s->b;
+		HIP_OK(hipSetDevice(b->device));
 		if (b->ctl_host->error) return b->ctl_host->error;
 		if (r->error) return r->error;
-		if (got != chunk) return PP_LIBRARY_ERROR;
-	}
-	return PP_SUCCESS;
+		const uint64_t last = (uint64_t)b->ctl_host->tick + iters + b->P.R + 1;
+		if (last >= 0xFFFFFFFFull) return PP_INVALID_ARGS;
+		uint32_t done = 0;
+		while (done < iters) {
+			const uint32_t chunk = min_u32(iters - done, b->stats_rows / 2);
+			const uint32_t step0 = r->rows_have - 1;
+			r->sink = out ? out + done : nullptr;
+			r->sink_from = step0 + 1;
+			int rc = enqueue_ticks(b, chunk);
+			if (!rc) rc = collect(b);
+			r->sink = nullptr;
+			if (rc) return rc;
+			const uint32_t got = r->rows_have - 1 - step0;
+			done += got;
+			if (b->ctl_host->error) return b->ctl_host->error;
+			if (r->error) return r->error;
+			if (got != chunk) return PP_LIBRARY_ERROR;
+		}
+		return PP_SUCCESS;
+	});
 }
 
 int pp_sim_step(pp_sim_t *s, pp_stats64_t *out) {
@@ -3887,9 +3934,11 @@ int pp_sim_step(pp_sim_t *s, pp_stats64_t *out) {
 }
 
 int pp_sim_simulate(pp_sim_t *s, pp_stats64_t *out) {
-	if (!s) return PP_INVALID_ARGS;
-	PP_TRY(pp_sim_init(s, out));
-	return pp_sim_run(s, s->prm.iters, out ? out + 1 : nullptr);
+	return guarded([&]() -> int {
+		if (!s) return PP_INVALID_ARGS;
+		PP_TRY(pp_sim_init(s, out));
+		return pp_sim_run(s, s->prm.iters, out ? out + 1 : nullptr);
+	});
 }
 
 uint32_t pp_sim_step_count(const pp_sim_t *s) {
@@ -3915,7 +3964,7 @@ uint64_t pp_sim_agent_steps(const pp_sim_t *s) {
 
 uint64_t pp_sim_capacity(const pp_sim_t *s) { return s ? s->cap : 0; }
 
-int pp_sim_counters(const pp_sim_t *s, uint64_t out[5]) {
+int pp_sim_counters(const pp_sim_t *s, uint64_t out[6]) {
 	if (!s || !out) return PP_INVALID_ARGS;
 	const Ctl *c = s->b->ctl_host;
 	out[0] = c->agent_steps;
@@ -3923,35 +3972,40 @@ int pp_sim_counters(const pp_sim_t *s, uint64_t out[5]) {
 	out[2] = c->sum_out;
 	out[3] = c->sum_occ;
 	out[4] = c->sum_eaten;
+	out[5] = c->sum_births;
 	return PP_SUCCESS;
 }
 
 int pp_sim_get_state(pp_sim_t *s, uint32_t *cell, int32_t *energy,
 	uint8_t *type, uint32_t *grass) {
-	Replica *r = sim_rep(s);
-	if (!r) return PP_INVALID_ARGS;
-	HIP_OK(hipSetDevice(s->b->device));
-	PP_TRY(read_ctl(s->b));
-	return state_of(s->b, *r, cell, energy, type, grass);
+	return guarded([&]() -> int {
+		Replica *r = sim_rep(s);
+		if (!r) return PP_INVALID_ARGS;
+		HIP_OK(hipSetDevice(s->b->device));
+		PP_TRY(read_ctl(s->b));
+		return state_of(s->b, *r, cell, energy, type, grass);
+	});
 }
 
 int pp_sim_dump(pp_sim_t *s, const char *agents_path, const char *cells_path,
 	uint32_t dump_type, int append) {
-	Replica *r = sim_rep(s);
-	if (!r) return PP_INVALID_ARGS;
-	HIP_OK(hipSetDevice(s->b->device));
-	PP_TRY(read_ctl(s->b));
-	if (r->slot < 0) return PP_INVALID_ARGS;
-	const uint64_t n = s->b->rctl_host[r->slot].n;
-	const uint64_t cells = (uint64_t)s->prm.grid_x * s->prm.grid_y;
-	std::vector<uint32_t> cell(n), grass(cells);
-	std::vector<int32_t> energy(n);
-	std::vector<uint8_t> type(n);
-	PP_TRY(state_of(s->b, *r, cell.data(), energy.data(), type.data(),
-		grass.data()));
-	return pp_dump_state(agents_path, cells_path, append, r->rows_have - 1,
-		dump_type, &s->prm, n, cell.data(), energy.data(), type.data(),
-		grass.data());
+	return guarded([&]() -> int {
+		Replica *r = sim_rep(s);
+		if (!r) return PP_INVALID_ARGS;
+		HIP_OK(hipSetDevice(s->b->device));
+		PP_TRY(read_ctl(s->b));
+		if (r->slot < 0) return PP_INVALID_ARGS;
+		const uint64_t n = s->b->rctl_host[r->slot].n;
+		const uint64_t cells = (uint64_t)s->prm.grid_x * s->prm.grid_y;
+		std::vector<uint32_t> cell(n), grass(cells);
+		std::vector<int32_t> energy(n);
+		std::vector<uint8_t> type(n);
+		PP_TRY(state_of(s->b, *r, cell.data(), energy.data(), type.data(),
+			grass.data()));
+		return pp_dump_state(agents_path, cells_path, append, r->rows_have - 1,
+			dump_type, &s->prm, n, cell.data(), energy.data(), type.data(),
+			grass.data());
+	});
 }
 
 int pp_sim_set_profiling(pp_sim_t *s, int enable) {
@@ -3964,18 +4018,24 @@ int pp_sim_prof_count(const pp_sim_t *s) { return s ? KI_COUNT : 0; }
 
 int pp_sim_prof_get(pp_sim_t *s, int i, const char **name, uint64_t *launches,
 	double *total_ns) {
-	if (!s) return PP_INVALID_ARGS;
-	return prof_get(s->b, i, name, launches, total_ns);
+	return guarded([&]() -> int {
+		if (!s) return PP_INVALID_ARGS;
+		return prof_get(s->b, i, name, launches, total_ns);
+	});
 }
 
 int pp_sim_prof_reset(pp_sim_t *s) {
-	if (!s) return PP_INVALID_ARGS;
-	return pp_batch_prof_reset(s->b);
+	return guarded([&]() -> int {
+		if (!s) return PP_INVALID_ARGS;
+		return pp_batch_prof_reset(s->b);
+	});
 }
 
 int pp_sim_prof_export(pp_sim_t *s, const char *path) {
-	if (!s || !path) return PP_INVALID_ARGS;
-	return prof_export(s->b, path);
+	return guarded([&]() -> int {
+		if (!s || !path) return PP_INVALID_ARGS;
+		return prof_export(s->b, path);
+	});
 }
 
 int pp_sim_info(const pp_sim_t *s, uint32_t *sort_passes, uint32_t *sort_tile,
@@ -3985,8 +4045,10 @@ int pp_sim_info(const pp_sim_t *s, uint32_t *sort_passes, uint32_t *sort_tile,
 }
 
 int pp_sim_set_sort(pp_sim_t *s, uint32_t bucket_shift) {
-	if (!s) return PP_INVALID_ARGS;
-	return pp_batch_set_sort(s->b, bucket_shift);
+	return guarded([&]() -> int {
+		if (!s) return PP_INVALID_ARGS;
+		return pp_batch_set_sort(s->b, bucket_shift);
+	});
 }
 
 int pp_sim_sync(pp_sim_t *s) {
@@ -3999,33 +4061,45 @@ int pp_sim_sync(pp_sim_t *s) {
 int pp_batch_create(uint64_t agent_capacity, uint64_t cell_capacity,
 	uint32_t max_replicas, uint32_t max_grass_restart, int device,
 	pp_batch_t **out) {
-	if (!out) return PP_INVALID_ARGS;
-	return batch_create(agent_capacity, cell_capacity, max_replicas,
-		max_grass_restart, device, out);
+	return guarded([&]() -> int {
+		if (!out) return PP_INVALID_ARGS;
+		return batch_create(agent_capacity, cell_capacity, max_replicas,
+			max_grass_restart, device, out);
+	});
 }
 
 void pp_batch_destroy(pp_batch_t *b) { batch_free(b); }
 
 uint64_t pp_batch_footprint(uint64_t agent_capacity, uint64_t cell_capacity,
 	uint32_t max_replicas, uint32_t max_grass_restart) {
-	if (agent_capacity == 0 || cell_capacity == 0 || max_replicas == 0 ||
-		max_grass_restart == 0 || agent_capacity >= (1ull << 29) ||
-		cell_capacity > 0xFFFFFFFEull)
+	try {
+		if (agent_capacity == 0 || cell_capacity == 0 || max_replicas == 0 ||
+			max_grass_restart == 0 || agent_capacity >= (1ull << 29) ||
+			cell_capacity > 0xFFFFFFFEull)
+			return 0;
+		return batch_bytes(agent_capacity, cell_capacity, max_replicas,
+			max_grass_restart, stats_rows_for(max_replicas),
+			max_hist_words(key_bits_for(cell_capacity), (uint32_t)agent_capacity));
+	} catch (...) {
 		return 0;
-	return batch_bytes(agent_capacity, cell_capacity, max_replicas,
-		max_grass_restart, stats_rows_for(max_replicas),
-		max_hist_words(key_bits_for(cell_capacity), (uint32_t)agent_capacity));
+	}
 }
 
 int pp_batch_add(pp_batch_t *b, const pp_params_t *p, uint64_t seed,
 	uint64_t agent_reserve, uint32_t *id) {
-	if (!b || !p) return PP_INVALID_ARGS;
-	return admit(b, p, seed, agent_reserve, p->iters, 1, id);
+	return guarded([&]() -> int {
+		if (!b || !p) return PP_INVALID_ARGS;
+		/* (UINT32_MAX is the unlimited count of a plain simulation) */
+		if (p->iters == ITERS_UNLIMITED) return PP_INVALID_ARGS;
+		return admit(b, p, seed, agent_reserve, p->iters, 1, id);
+	});
 }
 
 int pp_batch_run(pp_batch_t *b, uint32_t ticks) {
-	if (!b) return PP_INVALID_ARGS;
-	return advance(b, ticks);
+	return guarded([&]() -> int {
+		if (!b) return PP_INVALID_ARGS;
+		return advance(b, ticks);
+	});
 }
 
 int pp_batch_replica(pp_batch_t *b, uint32_t id, int *state, uint32_t *steps,
@@ -4054,15 +4128,17 @@ int pp_batch_rows(pp_batch_t *b, uint32_t id, uint32_t first, uint32_t n,
 
 int pp_batch_get_state(pp_batch_t *b, uint32_t id, uint32_t *n_agents,
 	uint32_t *cell, int32_t *energy, uint8_t *type, uint32_t *grass) {
-	if (!b) return PP_INVALID_ARGS;
-	auto it = b->reps.find(id);
-	if (it == b->reps.end()) return PP_INVALID_ARGS;
-	HIP_OK(hipSetDevice(b->device));
-	PP_TRY(read_ctl(b));
-	const Replica &r = it->second;
-	if (r.slot < 0) return PP_INVALID_ARGS;
-	if (n_agents) *n_agents = b->rctl_host[r.slot].n;
-	return state_of(b, r, cell, energy, type, grass);
+	return guarded([&]() -> int {
+		if (!b) return PP_INVALID_ARGS;
+		auto it = b->reps.find(id);
+		if (it == b->reps.end()) return PP_INVALID_ARGS;
+		HIP_OK(hipSetDevice(b->device));
+		PP_TRY(read_ctl(b));
+		const Replica &r = it->second;
+		if (r.slot < 0) return PP_INVALID_ARGS;
+		if (n_agents) *n_agents = b->rctl_host[r.slot].n;
+		return state_of(b, r, cell, energy, type, grass);
+	});
 }
 
 int pp_batch_release(pp_batch_t *b, uint32_t id) {
@@ -4095,7 +4171,7 @@ uint32_t pp_batch_tick(const pp_batch_t *b) {
 	return b ? b->ctl_host->tick : 0;
 }
 
-int pp_batch_counters(const pp_batch_t *b, uint64_t out[5]) {
+int pp_batch_counters(const pp_batch_t *b, uint64_t out[6]) {
 	if (!b || !out) return PP_INVALID_ARGS;
 	const Ctl *c = b->ctl_host;
 	out[0] = c->agent_steps;
@@ -4103,6 +4179,7 @@ int pp_batch_counters(const pp_batch_t *b, uint64_t out[5]) {
 	out[2] = c->sum_out;
 	out[3] = c->sum_occ;
 	out[4] = c->sum_eaten;
+	out[5] = c->sum_births;
 	return PP_SUCCESS;
 }
 
@@ -4118,65 +4195,97 @@ int pp_batch_info(const pp_batch_t *b, uint32_t *sort_passes,
 
 int pp_batch_set_shape(pp_batch_t *b, uint32_t keys_per_thread,
 	uint32_t max_digit_bits) {
-	if (!b) return PP_INVALID_ARGS;
-	if (keys_per_thread != 8 && keys_per_thread != 16) return PP_INVALID_ARGS;
-	if (max_digit_bits != 8 && max_digit_bits != 11) return PP_INVALID_ARGS;
-	const SortShape sh{(int)keys_per_thread, (int)max_digit_bits};
-	if (hist_bytes_for(sh, b->key_bits, b->P.cap) > b->hist_alloc_words * 4)
-		return PP_INVALID_ARGS;
-	HIP_OK(hipSetDevice(b->device));
-	if (sh.kpt == b->shape.kpt && sh.max_bits == b->shape.max_bits)
+	return guarded([&]() -> int {
+		if (!b) return PP_INVALID_ARGS;
+		if (keys_per_thread != 8 && keys_per_thread != 16) return PP_INVALID_ARGS;
+		if (max_digit_bits != 8 && max_digit_bits != 11) return PP_INVALID_ARGS;
+		const SortShape sh{(int)keys_per_thread, (int)max_digit_bits};
+		if (hist_bytes_for(sh, b->key_bits, b->P.cap) > b->hist_alloc_words * 4)
+			return PP_INVALID_ARGS;
+		HIP_OK(hipSetDevice(b->device));
+		if (sh.kpt == b->shape.kpt && sh.max_bits == b->shape.max_bits)
+			return PP_SUCCESS;
+		apply_shape(b, sh);
 		return PP_SUCCESS;
-	apply_shape(b, sh);
-	return PP_SUCCESS;
+	});
 }
 
 int pp_batch_set_sort(pp_batch_t *b, uint32_t mode) {
-	if (!b) return PP_INVALID_ARGS;
-	if (mode > 1 && (mode < 8 || mode > 11)) return PP_INVALID_ARGS;
-	HIP_OK(hipSetDevice(b->device));
-	if (mode == b->sort_mode) return PP_SUCCESS;
-	if (!b->ctl_fresh) PP_TRY(read_ctl(b));
-	b->sort_mode = mode;
-	return upload_table(b);    /* new layout; the next tick re-sorts */
+	return guarded([&]() -> int {
+		if (!b) return PP_INVALID_ARGS;
+		if (mode > 1 && (mode < 8 || mode > 11)) return PP_INVALID_ARGS;
+		HIP_OK(hipSetDevice(b->device));
+		if (mode == b->sort_mode) return PP_SUCCESS;
+		if (!b->ctl_fresh) PP_TRY(read_ctl(b));
+		b->sort_mode = mode;
+		return upload_table(b);    /* new layout; the next tick re-sorts */
+	});
 }
 
 int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks, uint32_t *bucket_shift,
 	uint32_t *keys_per_thread, uint32_t *max_digit_bits, double *sec_per_tick) {
-	if (!b || trial_ticks == 0 || trial_ticks + 8 > b->stats_rows / 2)
-		return PP_INVALID_ARGS;
-	HIP_OK(hipSetDevice(b->device));
-	/* candidates: the bucket sort at each block width, then the radix sort
-	 * at each shape */
-	struct Cand { uint32_t shift; SortShape sh; };
-	std::vector<Cand> cands;
-	for (uint32_t sft = 8; sft <= 11; sft++) cands.push_back({sft, b->shape});
-	for (const SortShape &sh : tune_candidates(b->key_bits, b->P.cap))
-		cands.push_back({0, sh});
-	Cand best{b->P.bucket_on ? b->P.bshift : 0u, b->shape};
-	double best_t = -1.0;
-	for (const Cand &c : cands) {
-		if (active_count(b) == 0) break;
-		PP_TRY(pp_batch_set_sort(b, c.shift));
-		PP_TRY(pp_batch_set_shape(b, (uint32_t)c.sh.kpt, (uint32_t)c.sh.max_bits));
-		if (c.shift && !b->P.bucket_on) continue;   /* layout too fine */
-		/* 8 untimed ticks re-sort if needed, build the graph, warm caches */
-		PP_TRY(enqueue_ticks(b, 8));
-		PP_TRY(collect(b));
-		double t = 0;
-		PP_TRY(time_ticks(b, trial_ticks, &t));
-		if (best_t < 0 || t < best_t) {
-			best_t = t;
-			best = c;
+	return guarded([&]() -> int {
+		if (!b || trial_ticks == 0 || trial_ticks + 8 > b->stats_rows / 2)
+			return PP_INVALID_ARGS;
+		HIP_OK(hipSetDevice(b->device));
+		/* candidates: the bucket sort at each block width, then the radix sort
+		 * at each shape */
+		struct Cand { uint32_t shift; SortShape sh; };
+		std::vector<Cand> cands;
+		for (uint32_t sft = 8; sft <= 11; sft++) cands.push_back({sft, b->shape});
+		for (const SortShape &sh : tune_candidates(b->key_bits, b->P.cap))
+			cands.push_back({0, sh});
+		const uint32_t mode0 = b->sort_mode;
+		const SortShape shape0 = b->shape;
+		Cand best{b->P.bucket_on ? b->P.bshift : 0u, b->shape};
+		double best_t = -1.0;
+		for (const Cand &c : cands) {
+			if (active_count(b) == 0) break;
+			PP_TRY(pp_batch_set_sort(b, c.shift));
+			PP_TRY(pp_batch_set_shape(b, (uint32_t)c.sh.kpt, (uint32_t)c.sh.max_bits));
+			if (c.shift && !b->P.bucket_on) continue;   /* layout too fine */
+			double t = 0;
+			/* 8 untimed ticks re-sort if needed, build the graph, warm
+			 * caches; a trial during which replicas finished (the table
+			 * changed) is settled and timed once more */
+			for (int attempt = 0; attempt < 2; attempt++) {
+				PP_TRY(enqueue_ticks(b, 8));
+				PP_TRY(collect(b));
+				const uint64_t v0 = b->table_version;
+				PP_TRY(time_ticks(b, trial_ticks, &t));
+				if (b->table_version == v0 || active_count(b) == 0) break;
+			}
+			if (best_t < 0 || t < best_t) {
+				best_t = t;
+				best = c;
+			}
 		}
-	}
-	PP_TRY(pp_batch_set_sort(b, best.shift));
-	PP_TRY(pp_batch_set_shape(b, (uint32_t)best.sh.kpt, (uint32_t)best.sh.max_bits));
-	if (bucket_shift) *bucket_shift = best.shift;
-	if (keys_per_thread) *keys_per_thread = (uint32_t)best.sh.kpt;
-	if (max_digit_bits) *max_digit_bits = (uint32_t)best.sh.max_bits;
-	if (sec_per_tick) *sec_per_tick = best_t < 0 ? 0.0 : best_t / trial_ticks;
-	return b->ctl_host->error;
+		if (best_t < 0) {
+			/* nothing was timed: the sort as it was */
+			PP_TRY(pp_batch_set_sort(b, mode0));
+			PP_TRY(pp_batch_set_shape(b, (uint32_t)shape0.kpt,
+				(uint32_t)shape0.max_bits));
+		} else {
+			/* the automatic choice when it is the winner (it then keeps
+			 * adapting to later replica sets), else the winner pinned */
+			const SortShape def = default_shape(b->key_bits);
+			PP_TRY(pp_batch_set_sort(b, 1));
+			PP_TRY(pp_batch_set_shape(b, (uint32_t)def.kpt, (uint32_t)def.max_bits));
+			const bool auto_wins = b->P.bucket_on ? best.shift == 11
+				: (best.shift == 0 && best.sh.kpt == def.kpt &&
+					best.sh.max_bits == def.max_bits);
+			if (!auto_wins) {
+				PP_TRY(pp_batch_set_sort(b, best.shift));
+				PP_TRY(pp_batch_set_shape(b, (uint32_t)best.sh.kpt,
+					(uint32_t)best.sh.max_bits));
+			}
+		}
+		if (bucket_shift) *bucket_shift = best.shift;
+		if (keys_per_thread) *keys_per_thread = (uint32_t)best.sh.kpt;
+		if (max_digit_bits) *max_digit_bits = (uint32_t)best.sh.max_bits;
+		if (sec_per_tick) *sec_per_tick = best_t < 0 ? 0.0 : best_t / trial_ticks;
+		return b->ctl_host->error;
+	});
 }
 
 int pp_batch_set_profiling(pp_batch_t *b, int enable) {
@@ -4189,16 +4298,20 @@ int pp_batch_prof_count(const pp_batch_t *b) { return b ? KI_COUNT : 0; }
 
 int pp_batch_prof_get(pp_batch_t *b, int i, const char **name,
 	uint64_t *launches, double *total_ns) {
-	if (!b) return PP_INVALID_ARGS;
-	return prof_get(b, i, name, launches, total_ns);
+	return guarded([&]() -> int {
+		if (!b) return PP_INVALID_ARGS;
+		return prof_get(b, i, name, launches, total_ns);
+	});
 }
 
 int pp_batch_prof_reset(pp_batch_t *b) {
-	if (!b) return PP_INVALID_ARGS;
-	const int rc = prof_collect(b);
-	memset(b->prof_launches, 0, sizeof(b->prof_launches));
-	memset(b->prof_ns, 0, sizeof(b->prof_ns));
-	return rc;
+	return guarded([&]() -> int {
+		if (!b) return PP_INVALID_ARGS;
+		const int rc = prof_collect(b);
+		memset(b->prof_launches, 0, sizeof(b->prof_launches));
+		memset(b->prof_ns, 0, sizeof(b->prof_ns));
+		return rc;
+	});
 }
 
 int pp_batch_sync(pp_batch_t *b) {

@@ -20,6 +20,12 @@ PP_UNABLE_TO_OPEN_PARAMS_FILE = -4
 PP_INVALID_PARAMS_FILE = -5
 PP_UNABLE_SAVE_STATS = -6
 PP_OUT_OF_RESOURCES = -8
+PP_ENERGY_OVERFLOW = -9   # no counterpart in the reference: energy limit
+
+# Event sums of pp_sim_counters / pp_batch_counters: agents at the start of
+# each iteration, alive after the move, after actions, occupied cells, grass
+# cells eaten, births.
+COUNTER_FIELDS = ("agent_steps", "live", "out", "occupied", "eaten", "births")
 
 # Field order of PPParameters (opencl/pp/pp_common.in.h:129-157) and the keys
 # of the parameter file that fill them (pp_common.c:55-167).
@@ -284,10 +290,9 @@ class Simulation:
 
     def counters(self) -> dict:
         """Exact sums over completed iterations since init."""
-        out = (ctypes.c_uint64 * 5)()
+        out = (ctypes.c_uint64 * 6)()
         self._check(_lib().pp_sim_counters(self._h, out), "counters")
-        return dict(zip(("agent_steps", "live", "out", "occupied", "eaten"),
-                        tuple(out)))
+        return dict(zip(COUNTER_FIELDS, tuple(out)))
 
     def info(self) -> dict:
         a, b, c, d = (ctypes.c_uint32() for _ in range(4))
@@ -410,7 +415,7 @@ class Batch:
         rid = ctypes.c_uint32()
         rc = _lib().pp_batch_add(self._h, ctypes.byref(params), seed, reserve,
                                  ctypes.byref(rid))
-        if rc == -8:
+        if rc == PP_OUT_OF_RESOURCES:
             return None
         self._check(rc, "add")
         self.params[rid.value] = params
@@ -475,10 +480,9 @@ class Batch:
         return _lib().pp_batch_tick(self._h)
 
     def counters(self) -> dict:
-        out = (ctypes.c_uint64 * 5)()
+        out = (ctypes.c_uint64 * 6)()
         self._check(_lib().pp_batch_counters(self._h, out), "counters")
-        return dict(zip(("agent_steps", "live", "out", "occupied", "eaten"),
-                        tuple(out)))
+        return dict(zip(COUNTER_FIELDS, tuple(out)))
 
     def info(self) -> dict:
         a, b, c, d = (ctypes.c_uint32() for _ in range(4))

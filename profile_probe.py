@@ -1,6 +1,6 @@
 """Workload driver for hardware-counter profiles of the step kernels.
 
-    python3 pphpc_amd/profile_probe.py [--grid 12800] [--steps 6] [--copy-mib 1024]
+    python3 profile_probe.py [--grid 12800] [--steps 6] [--copy-mib 1024]
                                        [--sort MODE]
 
 Runs, in one process:
@@ -20,7 +20,7 @@ import json
 import os
 import sys
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 

@@ -17,10 +17,14 @@ move with energy -1 and death below 1, grass countdown -1, then per cell in
 shuffled order: sheep eat live grass (countdown := R, +gain), wolves eat the
 first uneaten sheep of the list (+gain), reproduction splits energy
 (child floor(e/2)) when e > threshold and the draw is below the probability.
+Energy limit: an agent whose energy after its action exceeds 2^24 - 1 makes
+the step fail with -9 (the GPU's packed energy field is 24 bits wide).
 """
 from oracle.pp_oracle import philox
 
 SHEEP, WOLF = 0, 1
+ENERGY_MAX = (1 << 24) - 1
+ENERGY_OVERFLOW = -9
 
 
 def draw(key, a, step, stream, block):
@@ -83,6 +87,7 @@ class SpecModel:
             if self.countdown[xy] > 0:
                 self.countdown[xy] -= 1
         new_agents = []
+        over = False
         for (x, y) in sorted(cells, key=lambda xy: xy[1] * gx + xy[0]):
             lst = cells[(x, y)]
             c = y * gx + x
@@ -114,13 +119,18 @@ class SpecModel:
                     child = Agent(a.kind, x, y, a.energy // 2)
                     a.energy -= child.energy
                     a.child = child
+                if a.energy > ENERGY_MAX:
+                    over = True
             for a in lst:
                 if not a.eaten:
                     new_agents.append(a)
                 if a.child is not None:
                     new_agents.append(a.child)
+        if over:
+            return ENERGY_OVERFLOW
         self.agents = new_agents
         self.t += 1
+        return 0
 
     def state(self):
         gx = self.p["grid_x"]

@@ -74,3 +74,33 @@ def test_stats_parity_long(name, d, seed, iters):
     assert len(bad) == 0, (f"{name}: first differing row {bad[0]}: "
                            f"{rows[bad[0]]} != {ref[bad[0]]}")
     sim.close()
+
+
+def test_event_counters_match_oracle():
+    """The run's exact event sums (the counts the roofline's algorithmic
+    bytes are computed from) against the oracle's per-step events and
+    state: agents at the start of each iteration, survivors of the move,
+    agents after actions, grass eaten, births."""
+    from oracle.pp_oracle import Oracle
+    from pphpc_amd import Simulation
+    d = v2s(200)
+    _case("event counters v2s 200x200, 150 iterations")
+    sim = Simulation(gpu_params(d), seed=4, capacity=0, device=0)
+    orc = Oracle(d, seed=4, capacity=sim.capacity)
+    sim.init()
+    sim.run(150, collect=False)
+    c = sim.counters()
+    sim.close()
+    want = dict(agent_steps=0, live=0, out=0, eaten=0, births=0)
+    for _ in range(150):
+        n = orc.num_agents
+        assert orc.step() == 0
+        ev = orc.last_events()
+        want["agent_steps"] += n
+        want["live"] += n - ev["starved"]
+        want["out"] += orc.num_agents
+        want["eaten"] += ev["grass_eaten"]
+        want["births"] += ev["sheep_births"] + ev["wolf_births"]
+    orc.close()
+    assert {k: c[k] for k in want} == want
+    assert 0 < c["occupied"] <= c["live"]

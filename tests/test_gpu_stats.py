@@ -16,7 +16,7 @@ import sys
 
 import pytest
 
-from helpers import GOLDEN
+from helpers import GOLDEN, ROOT
 
 pytestmark = pytest.mark.gpu
 
@@ -59,12 +59,19 @@ def test_v2s_1600_replicas_equivalent_to_reference_rules():
     print(f"[stats] Java rules: {rep['raw_rejections']} of "
           f"{len(rep['measures'])} measures below alpha before Holm; "
           f"rejected after Holm: {rejected}", flush=True)
+    report = {"workload": "v2s 1600x1600, 4000 iterations, seeds 1..%d" % n,
+              "steady_from": fix["steady_from"], "java_rules": rep}
     if os.path.exists(NETLOGO):
         # Reported, not asserted: the NetLogo rule set differs from the Java
         # one in the wolf's choice of prey, which 30 replicas at this size
         # resolve (steady-state wolf count and energy).
         nl = compare(focal, json.load(open(NETLOGO))["focal"])
+        report["netlogo_rules"] = nl
         print(f"[stats] NetLogo rules: rejected after Holm: "
               f"{[m['measure'] for m in nl['measures'] if m['reject']]}",
               flush=True)
+    out_dir = os.path.join(ROOT, "gpurun_out")
+    if os.path.isdir(out_dir):
+        with open(os.path.join(out_dir, "stats_report_v2s1600.json"), "w") as f:
+            json.dump(report, f, indent=1)
     assert rep["pass"], rejected

@@ -62,3 +62,17 @@ def test_omp_capacity_failure_like_oracle(tmp_path):
                          "-t", "4"])
     assert r1.returncode == r2.returncode != 0
     assert r1.stderr.strip() == r2.stderr.strip()
+
+
+def test_omp_energy_limit_like_oracle(tmp_path):
+    """Both stop with -9 at the same iteration when a wolf's energy passes
+    2^24 - 1."""
+    d = dict(v2s(16, iters=600), wolves_gain_from_food=1 << 20,
+             wolves_reproduce_prob=0, sheep_reproduce_prob=60)
+    _write(tmp_path / "p.txt", d)
+    r1 = _run("pp_oracle", ["-p", str(tmp_path / "p.txt"), "-q", "-r", "3"])
+    r2 = _run("pp_omp", ["-p", str(tmp_path / "p.txt"), "-q", "-r", "3",
+                         "-t", "4"])
+    assert r1.returncode == r2.returncode == 256 - 9, (r1.stderr, r2.stderr)
+    assert "energy above its limit" in r1.stderr
+    assert r1.stderr.strip() == r2.stderr.strip()

@@ -283,6 +283,66 @@ def test_capacity_overflow_is_reported():
     o.close()
 
 
+# ------------------------------------------------------- energy limit
+
+ENERGY_MAX = (1 << 24) - 1
+
+
+def _step_rc(d, agents, grass, seed=0):
+    o = Oracle(d, seed=seed, capacity=64)
+    n = len(agents)
+    o.set_state(np.zeros(n), [e for _, e in agents], [t for t, _ in agents],
+                [grass])
+    rc = o.step()
+    o.close()
+    return rc
+
+
+def test_energy_limit_boundary():
+    """Energy may reach 2^24 - 1 after an action; one more fails with -9."""
+    g = 1 << 20
+    d = _one_cell(wolves_gain_from_food=g, sheep_gain_from_food=g)
+    # wolf: e - 1 + gain after eating the sheep
+    assert _step_rc(d, [(0, 5), (1, ENERGY_MAX - g + 1)], 5) == 0
+    assert _step_rc(d, [(0, 5), (1, ENERGY_MAX - g + 2)], 5) == -9
+    # sheep on live grass
+    assert _step_rc(d, [(0, ENERGY_MAX - g + 1)], 0) == 0
+    assert _step_rc(d, [(0, ENERGY_MAX - g + 2)], 0) == -9
+    # the move alone never exceeds the limit it started under
+    assert _step_rc(d, [(0, ENERGY_MAX)], 3) == 0
+
+
+def test_energy_limit_applies_after_the_split():
+    """The parent keeps ceil(e/2): above the limit only if that half is."""
+    d = _one_cell(sheep_gain_from_food=1 << 20, sheep_reproduce_prob=100)
+    e_max_ok = 2 * ENERGY_MAX - (1 << 20) + 1   # e - 1 + gain = 2 ENERGY_MAX
+    assert _step_rc(d, [(0, e_max_ok)], 0) == 0
+    assert _step_rc(d, [(0, e_max_ok + 1)], 0) == -9
+    out, _, _ = _run_one(d, [(0, e_max_ok)], grass=0)
+    assert out == [(0, ENERGY_MAX), (0, ENERGY_MAX)]
+
+
+def test_energy_limit_run_matches_spec():
+    """Wolves with a large gain and no reproduction grow until one passes
+    the limit: the oracle fails with -9 at the iteration the independent
+    statement does, and every row before it agrees."""
+    d = dict(v2s(gx=8, gy=8), init_sheep=60, init_wolves=6,
+             wolves_gain_from_food=1 << 20, wolves_reproduce_prob=0,
+             sheep_reproduce_prob=60)
+    o = Oracle(d, seed=7)
+    m = SpecModel(d, 7)
+    fail = None
+    for t in range(400):
+        rc_o, rc_m = o.step(), m.step()
+        assert rc_o == rc_m, f"step {t}: oracle {rc_o}, spec {rc_m}"
+        if rc_o:
+            fail = t
+            break
+        assert o.stats() == m.stats(), f"stats after {t + 1}"
+    o.close()
+    assert fail is not None and fail > 10 and rc_o == -9
+
+
 # ------------------------------------------------------- planted mistakes
 
 @pytest.mark.parametrize("mode", [1, 2, 3])
