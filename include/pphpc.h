@@ -202,6 +202,23 @@ int pp_sim_set_sort(pp_sim_t *s, uint32_t mode);
 /* Releases all device memory of the handle. NULL is accepted. */
 void pp_sim_destroy(pp_sim_t *s);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under stream/graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): the run of pp_sim_create + pp_sim_simulate +
+ * pp_sim_destroy with the same results (rows 0..p->iters, iters + 1 rows,
+ * into the caller's host array `rows`), every device buffer taken from a
+ * fixed 12 MiB scratch region of the library's device memory instead of
+ * allocated. It calls no hipMalloc, hipFree, pinned-host or other allocating
+ * routine, and captures no graph (kernels go to the null stream of `device`,
+ * < 0 = current). Limits: capacity (0 = 4 x initial agents) at most 262,144
+ * agents, grid_xy at most 262,144 cells, grass_restart at most 4096; larger
+ * sizes return PP_INVALID_ARGS with a message on stderr. Each GPU holds its
+ * own scratch until the process ends and every call on it reuses (and
+ * re-initialises) that one region: calls that overlap on one GPU race.
+ * Returns 0 or a PP_* code as pp_sim_simulate. */
+int pp_sim_simulate_noalloc(const pp_params_t *p, uint64_t seed,
+	uint64_t capacity, int device, pp_stats64_t *rows);
+
 /* Blocks until all work of the handle's stream has finished. */
 int pp_sim_sync(pp_sim_t *s);
 

@@ -20,6 +20,7 @@ from .sim import (  # noqa: F401
     load_params,
     stats_save,
     stats_rows_to_array,
+    simulate_noalloc,
     strerror,
     library_path,
     PP_SUCCESS,

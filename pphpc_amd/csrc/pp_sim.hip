@@ -71,6 +71,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -3455,8 +3456,8 @@ static uint64_t max_hist_words(uint32_t key_bits, uint32_t cap) {
 }
 
 /* The device buffers of a batch, each taken from take(&ptr, bytes): device
- * allocations (batch_alloc), or slices of a fixed region.
- * b->hist_alloc_words is set by the caller. */
+ * allocations (batch_alloc), or slices of the static scratch region
+ * (pp_sim_simulate_noalloc). b->hist_alloc_words is set by the caller. */
 template <typename Take>
 static int batch_buffers(pp_batch_t *b, Take &&take) {
 	const uint64_t A = b->P.cap, C = b->P.C;
@@ -3821,6 +3822,213 @@ static int time_ticks(pp_batch_t *b, uint32_t ticks, double *sec) {
 	return collect(b);
 }
 
+/* ------------------------------------------------ allocation-free runs */
+
+/* Limits of pp_sim_simulate_noalloc: one plain simulation of at most
+ * NOALLOC_MAX_AGENTS agents (its capacity) on at most NOALLOC_MAX_CELLS cells
+ * with grass restart at most NOALLOC_MAX_R, every device buffer a slice of
+ * `scratch` (about 10 MiB at the limits). */
+constexpr uint64_t NOALLOC_MAX_AGENTS = 1u << 18;
+constexpr uint64_t NOALLOC_MAX_CELLS = 1u << 18;
+constexpr uint32_t NOALLOC_MAX_R = 4096;
+constexpr size_t NOALLOC_BYTES = 12u << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+/* The device scratch of pp_sim_simulate_noalloc. Each GPU holds its own copy
+ * until the process ends. In host code the name is a host-side stand-in, not
+ * a device address: host code uses only the address hipGetSymbolAddress
+ * returns for the current device (noalloc_scratch). */
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static void *g_scratch_dev[NOALLOC_MAX_DEVICES];   /* its address per device */
+static std::mutex g_scratch_mu;
+/* Host copies of the control words of the run on each device. */
+static Ctl g_na_ctl[NOALLOC_MAX_DEVICES];
+static RepCtl g_na_rctl[NOALLOC_MAX_DEVICES];
+
+/* scratch's device address on device dev (current), looked up on the first
+ * call on that device and kept. */
+static int noalloc_scratch(int dev, unsigned char **out) {
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if (!g_scratch_dev[dev]) {
+		void *p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p) {
+			fprintf(stderr, "pphpc: hipGetSymbolAddress(scratch) on device %d: "
+				"%s\n", dev, hipGetErrorName(e));
+			return PP_LIBRARY_ERROR;
+		}
+		g_scratch_dev[dev] = p;
+	}
+	*out = (unsigned char *)g_scratch_dev[dev];
+	return PP_SUCCESS;
+}
+
+/* pp_sim_create + pp_sim_simulate + pp_sim_destroy with every buffer carved
+ * from the scratch region, on the null stream, without graphs: no device or
+ * pinned-host allocation and no heap allocation on the host (the batch record
+ * lives on the stack and its containers stay empty). Rows 0..iters go to
+ * rows. */
+static int simulate_noalloc(const pp_params_t *p, uint64_t seed,
+	uint64_t capacity, int device, pp_stats64_t *rows) {
+	if (!p || !rows) return PP_INVALID_ARGS;
+	PP_TRY(check_params(p));
+	const uint64_t C = (uint64_t)p->grid_x * p->grid_y;
+	const uint64_t n0 = (uint64_t)p->init_sheep + p->init_wolves;
+	if (capacity == 0) capacity = std::max<uint64_t>(4 * n0, 1);
+	if (capacity < n0) return PP_INVALID_ARGS;
+	if (capacity > NOALLOC_MAX_AGENTS || C > NOALLOC_MAX_CELLS ||
+		p->grass_restart > NOALLOC_MAX_R) {
+		fprintf(stderr, "pphpc: pp_sim_simulate_noalloc: capacity %llu, %llu "
+			"cells, grass restart %u; the limits are %llu agents, %llu cells, "
+			"grass restart %u\n", (unsigned long long)capacity,
+			(unsigned long long)C, p->grass_restart,
+			(unsigned long long)NOALLOC_MAX_AGENTS,
+			(unsigned long long)NOALLOC_MAX_CELLS, NOALLOC_MAX_R);
+		return PP_INVALID_ARGS;
+	}
+	if ((uint64_t)p->iters + p->grass_restart + 1 >= 0xFFFFFFFFull)
+		return PP_INVALID_ARGS;
+	int dev = device;
+	if (dev < 0) HIP_OK(hipGetDevice(&dev));
+	if (dev >= NOALLOC_MAX_DEVICES) return PP_INVALID_ARGS;
+	HIP_OK(hipSetDevice(dev));
+	unsigned char *base = nullptr;
+	PP_TRY(noalloc_scratch(dev, &base));
+
+	pp_batch_t bb;
+	pp_batch_t *b = &bb;
+	b->device = dev;
+	if (hipDeviceGetAttribute(&b->num_cu, hipDeviceAttributeMultiprocessorCount,
+			dev) != hipSuccess || b->num_cu <= 0)
+		b->num_cu = 256;
+	b->stream = nullptr;
+	b->no_graph = 1;
+	b->P.cap = (uint32_t)capacity;
+	b->P.C = (uint32_t)C;
+	b->P.R = p->grass_restart;
+	b->P.nb_cap = nb_cap_for(C);
+	b->P.bshift = 11;
+	b->sort_mode = 1;
+	b->max_rep = 1;
+	b->stats_rows = STATS_RING;
+	b->key_bits = key_bits_for(C);
+	b->grass_bytes = grass_bytes_for(p->grass_restart);
+	apply_shape(b, default_shape(b->key_bits));
+	b->hist_alloc_words = hist_bytes_for(b->shape, b->key_bits, b->P.cap) / 4;
+	size_t off = 0;
+	PP_TRY(batch_buffers(b, [&](void **ptr, size_t bytes) {
+		off = (off + 255) & ~(size_t)255;
+		*ptr = off + bytes <= NOALLOC_BYTES ? base + off : base;
+		off += bytes;
+		return hipSuccess;
+	}));
+	if (off > NOALLOC_BYTES) {
+		/* (the limits above keep every layout inside) */
+		fprintf(stderr, "pphpc: pp_sim_simulate_noalloc: %zu bytes of scratch "
+			"needed, %zu available\n", off, NOALLOC_BYTES);
+		return PP_INVALID_ARGS;
+	}
+	b->ctl_host = &g_na_ctl[dev];
+	b->rctl_host = &g_na_rctl[dev];
+
+	/* The scratch keeps the last call's contents: clear what must start at
+	 * zero (control block, partial sums, histograms, the replica's control
+	 * words, the bucket counts). */
+	const hipStream_t st = b->stream;
+	k_batch_reset<<<min_u32(ceil_div(b->hist_alloc_words, WG), 2048), WG, 0,
+		st>>>(b->ctl, b->hist, (uint32_t)b->hist_alloc_words, b->P.part);
+	HIP_OK(hipGetLastError());
+	HIP_OK(hipMemsetAsync(b->rctl_dev, 0, sizeof(RepCtl), st));
+	const size_t nbw = cnt_words(b->P.nb_cap);
+	HIP_OK(hipMemsetAsync(b->ocnt, 0, nbw * 4, st));
+	HIP_OK(hipMemsetAsync(b->ncnt, 0, nbw * 4, st));
+
+	/* The replica in slot 0, cells from 0 (as admit and upload_table do
+	 * for the one replica of a plain simulation). */
+	Rep rp{};
+	rp.cb = 0;
+	rp.C = (uint32_t)C;
+	rp.gx = p->grid_x;
+	div_magic(rp.gx, &rp.gx_magic, &rp.gx_shift);
+	rp.gy = p->grid_y;
+	rp.R = p->grass_restart;
+	rp.sheep_gain = p->sheep_gain_from_food;
+	rp.wolves_gain = p->wolves_gain_from_food;
+	rp.sheep_thr = p->sheep_reproduce_threshold;
+	rp.wolves_thr = p->wolves_reproduce_threshold;
+	rp.sheep_prob = p->sheep_reproduce_prob;
+	rp.wolves_prob = p->wolves_reproduce_prob;
+	rp.init_sheep = p->init_sheep;
+	rp.n0 = (uint32_t)n0;
+	rp.key0 = (uint32_t)seed;
+	rp.key1 = (uint32_t)(seed >> 32);
+	rp.t0 = 0;
+	rp.iters = p->iters;
+	rp.cap = (uint32_t)capacity;
+	rp.stats_rows = b->stats_rows;
+	rp.ring = b->ring;
+	rp.stats = b->stats_dev;
+	const uint32_t shift = 11;
+	const uint32_t h = rows_per_bucket(rp.gx, shift);
+	const uint64_t nbk = (uint64_t)((rp.gy + h - 1) / h) * row_blocks(rp.gx, shift);
+	const uint32_t on = (nbk <= b->P.nb_cap && nbk <= CNT_MAX_BUCKETS) ? 1u : 0u;
+	uint32_t top = 0;
+	while (top + 1 < (uint32_t)CNT_LEVELS && (nbk >> (6 * (top + 1))) > 0) top++;
+	b->P.bucket_on = on;
+	b->P.bshift = shift;
+	b->P.cnt_top = top;
+	b->P.one_slot = 0;
+	const uint32_t zero = 0, one = 1, nb32 = on ? (uint32_t)nbk : 0u;
+	const uint32_t bbase[2] = {0, (uint32_t)nbk};
+	HIP_OK(hipMemcpy(b->rep_dev, &rp, sizeof(Rep), hipMemcpyHostToDevice));
+	HIP_OK(hipMemcpy(b->cbs_dev, &zero, 4, hipMemcpyHostToDevice));
+	HIP_OK(hipMemcpy(b->order_dev, &zero, 4, hipMemcpyHostToDevice));
+	HIP_OK(hipMemcpy(b->bbase_dev, bbase, 8, hipMemcpyHostToDevice));
+	HIP_OK(hipMemcpy(&b->ctl->nrep, &one, 4, hipMemcpyHostToDevice));
+	HIP_OK(hipMemcpy(&b->ctl->nbuckets, &nb32, 4, hipMemcpyHostToDevice));
+	b->need_radix = 1;
+
+	const Tables T = b->tables();
+	const uint32_t cu = (uint32_t)b->num_cu;
+	k_rep_reset<<<1, WG, 0, st>>>(T, 0);
+	const uint32_t gc = min_u32(ceil_div(C, WG), 8 * cu);
+	if (b->grass_bytes == 1)
+		k_init_cells<uint8_t><<<gc, WG, 0, st>>>((uint8_t *)b->grass_at, T, 0);
+	else if (b->grass_bytes == 2)
+		k_init_cells<uint16_t><<<gc, WG, 0, st>>>((uint16_t *)b->grass_at, T, 0);
+	else
+		k_init_cells<uint32_t><<<gc, WG, 0, st>>>((uint32_t *)b->grass_at, T, 0);
+	const uint32_t ga = min_u32(ceil_div(n0 ? n0 : 1, WG), 8 * cu);
+	k_init_agents<<<ga, WG, 0, st>>>(b->cell, b->energy, b->type, T, 0, 0);
+	k_rep_init_done<<<1, 64, 0, st>>>(b->ctl, T, 0, 0);
+	k_move0<<<ga, WG, 0, st>>>(b->cell, b->energy, b->type, b->keys[0],
+		b->vals[0], b->ctl, T, 0, 0, b->P);
+	HIP_OK(hipGetLastError());
+	HIP_OK(hipMemcpy(&rows[0], b->stats_dev, sizeof(pp_stats64_t),
+		hipMemcpyDeviceToHost));
+
+	/* Chunks of at most half the stats ring, rows copied after each. */
+	uint32_t done = 0;
+	while (done < p->iters) {
+		const uint32_t chunk = min_u32(p->iters - done, b->stats_rows / 2);
+		PP_TRY(enqueue_ticks(b, chunk));
+		HIP_OK(hipMemcpy(b->ctl_host, b->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+		HIP_OK(hipMemcpy(b->rctl_host, b->rctl_dev, sizeof(RepCtl),
+			hipMemcpyDeviceToHost));
+		if (b->ctl_host->error) return b->ctl_host->error;
+		const RepCtl &rc = *b->rctl_host;
+		const uint32_t last = rc.step;   /* rows 0..last exist */
+		for (uint32_t i = done + 1; i <= last; i++)
+			HIP_OK(hipMemcpy(&rows[i], b->stats_dev + i % b->stats_rows,
+				sizeof(pp_stats64_t), hipMemcpyDeviceToHost));
+		if (rc.status == RS_FAILED) return rc.error;
+		if (last != done + chunk) return PP_LIBRARY_ERROR;
+		done = last;
+	}
+	return PP_SUCCESS;
+}
+
 /* Runs the body of an entry point: no C++ exception crosses the C ABI (an
  * allocation that fails becomes PP_OUT_OF_RESOURCES, anything else
  * PP_LIBRARY_ERROR). */
@@ -4057,6 +4265,13 @@ int pp_sim_sync(pp_sim_t *s) {
 }
 
 /* ---------------------------------------------------------------- batches */
+
+int pp_sim_simulate_noalloc(const pp_params_t *p, uint64_t seed,
+	uint64_t capacity, int device, pp_stats64_t *rows) {
+	return guarded([&]() -> int {
+		return simulate_noalloc(p, seed, capacity, device, rows);
+	});
+}
 
 int pp_batch_create(uint64_t agent_capacity, uint64_t cell_capacity,
 	uint32_t max_replicas, uint32_t max_grass_restart, int device,

@@ -129,6 +129,8 @@ def _lib():
         "pp_sim_destroy": (None, [vp]),
         "pp_sim_sync": (i32, [vp]),
         "pp_sim_set_sort": (i32, [vp, u32]),
+        "pp_sim_simulate_noalloc": (i32, [P(Params), u64, u64, i32,
+                                          P(Stats)]),
         "pp_sim_dump": (i32, [vp, ctypes.c_char_p, ctypes.c_char_p, u32,
                               i32]),
         "pp_dump_state": (i32, [ctypes.c_char_p, ctypes.c_char_p, i32, u32,
@@ -190,6 +192,19 @@ def stats_save(path: Optional[str], rows, params: Params) -> None:
                               len(arr), ctypes.byref(params))
     if rc:
         raise PPError(rc, f"stats_save({path!r})")
+
+
+def simulate_noalloc(params: Params, seed: int = 0, capacity: int = 0,
+                     device: int = -1) -> np.ndarray:
+    """pp_sim_simulate_noalloc: rows 0..iters of one simulation run in the
+    library's fixed device scratch (no device allocation), as an (iters+1, 7)
+    uint64 array."""
+    out = (Stats * (params.iters + 1))()
+    rc = _lib().pp_sim_simulate_noalloc(ctypes.byref(params), seed, capacity,
+                                        device, out)
+    if rc:
+        raise PPError(rc, "simulate_noalloc")
+    return stats_rows_to_array(out)
 
 
 def stats_rows_to_array(rows) -> np.ndarray:
