@@ -119,10 +119,20 @@ constexpr uint32_t RS_ACTIVE = 1;
 constexpr uint32_t RS_DONE = 2;
 constexpr uint32_t RS_FAILED = 3;
 
-/* Counters k_act accumulates per replica for the statistics row; AC_OVF
- * counts agents above the energy limit, AC_BIRTH the newborns. */
+/* Counters k_act accumulates per replica for the statistics row. Two hold a
+ * second count in their high 32 bits (each half stays below 2^29, the agent
+ * capacity, so no carry crosses): AC_OCC the occupied cells (low) and the
+ * agents above the energy limit (high), AC_OUT the outputs (low) and the
+ * newborns among them (high). */
 enum { AC_SHEEP = 0, AC_WOLVES, AC_SHEEP_EN, AC_WOLVES_EN, AC_EATEN, AC_OCC,
-	AC_OUT, AC_OVF, AC_BIRTH, NACC };
+	AC_OUT, NACC };
+constexpr unsigned long long HI32 = 1ull << 32;
+__host__ __device__ __forceinline__ uint32_t lo32(unsigned long long v) {
+	return (uint32_t)v;
+}
+__host__ __device__ __forceinline__ uint32_t hi32(unsigned long long v) {
+	return (uint32_t)(v >> 32);
+}
 
 /* Per-replica constants, written by the host at admission. */
 struct Rep {
@@ -231,7 +241,7 @@ struct Params {
  * [0, NACC) of a slot: the action counters of a one-replica batch, summed by
  * k_act_scan; word PART_LIVE: the move's survivors, summed into Ctl::live by
  * the next tick's first kernel. Both consumers clear what they sum. */
-constexpr uint32_t PART_SLOTS = 64, PART_STRIDE = 16, PART_LIVE = 12;
+constexpr uint32_t PART_SLOTS = 64, PART_STRIDE = 16, PART_LIVE = 8;
 static_assert(PART_SLOTS == (uint32_t)WAVE, "one wave sums the slots");
 static_assert(NACC <= (int)PART_LIVE && PART_LIVE < PART_STRIDE,
 	"slot layout");
@@ -1669,8 +1679,8 @@ __device__ __forceinline__ void mix_add(unsigned long long (*s_mix)[NACC],
 __device__ __noinline__ void wave_flush(unsigned long long a0,
 	unsigned long long a1, unsigned long long a2, unsigned long long a3,
 	unsigned long long a4, unsigned long long a5, unsigned long long a6,
-	unsigned long long a7, unsigned long long a8, unsigned long long *dst) {
-	unsigned long long v[NACC] = {a0, a1, a2, a3, a4, a5, a6, a7, a8};
+	unsigned long long *dst) {
+	unsigned long long v[NACC] = {a0, a1, a2, a3, a4, a5, a6};
 	const int lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
 	for (int k = 0; k < NACC; k++) {
@@ -1699,9 +1709,9 @@ __device__ __forceinline__ void act_wg_add(unsigned long long (&acc)[NACC],
 	const uint32_t ns = wave_sum32((uint32_t)acc[AC_SHEEP]);
 	const uint32_t nw = wave_sum32((uint32_t)acc[AC_WOLVES]);
 	const uint32_t ne = wave_sum32((uint32_t)acc[AC_EATEN]);
-	const uint32_t no = wave_sum32((uint32_t)acc[AC_OCC]);
-	const uint32_t nv = wave_sum32((uint32_t)acc[AC_OVF]);
-	const uint32_t nb = wave_sum32((uint32_t)acc[AC_BIRTH]);
+	const uint32_t no = wave_sum32(lo32(acc[AC_OCC]));
+	const uint32_t nv = wave_sum32(hi32(acc[AC_OCC]));
+	const uint32_t nb = wave_sum32(hi32(acc[AC_OUT]));
 	unsigned long long es, ew;
 	if (__ballot(((acc[AC_SHEEP_EN] | acc[AC_WOLVES_EN]) >> 48) != 0) == 0) {
 		es = wave_sum32((uint32_t)(acc[AC_SHEEP_EN] & 0xFFFFFFu))
@@ -1718,10 +1728,8 @@ __device__ __forceinline__ void act_wg_add(unsigned long long (&acc)[NACC],
 		s_red[w][AC_SHEEP_EN] = es;
 		s_red[w][AC_WOLVES_EN] = ew;
 		s_red[w][AC_EATEN] = ne;
-		s_red[w][AC_OCC] = no;
-		s_red[w][AC_OUT] = (unsigned long long)ns + nw;
-		s_red[w][AC_OVF] = nv;
-		s_red[w][AC_BIRTH] = nb;
+		s_red[w][AC_OCC] = no + nv * HI32;
+		s_red[w][AC_OUT] = (unsigned long long)(ns + nw) + nb * HI32;
 	}
 #pragma unroll
 	for (int k = 0; k < NACC; k++) acc[k] = 0;
@@ -1752,7 +1760,7 @@ struct ActLds {
  * the table. MIXED = false: one replica (ra), counters summed into acc;
  * MIXED = true: several, each cell looks up its replica and counters go
  * through LDS slots. See k_act for the rules. Agents whose energy after
- * their action exceeds V_ENERGY are counted in AC_OVF. */
+ * their action exceeds V_ENERGY are counted in the high half of AC_OCC. */
 template <typename GT, bool MIXED>
 __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 	uint32_t cnt, const uint32_t *__restrict__ keys, uint32_t *vals,
@@ -1850,10 +1858,8 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 		acc[AC_SHEEP_EN] += f_se;
 		acc[AC_WOLVES_EN] += f_we;
 		acc[AC_EATEN] += f_eaten;
-		acc[AC_OCC] += f_occ;
-		acc[AC_OUT] += f_sheep + f_wolves;
-		acc[AC_OVF] += f_ovf;
-		acc[AC_BIRTH] += f_birth;
+		acc[AC_OCC] += f_occ + f_ovf * HI32;
+		acc[AC_OUT] += (f_sheep + f_wolves) + f_birth * HI32;
 	}
 	__syncthreads();
 	const uint32_t nslow = S.nslow;
@@ -1955,10 +1961,9 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 			sc[AC_WOLVES] = n_wolves;
 			sc[AC_SHEEP_EN] = e_sheep;
 			sc[AC_WOLVES_EN] = e_wolves;
-			sc[AC_OUT] = n_sheep + n_wolves;
+			sc[AC_OUT] = (n_sheep + n_wolves) + births * HI32;
 			sc[AC_EATEN] = ate ? 1 : 0;
-			sc[AC_OVF] = ovf;
-			sc[AC_BIRTH] = births;
+			sc[AC_OCC] = 1 + ovf * HI32;
 		} else {
 			if (k > 1) {
 				uint4 r = make_uint4(0, 0, 0, 0);
@@ -2006,7 +2011,7 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 					nv |= V_CHILD | ((e & 1) ? V_ODD : 0u);
 					e -= e / 2;
 				}
-				sc[AC_OVF] += e > (int32_t)V_ENERGY;
+				if (e > (int32_t)V_ENERGY) sc[AC_OCC] += HI32;
 				sv.set(q, nv | ((uint32_t)e & V_ENERGY));
 			}
 			if (ate) grass_cell(grass_at, c) = (GT)(tick + 1 + ra.R);
@@ -2025,8 +2030,7 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 					sc[AC_SHEEP] += outs;
 					sc[AC_SHEEP_EN] += ea;
 				}
-				sc[AC_OUT] += outs;
-				sc[AC_BIRTH] += (v & V_CHILD) != 0;
+				sc[AC_OUT] += outs + ((v & V_CHILD) ? HI32 : 0ull);
 				if (q < (uint32_t)A_TILE) nout += outs;
 				else if (outs) tile_add(act_cnt, l1o, tile + q / A_TILE, outs);
 			}
@@ -2177,7 +2181,7 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 			if (k_first != acc_k) {
 				if (acc_k != NO_REP) {
 					wave_flush(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6],
-						acc[7], acc[8], &T.rctl[T.order[acc_k]].acc[0]);
+						&T.rctl[T.order[acc_k]].acc[0]);
 #pragma unroll
 					for (int i = 0; i < NACC; i++) acc[i] = 0;
 				}
@@ -2345,9 +2349,9 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 			rc.base = 0;
 			if (rc1.status == RS_ACTIVE) {
 				const uint32_t s = tick - rp1.t0;
-				if (a[AC_OVF] || out > rp1.cap) {
+				if (hi32(a[AC_OCC]) || out > rp1.cap) {
 					rc.status = RS_FAILED;
-					rc.error = a[AC_OVF] ? PP_ENERGY_OVERFLOW : PP_OUT_OF_RESOURCES;
+					rc.error = hi32(a[AC_OCC]) ? PP_ENERGY_OVERFLOW : PP_OUT_OF_RESOURCES;
 					rc.err_step = s;
 					rc.n = 0;
 				} else {
@@ -2364,9 +2368,9 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 					write_row(a, (unsigned long long)rp1.C - D, S,
 						&rp1.stats[(s + 1) % rp1.stats_rows]);
 					bs[0] = rc1.n;
-					bs[1] = a[AC_OCC];
+					bs[1] = lo32(a[AC_OCC]);
 					bs[2] = E;
-					bs[3] = a[AC_BIRTH];
+					bs[3] = hi32(a[AC_OUT]);
 					rc.agent_steps = rc1.agent_steps + rc1.n;
 					rc.step = s + 1;
 					rc.n = out;
@@ -2400,9 +2404,9 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 			rc.base = carry + pre;
 			if (rc.status == RS_ACTIVE) {
 				const uint32_t s = tick - rp.t0;
-				if (a[AC_OVF] || out > rp.cap) {
+				if (hi32(a[AC_OCC]) || out > rp.cap) {
 					rc.status = RS_FAILED;
-					rc.error = a[AC_OVF] ? PP_ENERGY_OVERFLOW : PP_OUT_OF_RESOURCES;
+					rc.error = hi32(a[AC_OCC]) ? PP_ENERGY_OVERFLOW : PP_OUT_OF_RESOURCES;
 					rc.err_step = s;
 					rc.n = 0;
 				} else {
@@ -2420,9 +2424,9 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 					rc.cd_sum = S;
 					write_row(a, rc, rp, &rp.stats[(s + 1) % rp.stats_rows]);
 					bs[0] += rc.n;
-					bs[1] += a[AC_OCC];
+					bs[1] += lo32(a[AC_OCC]);
 					bs[2] += E;
-					bs[3] += a[AC_BIRTH];
+					bs[3] += hi32(a[AC_OUT]);
 					rc.agent_steps += rc.n;
 					rc.step = s + 1;
 					rc.n = out;

@@ -137,3 +137,40 @@ def test_batch_footprint_counts_every_buffer():
     f = pp.Batch.footprint(A, C, 8, 10)
     assert 33 * A + C < f < 33 * A + C + 8 * 1024 * 56 + (40 << 20)
     assert pp.Batch.footprint(A, C, 8, 300) - f == C + 8 * 290 * 8
+
+
+def test_step_kernels_do_not_spill(tmp_path):
+    """The step kernels keep everything in registers: a spill to scratch
+    memory slows a kernel many times over while every result stays the
+    same, so no parity test sees it. Compiles the library's HIP source with
+    the compiler's resource remarks and checks each step kernel's scratch
+    (the per-bucket sort keeps the 36 bytes a lane it has always had)."""
+    import re
+    src = os.path.join(ROOT, "pphpc_amd", "csrc", "pp_sim.hip")
+    r = subprocess.run(
+        ["/opt/rocm/bin/hipcc", "-O3", "-std=c++17", "-fPIC",
+         "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+         "-packed-fp32-ops", "-I" + os.path.join(ROOT, "include"), "-c",
+         "-o", str(tmp_path / "x.o"), src,
+         "-Rpass-analysis=kernel-resource-usage"],
+        capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    scratch, name = {}, None
+    for line in r.stderr.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+            continue
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and name:
+            scratch[name] = int(m.group(1))
+    limits = {"k_act": 0, "k_act_mixed": 0, "k_emit_move": 0,
+              "k_act_scan": 0, "k_sort_down": 0,
+              "k_sort_up": 0, "k_bucket_sort": 36}
+    seen = set()
+    for mangled, size in scratch.items():
+        for k, lim in limits.items():
+            if re.search(r"\d%s[IE]" % re.escape(k), mangled):
+                seen.add(k)
+                assert size <= lim, f"{k} ({mangled}) spills {size} B/lane"
+    assert seen == set(limits), set(limits) - seen
