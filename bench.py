@@ -429,6 +429,10 @@ def serving_leg(seconds, rate, hbm_gb, chunk, draw_key=5):
     with Server(int(hbm_gb * (1 << 30)), chunk=chunk) as srv:
         reqs, summary = srv.run(stream, log=log)
     summary.update({"arrival_rate_per_s": rate, "window_s": seconds,
+                    "window_note": "arrivals over a 60 s window (the workload "
+                                   "names 10 minutes; the bench keeps its run "
+                                   "to minutes); the loop then serves the "
+                                   "queue to its end",
                     "chunk_ticks": chunk, "hbm_budget_gib": hbm_gb,
                     "sides": "log-uniform over 100..6400",
                     "iters": "uniform 1000..4000"})
@@ -461,7 +465,7 @@ def main():
                     help="replica counts of the 1600x1600 batched-vs-plain "
                          "leg (empty: skip)")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
-    ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-hbm-gb", type=float, default=48.0)
     ap.add_argument("--serve-chunk", type=int, default=32)
     ap.add_argument("--no-serve", action="store_true")

@@ -315,14 +315,26 @@ int pp_batch_set_sort(pp_batch_t *b, uint32_t mode);
 int pp_batch_set_shape(pp_batch_t *b, uint32_t keys_per_thread,
 	uint32_t max_digit_bits);
 
+/* Launch shape, the counterpart of the reference's work sizes
+ * (ppg_worksizes_compute, pp_gpu.c:1001-1158): the action and emit kernels
+ * launch tile_rounds (1, 2, 4 or 8; default 4) rounds of the work-groups the
+ * GPU holds at once, the bucket sort bucket_eighths eighths (1..8; default 8)
+ * of a round. Results do not depend on it. */
+int pp_batch_set_launch(pp_batch_t *b, uint32_t tile_rounds,
+	uint32_t bucket_eighths);
+int pp_batch_get_launch(const pp_batch_t *b, uint32_t *tile_rounds,
+	uint32_t *bucket_eighths);
+
 /* Autotuner: runs trial_ticks iterations of the batch under each candidate
  * sort (bucket widths 2^8..2^11, radix shapes; the iterations count as
  * normal progress), keeps the fastest and reports it (bucket_shift 0 = radix
  * sort) with its seconds per tick. When the fastest is what the automatic
  * mode (pp_batch_set_sort 1) picks for the current replicas, the batch stays
  * in the automatic mode; otherwise the winner is pinned as if set with
- * pp_batch_set_sort / pp_batch_set_shape. A trial during which replicas
- * finish is timed again. */
+ * pp_batch_set_sort / pp_batch_set_shape. Then, under that sort, each
+ * launch shape (tile_rounds 1, 2, 4, 8; bucket_eighths 4, 8) is timed and
+ * the fastest kept (pp_batch_get_launch reports it). A trial during which
+ * replicas finish is timed again. */
 int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks, uint32_t *bucket_shift,
 	uint32_t *keys_per_thread, uint32_t *max_digit_bits, double *sec_per_tick);
 

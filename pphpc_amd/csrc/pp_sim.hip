@@ -95,6 +95,13 @@ static_assert(A_IPT == 2 || A_IPT == 4, "the emit loads a thread's pairs as one 
 constexpr int A_STAGE = 2 * A_TILE; /* staged outputs per action tile */
 constexpr int MIX_SLOTS = 16;       /* replicas per action tile summed in LDS */
 constexpr uint32_t K_REG = 5;       /* agents of a cell acted on in registers */
+/* Launch shape defaults (pp_batch_set_launch; the tuner tries the others):
+ * the action and emit kernels take four times the groups the GPU holds at
+ * once, so that groups that finish early are replaced, which evens out tiles
+ * of uneven work (measured at 12800 x 12800: 7% faster than one round); the
+ * bucket sort takes one round of resident groups (eighths of a round). */
+constexpr uint32_t TILE_ROUNDS = 4;
+constexpr uint32_t BUCKET_EIGHTHS = 8;
 constexpr int SCAN_WG = 1024;       /* threads of the one-group scans */
 constexpr int SCAN_UNROLL = 8;
 /* Grass regrow ticks are stored modulo 2^(8 sizeof(GT)); alive cells are
@@ -2848,6 +2855,9 @@ struct pp_batch {
 	SortShape shape{16, 11};
 	int rbits = 11;
 	uint32_t grid_up = 0, grid_down = 0, grid_act = 0, grid_emit = 0;
+	/* launch shape: rounds of resident groups of the action and emit grids,
+	 * eighths of a round of the bucket sort's */
+	uint32_t tile_rounds = TILE_ROUNDS, bucket_eighths = BUCKET_EIGHTHS;
 
 	uint32_t *cell = nullptr;
 	int32_t *energy = nullptr;
@@ -3099,18 +3109,13 @@ static uint32_t resident(const pp_batch_t *b, const void *f) {
 	return (uint32_t)per_cu * (uint32_t)b->num_cu;
 }
 
-/* The action and emit kernels take four times the groups the GPU holds at
- * once: groups that finish early are replaced, which evens out tiles of
- * uneven work (measured at 12800 x 12800: 7% faster than one round). The
- * bucket sort takes one round of resident groups. */
-constexpr uint32_t TILE_ROUNDS = 4;
-
 template <typename GT>
 static void set_grids_gt(pp_batch_t *b) {
 	const Params &P = b->P;
 	b->grid_act = min_u32(ceil_div(P.cap, A_TILE),
-		TILE_ROUNDS * resident(b, (const void *)k_act<GT>));
-	b->grid_bucket = min_u32(P.nb_cap, resident(b, (const void *)k_bucket_sort<GT>));
+		b->tile_rounds * resident(b, (const void *)k_act<GT>));
+	b->grid_bucket = min_u32(P.nb_cap, std::max<uint32_t>(1u,
+		b->bucket_eighths * resident(b, (const void *)k_bucket_sort<GT>) / 8));
 }
 
 /* Sets the sort shape and the launch grids that depend on it. */
@@ -3130,10 +3135,19 @@ static void apply_shape(pp_batch_t *b, const SortShape &sh) {
 	 * digits: one) */
 	b->grid_down = min_u32(P.hist_cols, (b->rbits == 11 ? 1 : 3) * cu);
 	b->grid_emit = min_u32(ceil_div(P.cap, A_TILE),
-		TILE_ROUNDS * resident(b, (const void *)k_emit_move));
+		b->tile_rounds * resident(b, (const void *)k_emit_move));
 	if (b->grass_bytes == 1) set_grids_gt<uint8_t>(b);
 	else if (b->grass_bytes == 2) set_grids_gt<uint16_t>(b);
 	else set_grids_gt<uint32_t>(b);
+}
+
+/* Sets the launch shape (grids of the action, emit and bucket-sort kernels). */
+static void apply_launch(pp_batch_t *b, uint32_t tile_rounds,
+	uint32_t bucket_eighths) {
+	drop_graph(b);
+	b->tile_rounds = tile_rounds;
+	b->bucket_eighths = bucket_eighths;
+	apply_shape(b, b->shape);
 }
 
 /* Capture `steps` ticks into one executable graph (kernels only; no memset
@@ -4429,6 +4443,29 @@ int pp_batch_set_shape(pp_batch_t *b, uint32_t keys_per_thread,
 	});
 }
 
+int pp_batch_set_launch(pp_batch_t *b, uint32_t tile_rounds,
+	uint32_t bucket_eighths) {
+	return guarded([&]() -> int {
+		if (!b) return PP_INVALID_ARGS;
+		if (tile_rounds != 1 && tile_rounds != 2 && tile_rounds != 4 &&
+			tile_rounds != 8)
+			return PP_INVALID_ARGS;
+		if (bucket_eighths < 1 || bucket_eighths > 8) return PP_INVALID_ARGS;
+		HIP_OK(hipSetDevice(b->device));
+		if (tile_rounds != b->tile_rounds || bucket_eighths != b->bucket_eighths)
+			apply_launch(b, tile_rounds, bucket_eighths);
+		return PP_SUCCESS;
+	});
+}
+
+int pp_batch_get_launch(const pp_batch_t *b, uint32_t *tile_rounds,
+	uint32_t *bucket_eighths) {
+	if (!b) return PP_INVALID_ARGS;
+	if (tile_rounds) *tile_rounds = b->tile_rounds;
+	if (bucket_eighths) *bucket_eighths = b->bucket_eighths;
+	return PP_SUCCESS;
+}
+
 int pp_batch_set_sort(pp_batch_t *b, uint32_t mode) {
 	return guarded([&]() -> int {
 		if (!b) return PP_INVALID_ARGS;
@@ -4458,15 +4495,11 @@ int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks, uint32_t *bucket_shift,
 		const SortShape shape0 = b->shape;
 		Cand best{b->P.bucket_on ? b->P.bshift : 0u, b->shape};
 		double best_t = -1.0;
-		for (const Cand &c : cands) {
-			if (active_count(b) == 0) break;
-			PP_TRY(pp_batch_set_sort(b, c.shift));
-			PP_TRY(pp_batch_set_shape(b, (uint32_t)c.sh.kpt, (uint32_t)c.sh.max_bits));
-			if (c.shift && !b->P.bucket_on) continue;   /* layout too fine */
-			double t = 0;
-			/* 8 untimed ticks re-sort if needed, build the graph, warm
-			 * caches; a trial during which replicas finished (the table
-			 * changed) is settled and timed once more */
+		/* One trial of the current shapes: 8 untimed ticks re-sort if
+		 * needed, build the graph, warm caches; a trial during which
+		 * replicas finished (the table changed) is settled and timed once
+		 * more. */
+		auto trial = [&](double &t) -> int {
 			for (int attempt = 0; attempt < 2; attempt++) {
 				PP_TRY(enqueue_ticks(b, 8));
 				PP_TRY(collect(b));
@@ -4474,6 +4507,15 @@ int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks, uint32_t *bucket_shift,
 				PP_TRY(time_ticks(b, trial_ticks, &t));
 				if (b->table_version == v0 || active_count(b) == 0) break;
 			}
+			return PP_SUCCESS;
+		};
+		for (const Cand &c : cands) {
+			if (active_count(b) == 0) break;
+			PP_TRY(pp_batch_set_sort(b, c.shift));
+			PP_TRY(pp_batch_set_shape(b, (uint32_t)c.sh.kpt, (uint32_t)c.sh.max_bits));
+			if (c.shift && !b->P.bucket_on) continue;   /* layout too fine */
+			double t = 0;
+			PP_TRY(trial(t));
 			if (best_t < 0 || t < best_t) {
 				best_t = t;
 				best = c;
@@ -4498,6 +4540,28 @@ int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks, uint32_t *bucket_shift,
 				PP_TRY(pp_batch_set_shape(b, (uint32_t)best.sh.kpt,
 					(uint32_t)best.sh.max_bits));
 			}
+		}
+		/* Then the launch shape under the chosen sort: rounds of resident
+		 * groups of the action and emit grids, and the bucket sort's half
+		 * or whole round. */
+		if (best_t >= 0) {
+			uint32_t best_r = b->tile_rounds, best_e = b->bucket_eighths;
+			double best_l = -1.0;
+			for (uint32_t r : {1u, 2u, 4u, 8u})
+				for (uint32_t e8 : {4u, 8u}) {
+					if (active_count(b) == 0) break;
+					if (!b->P.bucket_on && e8 != BUCKET_EIGHTHS) continue;
+					apply_launch(b, r, e8);
+					double t = 0;
+					PP_TRY(trial(t));
+					if (best_l < 0 || t < best_l) {
+						best_l = t;
+						best_r = r;
+						best_e = e8;
+					}
+				}
+			apply_launch(b, best_r, best_e);
+			if (best_l >= 0) best_t = best_l;
 		}
 		if (bucket_shift) *bucket_shift = best.shift;
 		if (keys_per_thread) *keys_per_thread = (uint32_t)best.sh.kpt;

@@ -151,6 +151,8 @@ def _lib():
         "pp_batch_tune": (i32, [vp, u32, P(u32), P(u32), P(u32),
                                 P(ctypes.c_double)]),
         "pp_batch_set_sort": (i32, [vp, u32]),
+        "pp_batch_set_launch": (i32, [vp, u32, u32]),
+        "pp_batch_get_launch": (i32, [vp, P(u32), P(u32)]),
         "pp_batch_set_profiling": (i32, [vp, i32]),
         "pp_batch_prof_get": (i32, [vp, i32, P(ctypes.c_char_p), P(u64),
                                     P(ctypes.c_double)]),
@@ -522,8 +524,20 @@ class Batch:
         self._check(_lib().pp_batch_tune(self._h, trial_ticks, ctypes.byref(s),
                                          ctypes.byref(k), ctypes.byref(b),
                                          ctypes.byref(t)), "tune")
-        return {"bucket_shift": s.value, "keys_per_thread": k.value,
-                "max_digit_bits": b.value, "sec_per_tick": t.value}
+        out = {"bucket_shift": s.value, "keys_per_thread": k.value,
+               "max_digit_bits": b.value, "sec_per_tick": t.value}
+        out.update(self.launch())
+        return out
+
+    def set_launch(self, tile_rounds: int, bucket_eighths: int):
+        self._check(_lib().pp_batch_set_launch(self._h, tile_rounds,
+                                               bucket_eighths), "set_launch")
+
+    def launch(self) -> dict:
+        r, e = ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(_lib().pp_batch_get_launch(self._h, ctypes.byref(r),
+                                               ctypes.byref(e)), "launch")
+        return {"tile_rounds": r.value, "bucket_eighths": e.value}
 
     def set_profiling(self, enable: bool):
         self._check(_lib().pp_batch_set_profiling(self._h, int(enable)),

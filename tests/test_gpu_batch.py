@@ -166,12 +166,22 @@ def test_shapes_and_tuner_do_not_change_results():
         b.run(40)
         b.set_shape(16, 11)
         b.run(30)
+        # launch shapes: one round of action and emit groups, half a round
+        # of bucket-sort groups; then eight rounds
+        b.set_launch(1, 4)
+        b.run(30)
+        b.set_launch(8, 8)
+        b.run(30)
         t = b.tune(16)
         assert t["keys_per_thread"] in (8, 16)
         assert t["max_digit_bits"] in (8, 11)
+        assert t["tile_rounds"] in (1, 2, 4, 8)
+        assert t["bucket_eighths"] in (4, 8)
         b.run(1000)
         for rid, (d, s) in zip(ids, ds):
             _check_rows(b, rid, d, s, f"seed {s} after tuning")
+        with pytest.raises(Exception):
+            b.set_launch(3, 8)
 
 
 def test_admission_waits_for_room():
