@@ -143,14 +143,18 @@ def compulsory_bytes(name, c):
             "bucket_sort": 8 * N + 8 * L}.get(name)
 
 
-def counted_traffic(kernel, sort):
-    """HBM bytes per dispatch of a kernel at 12800x12800 from the committed
-    rocprofv3 counter runs (profiles/hbm_traffic_v2s12800.md: FETCH_SIZE and
-    WRITE_SIZE, scaled by a 1 GiB copy), or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_v2s12800.json")
+def counted_traffic(kernel, which):
+    """Bytes per dispatch of a kernel from the committed rocprofv3 counter
+    runs (profiles/hbm_traffic.json: FETCH_SIZE and WRITE_SIZE, scaled by a
+    1 GiB copy; summaries in profiles/hbm_traffic_v2s12800.md and
+    profiles/counters_v2s12800.md): which = "radix" or "auto" at
+    12800x12800, "v2s1600" at 1600x1600. None when not measured."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         d = json.load(open(path))
-        return d["per_dispatch"][sort]["k_" + kernel]
+        table = (d["per_dispatch_v2s1600"] if which == "v2s1600"
+                 else d["per_dispatch"][which])
+        return table["k_" + kernel]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -508,6 +512,11 @@ def main():
     log("profiled run for the roofline")
     roof = measure_roofline(params, capacity, min(args.roofline_steps,
                                                   args.steps), args.seed)
+    if args.workload == "v2s1600":
+        roof["traffic"] = counted_traffic(roof["kernel"], "v2s1600")
+        roof["traffic_source"] = ("profiles/counters_v2s12800.md (FETCH_SIZE "
+                                  "+ WRITE_SIZE; at this size mostly "
+                                  "Infinity Cache, not HBM)")
     log(f"dominant kernel {roof['kernel']}: {roof['achieved']} GB/s")
 
     roof_big = roof_big_b = None
@@ -524,7 +533,7 @@ def main():
         roof_big_b = measure_roofline(big, 4 * nb, args.big_roofline_steps,
                                       args.seed)
         roof_big_b["traffic"] = counted_traffic(roof_big_b["kernel"], "auto")
-        roof_big_b["traffic_source"] = "profiles/hbm_traffic_v2s12800.md"
+        roof_big_b["traffic_source"] = "profiles/counters_v2s12800.md"
         log(f"12800^2 auto (buckets): dominant kernel {roof_big_b['kernel']}: "
             f"{roof_big_b['achieved']} GB/s, "
             f"{roof_big_b['ms_per_iteration']} ms")
