@@ -1115,12 +1115,17 @@ constexpr uint32_t W_MAX = 2048; /* cells of a bucket block: 2^S, S <= 11 */
 static_assert(W_MAX <= 2 * BCAP, "a dense bucket's counters reuse the staging");
 constexpr uint32_t BG_IPT = 6;   /* source positions per thread per round (1536: the three bucket rows of 12800^2) */
 static_assert(NWAVE == 4, "the gather reads the waves' counts as one uint4");
-/* Most buckets the automatic choice sorts by buckets in a batch of several
- * replicas (above, the radix sort is faster there: each bucket costs a few
- * dependent round trips to memory, and many small replicas make many
- * sparse buckets); one replica sorts by buckets at any size (12800 x 12800:
- * 89,600 buckets, 0.67 ms against 1.34 ms for four radix passes). */
+/* The automatic choice (sort mode 1): one replica sorts by buckets at any
+ * size (12800 x 12800: 89,600 buckets, 0.41 ms against 1.16 ms for four
+ * radix passes); a batch of several sorts by buckets while it has at most
+ * BUCKET_AUTO_MAX buckets, or at least BUCKET_AUTO_DENSITY agents a bucket
+ * on average, else by the radix sort. Each bucket costs a few dependent
+ * round trips to memory whatever it holds, so sparse buckets (many small
+ * replicas, or sparse populations) favour the radix sort; dense ones do not
+ * (30 replicas of 1600 x 1600, 48,000 buckets of about 390 agents: 2.48 s
+ * against 3.47 s by radix for 4000 iterations). */
 constexpr uint64_t BUCKET_AUTO_MAX = 16384;
+constexpr uint64_t BUCKET_AUTO_DENSITY = 64;
 
 /* Clears the count tables, all levels (a tick sorted by the radix sort after
  * a change of the bucket layout: what the previous emit counted is void). */
@@ -3245,9 +3250,15 @@ static int upload_table(pp_batch_t *b) {
 	/* Row buckets when chosen, or automatically: always for one replica,
 	 * for several while few enough that their per-bucket latency stays
 	 * below the radix sort's passes. */
+	/* agents of the replicas now (their initial agents before a first
+	 * collection) */
+	uint64_t agents = 0;
+	for (const auto &kv : b->reps)
+		if (kv.second.slot >= 0) agents += kv.second.n;
 	const uint32_t on = (b->sort_mode != 0 && nbk <= b->P.nb_cap &&
 		nbk <= CNT_MAX_BUCKETS &&
-		(b->sort_mode != 1 || nrep <= 1 || nbk <= BUCKET_AUTO_MAX)) ? 1u : 0u;
+		(b->sort_mode != 1 || nrep <= 1 || nbk <= BUCKET_AUTO_MAX ||
+			agents >= BUCKET_AUTO_DENSITY * nbk)) ? 1u : 0u;
 	uint32_t top = 0;
 	while (top + 1 < (uint32_t)CNT_LEVELS && (nbk >> (6 * (top + 1))) > 0) top++;
 	const uint32_t one = nrep == 1 ? order[0] : 0u;
