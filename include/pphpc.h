@@ -306,8 +306,9 @@ int pp_batch_info(const pp_batch_t *b, uint32_t *sort_passes,
 /* Sort of the agents by cell: mode 0 = the radix sort every iteration;
  * 8..11 = from the second iteration after a change of the replicas, row
  * buckets (blocks of 2^mode cells of a grid row); 1 (the default) = row
- * buckets of 2^11 cells while the batch has at most 16384 of them, else the
- * radix sort. Results do not depend on it. */
+ * buckets of 2^11 cells for a single replica, and for a batch with at most
+ * 16384 of them or at least 64 agents per bucket (its replicas' agents as
+ * last collected), else the radix sort. Results do not depend on it. */
 int pp_batch_set_sort(pp_batch_t *b, uint32_t mode);
 
 /* Radix sort launch shape: keys per thread (8 or 16; tile = 256 x that) and

@@ -3248,10 +3248,10 @@ static int upload_table(pp_batch_t *b) {
 	HIP_OK(hipMemcpy(b->bbase_dev, bbase.data(), (nrep + 1) * 4,
 		hipMemcpyHostToDevice));
 	/* Row buckets when chosen, or automatically: always for one replica,
-	 * for several while few enough that their per-bucket latency stays
-	 * below the radix sort's passes. */
-	/* agents of the replicas now (their initial agents before a first
-	 * collection) */
+	 * for several while the buckets are few or dense enough that their
+	 * per-bucket latency stays below the radix sort's passes. The density
+	 * counts the replicas' agents as last collected (their initial agents
+	 * before a first collection). */
 	uint64_t agents = 0;
 	for (const auto &kv : b->reps)
 		if (kv.second.slot >= 0) agents += kv.second.n;
