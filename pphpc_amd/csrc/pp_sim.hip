@@ -188,6 +188,10 @@ struct Ctl {
 	/* the emit that runs with this tick (the last of a run of ticks) writes
 	 * the SoA state; only the host reads it */
 	uint32_t state_tick;
+	/* the tick the emit moves for, published by the action kernel when the
+	 * table holds one replica: the emit's work-group 0 then ends the tick
+	 * (advances Ctl::tick) while the others still run, so they read this */
+	uint32_t emit_tick;
 	/* sums over completed ticks of: agents at the start, alive after the
 	 * move, after actions, occupied cells, grass eaten, births */
 	unsigned long long agent_steps, sum_live, sum_out, sum_occ, sum_eaten,
@@ -236,6 +240,7 @@ struct Params {
 	uint32_t nb_cap;     /* bucket capacity of the count tables */
 	uint32_t cnt_top;    /* highest level of the bucket count tables in use */
 	uint32_t one_slot;   /* slot of the replica when the table holds one */
+	uint32_t solo;       /* the table holds one replica: no k_act_scan */
 	uint32_t act_l1;     /* offset of the per-64-tile sums in the action counts */
 	unsigned long long *part;  /* partial-sum slots (PART_*) */
 };
@@ -252,6 +257,15 @@ constexpr uint32_t PART_SLOTS = 64, PART_STRIDE = 16, PART_LIVE = 8;
 static_assert(PART_SLOTS == (uint32_t)WAVE, "one wave sums the slots");
 static_assert(NACC <= (int)PART_LIVE && PART_LIVE < PART_STRIDE,
 	"slot layout");
+/* The action counters of a one-replica batch go to words [0, NACC) of a
+ * slot at even ticks and [PART_ODD, PART_ODD + NACC) at odd ones: the emit
+ * reads tick t's while the next tick's first kernel clears them (fold_live),
+ * before tick t + 2's actions add to them again. */
+constexpr uint32_t PART_ODD = PART_LIVE + 1;
+static_assert(PART_ODD + NACC <= PART_STRIDE, "slot layout");
+__host__ __device__ __forceinline__ uint32_t part_acc(uint32_t tick) {
+	return (tick & 1u) ? PART_ODD : 0u;
+}
 
 /* Replica tables in device memory: constants and control words by slot, and
  * the replicas of the batch sorted by first cell (cbs ascending, order[k] =
@@ -377,15 +391,20 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 }
 
 /* Adds the survivors of the last move, which the emit's work-groups left in
- * the partial-sum slots, to live[tick & 1], and clears the slots. Called by
- * the first wave of one work-group of a tick's first kernel (no other
- * work-group writes Ctl::live then, and the kernels that read it follow). */
+ * the partial-sum slots, to live[tick & 1], and clears the slots; clears the
+ * action counters of the previous tick (read by its emit, reused by the
+ * next tick's actions). Called by the first wave of one work-group of a
+ * tick's first kernel (no other work-group writes Ctl::live then, and the
+ * kernels that read it follow). */
 __device__ __forceinline__ void fold_live(Ctl *ctl, unsigned long long *part,
 	uint32_t tick) {
 	const int lane = threadIdx.x;
 	unsigned long long &p = part[lane * PART_STRIDE + PART_LIVE];
 	const unsigned long long s = wave_sum(p);
 	p = 0;      /* issued after the load returned: s depends on it */
+	unsigned long long *q = part + lane * PART_STRIDE + part_acc(tick + 1);
+#pragma unroll
+	for (int i = 0; i < NACC; i++) q[i] = 0;
 	if (lane == 0 && s) ctl->live[tick & 1] += (uint32_t)s;
 }
 
@@ -2124,6 +2143,9 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 			ocnt[edge_last_off(P.nb_cap) + i] = 0;
 		}
 	}
+	/* one replica: the emit ends the tick (no k_act_scan) and moves for
+	 * the next; it reads that tick here */
+	if (P.solo && blockIdx.x == 0 && tid == 0) ctl->emit_tick = tick + 1;
 	const uint32_t L = H.live(tick);
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
 	unsigned long long acc[NACC] = {0, 0, 0, 0, 0, 0, 0};
@@ -2205,10 +2227,11 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 				act_cnt, P.act_l1, T, nrep, tick, L, P.R, k_first, k_last, ra, acc);
 		}
 	}
-	/* one replica: a partial-sum slot (k_act_scan adds them up) */
+	/* one replica: a partial-sum slot, words of the tick's parity (the
+	 * emit adds them up) */
 	if (acc_k != NO_REP)
 		act_wg_add(acc, S.red, nrep == 1
-			? P.part + (blockIdx.x % PART_SLOTS) * PART_STRIDE
+			? P.part + (blockIdx.x % PART_SLOTS) * PART_STRIDE + part_acc(tick)
 			: &T.rctl[T.order[acc_k]].acc[0]);
 }
 
@@ -2528,6 +2551,72 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 	return live;
 }
 
+/* The end of tick t for a batch of one replica (every plain simulation), by
+ * the first wave of the emit's work-group 0: what k_act_scan does for a
+ * batch of several, from the action counters the actions left in the
+ * partial-sum slots (words of tick t's parity). The emit's other work-
+ * groups read nothing it writes: they take the tick from Ctl::emit_tick and
+ * derive the replica's state after the tick from the same counters. The
+ * caller has checked the outputs against the agent buffers. */
+__device__ void end_tick_solo(Ctl *ctl, const Tables &T, const Params &P,
+	uint32_t tick, uint32_t L) {
+	const int lane = threadIdx.x;
+	const unsigned long long *q = P.part + lane * PART_STRIDE + part_acc(tick);
+	unsigned long long a[NACC];
+#pragma unroll
+	for (int i = 0; i < NACC; i++) a[i] = wave_sum(q[i]);
+	if (lane != 0) return;
+	RepCtl &rc = T.rctl[P.one_slot];
+	const Rep &rp = T.rep[P.one_slot];
+	const uint32_t out = (uint32_t)a[AC_OUT];
+	unsigned long long steps = 0, occ = 0, eaten = 0, births = 0;
+	rc.base = 0;
+	if (rc.status == RS_ACTIVE) {
+		const uint32_t s = tick - rp.t0;
+		if (hi32(a[AC_OCC]) || out > rp.cap) {
+			rc.status = RS_FAILED;
+			rc.error = hi32(a[AC_OCC]) ? PP_ENERGY_OVERFLOW : PP_OUT_OF_RESOURCES;
+			rc.err_step = s;
+			rc.n = 0;
+		} else {
+			const uint32_t M = rp.R + 1;
+			unsigned long long D = rc.dead_cells, S = rc.cd_sum;
+			S -= D;
+			const uint32_t expire = (tick + 1) % M;
+			D -= rp.ring[expire];
+			rp.ring[expire] = 0;
+			const unsigned long long E = a[AC_EATEN];
+			rp.ring[tick % M] += E;
+			D += E;
+			S += E * (unsigned long long)rp.R;
+			rc.dead_cells = D;
+			rc.cd_sum = S;
+			write_row(a, (unsigned long long)rp.C - D, S,
+				&rp.stats[(s + 1) % rp.stats_rows]);
+			steps = rc.n;
+			occ = lo32(a[AC_OCC]);
+			eaten = E;
+			births = hi32(a[AC_OUT]);
+			rc.agent_steps += rc.n;
+			rc.step = s + 1;
+			rc.n = out;
+			if (s + 1 >= rp.iters) {
+				rc.status = RS_DONE;
+				rc.n = 0;
+			}
+		}
+	}
+	ctl->agent_steps += steps;
+	ctl->sum_live += L;
+	ctl->sum_out += out;
+	ctl->sum_occ += occ;
+	ctl->sum_eaten += eaten;
+	ctl->sum_births += births;
+	ctl->n = out;
+	ctl->live[(tick + 1) & 1] = 0;
+	ctl->tick = tick + 1;
+}
+
 /* Emits the new state and moves it for the next iteration. Sorted position
  * p emits its agent unless eaten, then its newborn if any; the agents' batch
  * positions in the new state are these outputs in position order (tile
@@ -2552,9 +2641,12 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 
 	const CtlHead H = ctl_head(ctl);
 	const uint32_t state_tick = ctl->state_tick;
+	const uint32_t emit_tick = ctl->emit_tick;
 	if (H.error) return;
 	const int tid = threadIdx.x;
-	const uint32_t tick = H.tick;               /* tick being moved for */
+	const bool solo = P.solo != 0;
+	/* the tick being moved for (the acted tick + 1) */
+	const uint32_t tick = solo ? emit_tick : H.tick;
 	const bool keep = tick == state_tick;
 	const uint32_t L = H.live(tick - 1);
 	const uint32_t nrep = H.nrep;
@@ -2563,8 +2655,36 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	unsigned long long live = 0;
 	/* A batch of one replica (every plain simulation) needs no lookups. */
 	RM m_one{};
-	if (nrep == 1 && ntiles > blockIdx.x)
+	if (solo) {
+		/* The end of the acted tick without k_act_scan: every group sums
+		 * the replica's outputs and energy-limit count from the partial-sum
+		 * slots and derives the replica's state after the tick (the status
+		 * it reads is the one before or, once group 0 has written it, after
+		 * the tick: both give the same activity); group 0 also ends the
+		 * tick. Outputs beyond the agent buffers fail the batch. */
+		if (ntiles <= blockIdx.x && blockIdx.x != 0) return;
+		const unsigned long long *q = P.part + (tid & (WAVE - 1)) * PART_STRIDE
+			+ part_acc(tick - 1);
+		const unsigned long long a_out = wave_sum(q[AC_OUT]);
+		const unsigned long long a_occ = wave_sum(q[AC_OCC]);
+		const uint32_t out = (uint32_t)a_out;
+		if (out > P.cap) {
+			if (blockIdx.x == 0 && tid == 0)
+				set_error(ctl, PP_OUT_OF_RESOURCES, tick - 1, -1);
+			return;
+		}
 		m_one = load_rm_uniform(T, 0, tick, S, P.one_slot);
+		const Rep &rp = T.rep[P.one_slot];
+		const uint32_t cap = __builtin_amdgcn_readfirstlane(rp.cap);
+		const uint32_t t0 = __builtin_amdgcn_readfirstlane(rp.t0);
+		const uint32_t iters = __builtin_amdgcn_readfirstlane(rp.iters);
+		const bool failed = hi32(a_occ) != 0 || out > cap;
+		m_one.active = m_one.active && !failed && tick - t0 < iters;
+		m_one.base = 0;
+		if (blockIdx.x == 0 && tid < WAVE) end_tick_solo(ctl, T, P, tick - 1, L);
+	} else if (nrep == 1 && ntiles > blockIdx.x) {
+		m_one = load_rm_uniform(T, 0, tick, S, P.one_slot);
+	}
 
 	for (uint32_t i = tid; i < 2 * BWIN; i += WG) s_win[i] = 0;
 	for (uint32_t i = tid; i < 2 * L1W; i += WG) s_l1[i] = 0;
@@ -2611,15 +2731,31 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
 	const uint32_t t_beg = blockIdx.x * per;
 	const uint32_t t_end = min(ntiles, t_beg + per);
+	/* One replica: the output offset of the group's first tile from the
+	 * unscanned block sums (those before its block) and the counts of the
+	 * tiles before it in its block; each later tile's adds the outputs of
+	 * the one before. */
+	uint32_t run_off = 0;
+	if (solo && t_beg < t_end) {
+		const uint32_t nbl = t_beg >> 6;
+		uint32_t v = tid < (int)(t_beg & 63u) ? act_off[(t_beg & ~63u) + tid] : 0u;
+		for (uint32_t i = tid; i < nbl; i += WG) v += act_off[P.act_l1 + i];
+		uint32_t tot;
+		(void)wg_excl_scan<WG>(v, s_tmp, tot);
+		run_off = tot;
+	}
 	for (uint32_t tile = t_beg; tile < t_end; tile++) {
 		const uint32_t base = tile * A_TILE;
-		/* the tile's output offset: its block's scanned sum plus the counts
-		 * of the block's tiles before it (a wave-wide load and sum, issued
-		 * with the tile's pairs) */
-		const uint32_t tile_lane = tid & (WAVE - 1);
-		const uint32_t before = tile_lane < (tile & 63u)
-			? act_off[(tile & ~63u) + tile_lane] : 0u;
-		const uint32_t goff = act_off[P.act_l1 + (tile >> 6)] + wave_sum32(before);
+		/* the tile's output offset: with several replicas, its block's
+		 * scanned sum plus the counts of the block's tiles before it (a
+		 * wave-wide load and sum, issued with the tile's pairs) */
+		uint32_t goff = run_off;
+		if (!solo) {
+			const uint32_t tile_lane = tid & (WAVE - 1);
+			const uint32_t before = tile_lane < (tile & 63u)
+				? act_off[(tile & ~63u) + tile_lane] : 0u;
+			goff = act_off[P.act_l1 + (tile >> 6)] + wave_sum32(before);
+		}
 		/* the tile's first cell, requested with its pairs (uniform: a
 		 * scalar load) */
 		const uint32_t c_first = __builtin_amdgcn_readfirstlane(keys[base]);
@@ -2651,6 +2787,7 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 			nout += ((v[j] & V_KILLED) == 0) + ((v[j] & V_CHILD) != 0);
 		uint32_t total;
 		uint32_t o = wg_excl_scan<WG>(nout, s_tmp, total);
+		run_off += total;
 #pragma unroll
 		for (int j = 0; j < A_IPT; j++) {
 			const uint32_t e = v[j] & V_ENERGY;
@@ -3079,15 +3216,19 @@ static int enqueue_step(pp_batch_t *b, int radix) {
 	else if (b->grass_bytes == 2) launch_act<uint16_t>(b, sorted);
 	else launch_act<uint32_t>(b, sorted);
 	prof_end(b, pe);
-	prof_begin(b, KI_ACT_SCAN, pe);
-	/* block sums that one wave scans in at most 8 steps: one wave */
-	if (b->P.act_l1 <= 64u * 64u * 8u)
-		k_act_scan<WAVE><<<1, WAVE, 0, b->stream>>>(b->act_cnt, b->ctl,
-			b->tables(), b->P);
-	else
-		k_act_scan<SCAN_WG><<<1, SCAN_WG, 0, b->stream>>>(b->act_cnt, b->ctl,
-			b->tables(), b->P);
-	prof_end(b, pe);
+	/* the end of tick: k_act_scan for several replicas, the emit's work-
+	 * group 0 for one */
+	if (!b->P.solo) {
+		prof_begin(b, KI_ACT_SCAN, pe);
+		/* block sums that one wave scans in at most 8 steps: one wave */
+		if (b->P.act_l1 <= 64u * 64u * 8u)
+			k_act_scan<WAVE><<<1, WAVE, 0, b->stream>>>(b->act_cnt, b->ctl,
+				b->tables(), b->P);
+		else
+			k_act_scan<SCAN_WG><<<1, SCAN_WG, 0, b->stream>>>(b->act_cnt, b->ctl,
+				b->tables(), b->P);
+		prof_end(b, pe);
+	}
 	prof_begin(b, KI_EMIT_MOVE, pe);
 	k_emit_move<<<b->grid_emit, WG, 0, b->stream>>>(b->keys[sorted],
 		b->vals[sorted], b->act_cnt, b->cell, b->energy, b->type, b->keys[0],
@@ -3262,12 +3403,14 @@ static int upload_table(pp_batch_t *b) {
 	uint32_t top = 0;
 	while (top + 1 < (uint32_t)CNT_LEVELS && (nbk >> (6 * (top + 1))) > 0) top++;
 	const uint32_t one = nrep == 1 ? order[0] : 0u;
+	const uint32_t solo = nrep == 1 ? 1u : 0u;
 	if (on != b->P.bucket_on || shift != b->P.bshift || top != b->P.cnt_top ||
-		one != b->P.one_slot) {
+		one != b->P.one_slot || solo != b->P.solo) {
 		b->P.bucket_on = on;
 		b->P.bshift = shift;
 		b->P.cnt_top = top;
 		b->P.one_slot = one;
+		b->P.solo = solo;
 		drop_graph(b);     /* Params are launch arguments of the graphs */
 	}
 	const uint32_t nb32 = on ? (uint32_t)nbk : 0u;
@@ -4008,6 +4151,7 @@ static int simulate_noalloc(const pp_params_t *p, uint64_t seed,
 	b->P.bshift = shift;
 	b->P.cnt_top = top;
 	b->P.one_slot = 0;
+	b->P.solo = 1;
 	const uint32_t zero = 0, one = 1, nb32 = on ? (uint32_t)nbk : 0u;
 	const uint32_t bbase[2] = {0, (uint32_t)nbk};
 	HIP_OK(hipMemcpy(b->rep_dev, &rp, sizeof(Rep), hipMemcpyHostToDevice));

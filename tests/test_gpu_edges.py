@@ -59,7 +59,8 @@ def test_cli_writes_the_oracle_stats_file(tmp_path):
     assert open(ours, "rb").read() == open(ref, "rb").read()
     lines = open(prof).read().splitlines()
     names = [ln.split("\t")[0].strip('"') for ln in lines]
-    assert {"move0", "sort_scan", "sort_down", "act", "act_scan",
+    # one replica: the emit ends each tick (no act_scan launches)
+    assert {"move0", "sort_scan", "sort_down", "act",
             "emit_move"} <= set(names)
     # %lg keeps 6 significant digits per line
     assert abs(sum(float(ln.split("\t")[2]) for ln in lines) - 1.0) < 1e-5
@@ -179,7 +180,8 @@ def test_reinit_graph_and_profiling_modes_agree():
     assert np.array_equal(a, b) and np.array_equal(a, c) and \
         np.array_equal(a, e)
     assert prof["act"][0] == 300 and prof["emit_move"][0] == 300
-    assert prof["act_scan"][0] == 300 and prof["move0"][0] == 1
+    # a plain simulation ends each tick in the emit's work-group 0
+    assert prof.get("act_scan", (0, 0))[0] == 0 and prof["move0"][0] == 1
     # iteration 1 sorts the freshly placed agents with the radix sort
     # (40000 cells: 16 bits = 2 passes of 8) after clearing the bucket
     # counts, the 299 others by row buckets; the 200 buckets fit one scan
