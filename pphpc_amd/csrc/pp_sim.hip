@@ -9,7 +9,9 @@
  *   ---------------------------------------  -----------------------------------
  *   init_cell, init_agent (pp_gpu.cl:368)    k_init_cells, k_init_agents
  *   grass (pp_gpu.cl:448)                    lazy regrow step per cell plus an
- *                                            O(1) ring update in k_act_scan
+ *                                            O(1) ring update at the end of
+ *                                            tick (k_act_scan, or the emit's
+ *                                            group 0 for one replica)
  *   move_agent (pp_gpu.cl:744)               k_emit_move (fused with writing
  *                                            the new state), k_move0 after init
  *   cl_ops sbitonic sort (pp_gpu.c:826)      LSD radix sort of (cell, value)
@@ -21,11 +23,13 @@
  *                                            that of the key cell<<32 | id
  *   find_cell_idx (pp_gpu.cl:834)            segment heads found in k_act
  *   action_agent (pp_gpu.cl:883)             k_act: one lane per occupied cell,
- *                                            shuffled order; k_act_scan: output
- *                                            offsets; k_emit_move: compaction
- *                                            and spawning
- *   reduce_grass1/2, reduce_agent1/2         fused reductions (k_act,
- *   (pp_gpu.cl:485-736)                      k_act_scan) with exact 64-bit sums
+ *                                            shuffled order; output offsets
+ *                                            (k_act_scan, or each emit group
+ *                                            for one replica); k_emit_move:
+ *                                            compaction and spawning
+ *   reduce_grass1/2, reduce_agent1/2         fused reductions (k_act, then
+ *   (pp_gpu.cl:485-736)                      k_act_scan or end_tick_solo) with
+ *                                            exact 64-bit sums
  *   stats readback every iteration           device rings of stats rows, one
  *   (pp_gpu.c:766, :840)                     copy per chunk of iterations
  *
@@ -2261,12 +2265,12 @@ __global__ __launch_bounds__(WG) void k_act_mixed(
 	}
 }
 
-/* One work-group: exclusive scan of the output counts of the action tiles'
+/* A batch of several replicas (one replica: end_tick_solo, in the emit).
+ * One work-group: exclusive scan of the output counts of the action tiles'
  * blocks of 64 (the block output offsets; a tile's offset adds the counts of
  * the tiles before it in its block), then the end of tick t for every
- * replica of the batch
- * (in table order, so that its agents' batch positions are the prefix of the
- * outputs of the replicas before it). Grass statistics come from the
+ * replica of the batch (in table order, so that its agents' batch positions
+ * are the prefix of the outputs of the replicas before it). Grass statistics come from the
  * regrow-tick encoding without touching the grid: with D dead cells and
  * countdown sum S, every dead countdown drops by one (S -= D), the cells whose
  * regrow tick is t+1 come alive (ring slot (t+1) mod (R+1)), and the E cells
@@ -2276,28 +2280,15 @@ __global__ __launch_bounds__(WG) void k_act_mixed(
  * energy limit and capacity checked (pp_gpu.c:847-852) and its counters
  * cleared; a replica that reached its iteration count is done, one with an
  * agent above the energy limit or above its capacity failed (the energy
- * limit first, as the oracle checks), and the emit drops the agents of both. The batch fails when its outputs
- * exceed the agent buffers. */
+ * limit first, as the oracle checks), and the emit drops the agents of
+ * both. The batch fails when its outputs exceed the agent buffers. */
 
 template <int NT>
 __global__ __launch_bounds__(NT) void k_act_scan(
 	uint32_t *__restrict__ act_cnt, Ctl *ctl, Tables T, Params P) {
 	__shared__ uint32_t s_tmp[NT / WAVE];
-	/* A batch of one replica (every plain simulation; its slot is a launch
-	 * parameter): the action counters it left in the partial-sum slots and
-	 * its control words and constants are requested with the batch's
-	 * control words, its grass ring entries as soon as the tick is known,
-	 * so that its end of tick waits on no further round trip. */
-	unsigned long long pv[NACC];
-	if (threadIdx.x < WAVE) {
-		const unsigned long long *q = P.part + threadIdx.x * PART_STRIDE;
-#pragma unroll
-		for (int i = 0; i < NACC; i++) pv[i] = q[i];
-	}
-	const RepCtl rc1 = T.rctl[P.one_slot];
-	const Rep rp1 = T.rep[P.one_slot];
 	/* the one-wave form: the first 64 block sums (of those allocated),
-	 * requested with the control words too */
+	 * requested with the control words */
 	uint32_t *blk = act_cnt + P.act_l1;
 	uint32_t v_first = 0;
 	if (NT == WAVE && threadIdx.x < (P.act_l1 + 63) / 64 + 1)
@@ -2307,15 +2298,6 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 	const uint32_t tick = H.tick;
 	const uint32_t L = H.live(tick);
 	const uint32_t nt = (L + A_TILE - 1) / A_TILE;
-	uint32_t r_exp = 0, r_cur = 0;
-	unsigned long long ring_exp = 0, ring_cur = 0;
-	if (H.nrep == 1) {
-		const uint32_t M = rp1.R + 1;
-		r_exp = (tick + 1) % M;
-		r_cur = tick % M;
-		ring_exp = rp1.ring[r_exp];
-		ring_cur = rp1.ring[r_cur];
-	}
 	/* the sums of blocks of 64 tiles, scanned in place: the emit adds the
 	 * counts of the tiles before its own within the block */
 	const uint32_t nbk = (nt + 63) / 64;
@@ -2369,58 +2351,8 @@ __global__ __launch_bounds__(NT) void k_act_scan(
 	const uint32_t nrep = H.nrep;
 	/* agent-steps, occupied, eaten, births */
 	unsigned long long bs[4] = {0, 0, 0, 0};
-	if (nrep == 1) {
-		/* the general loop below for k = 0, from the requested values */
-		unsigned long long a[NACC];
-#pragma unroll
-		for (int i = 0; i < NACC; i++) a[i] = rc1.acc[i] + wave_sum(pv[i]);
-		/* cleared after the loads returned (the sums depend on them) */
-		unsigned long long *q = P.part + lane * PART_STRIDE;
-#pragma unroll
-		for (int i = 0; i < NACC; i++) q[i] = 0;
-		const uint32_t out = (uint32_t)a[AC_OUT];
-		if (lane == 0) {
-			RepCtl &rc = T.rctl[P.one_slot];
-			rc.base = 0;
-			if (rc1.status == RS_ACTIVE) {
-				const uint32_t s = tick - rp1.t0;
-				if (hi32(a[AC_OCC]) || out > rp1.cap) {
-					rc.status = RS_FAILED;
-					rc.error = hi32(a[AC_OCC]) ? PP_ENERGY_OVERFLOW : PP_OUT_OF_RESOURCES;
-					rc.err_step = s;
-					rc.n = 0;
-				} else {
-					unsigned long long D = rc1.dead_cells, S = rc1.cd_sum;
-					S -= D;
-					D -= ring_exp;
-					rp1.ring[r_exp] = 0;
-					const unsigned long long E = a[AC_EATEN];
-					rp1.ring[r_cur] = ring_cur + E;
-					D += E;
-					S += E * (unsigned long long)rp1.R;
-					rc.dead_cells = D;
-					rc.cd_sum = S;
-					write_row(a, (unsigned long long)rp1.C - D, S,
-						&rp1.stats[(s + 1) % rp1.stats_rows]);
-					bs[0] = rc1.n;
-					bs[1] = lo32(a[AC_OCC]);
-					bs[2] = E;
-					bs[3] = hi32(a[AC_OUT]);
-					rc.agent_steps = rc1.agent_steps + rc1.n;
-					rc.step = s + 1;
-					rc.n = out;
-					if (s + 1 >= rp1.iters) {
-						rc.status = RS_DONE;
-						rc.n = 0;
-					}
-				}
-			}
-#pragma unroll
-			for (int i = 0; i < NACC; i++) rc.acc[i] = 0;
-		}
-	}
 	uint32_t carry = 0;
-	for (uint32_t k0 = 0; nrep > 1 && k0 < nrep; k0 += WAVE) {
+	for (uint32_t k0 = 0; k0 < nrep; k0 += WAVE) {
 		const uint32_t k = k0 + lane;
 		uint32_t slot = 0, out = 0;
 		unsigned long long a[NACC];
@@ -2620,7 +2552,8 @@ __device__ void end_tick_solo(Ctl *ctl, const Tables &T, const Params &P,
 /* Emits the new state and moves it for the next iteration. Sorted position
  * p emits its agent unless eaten, then its newborn if any; the agents' batch
  * positions in the new state are these outputs in position order (tile
- * offset from k_act_scan plus a work-group scan), and an agent's local id is
+ * offset from k_act_scan, or from the unscanned tile counts with one replica,
+ * plus a work-group scan), and an agent's local id is
  * its position minus its replica's base. Each output is staged in LDS, then
  * written with coalesced stores to the SoA state and immediately moved for
  * tick t+1; the bucket counts of the next sort are summed on the way.
