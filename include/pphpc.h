@@ -93,6 +93,14 @@ int pp_load_params(const char *path, pp_params_t *out);
 int pp_stats_save(const char *path, const pp_stats64_t *rows, uint32_t nrows,
 	const pp_params_t *p);
 
+/* pp_stats_save byte for byte as the reference writes it (pp_common.c:216-
+ * 224): counts and energy sums as its 32-bit cl_uint fields (sums above
+ * 2^32 - 1 wrap, as there), counts printed with "%d", and each mean a
+ * single-precision quotient printed with "%f". For files compared with
+ * ones pp_gpu wrote; pp_stats_save keeps exact sums and double means. */
+int pp_stats_save_f32(const char *path, const pp_stats64_t *rows,
+	uint32_t nrows, const pp_params_t *p);
+
 /* Message for a PP_* code. */
 const char *pp_strerror(int code);
 

@@ -99,6 +99,34 @@ int pp_stats_save(const char *path, const pp_stats64_t *rows, uint32_t nrows,
 	return PP_SUCCESS;
 }
 
+int pp_stats_save_f32(const char *path, const pp_stats64_t *rows,
+	uint32_t nrows, const pp_params_t *p) {
+	FILE *fp;
+	if (!p || (nrows && !rows)) return PP_INVALID_ARGS;
+	fp = fopen(path ? path : "stats.txt", "w");
+	if (!fp) return PP_UNABLE_SAVE_STATS;
+	for (uint32_t i = 0; i < nrows; i++) {
+		const pp_stats64_t *r = rows + i;
+		/* the reference's cl_uint fields (pp_common.in.h:109-123) */
+		const uint32_t sheep = (uint32_t)r->sheep, wolves = (uint32_t)r->wolves;
+		const uint32_t grass = (uint32_t)r->grass, sheep_en = (uint32_t)r->sheep_en;
+		const uint32_t wolves_en = (uint32_t)r->wolves_en;
+		const uint32_t grass_en = (uint32_t)r->grass_en;
+		/* single-precision quotients, as pp_common.c:218-224 computes them */
+		const float sheep_mean = sheep > 0 ? (float)sheep_en / (float)sheep : 0.0f;
+		const float wolf_mean = wolves > 0 ? (float)wolves_en / (float)wolves : 0.0f;
+		const float grass_mean = (float)grass_en / (float)p->grid_xy;
+		if (fprintf(fp, "%d\t%d\t%d\t%f\t%f\t%f\n", (int)sheep, (int)wolves,
+				(int)grass, (double)sheep_mean, (double)wolf_mean,
+				(double)grass_mean) < 0) {
+			fclose(fp);
+			return PP_UNABLE_SAVE_STATS;
+		}
+	}
+	if (fclose(fp) != 0) return PP_UNABLE_SAVE_STATS;
+	return PP_SUCCESS;
+}
+
 const char *pp_strerror(int code) {
 	switch (code) {
 	case PP_SUCCESS: return "success";

@@ -41,7 +41,7 @@ static double now_s(void) {
 static void usage(FILE *fp) {
 	fprintf(fp,
 		"Usage: pp [-p FILE] [-s FILE] [-r SEED] [-m MAX_AGENTS] [-d INDEX] "
-		"[-i FILE] [-D PREFIX [-T TYPE]]\n"
+		"[-i FILE] [-D PREFIX [-T TYPE]] [-F]\n"
 		"  -p, --params FILE      parameters file (default config.txt)\n"
 		"  -s, --stats FILE       statistics output (default stats.txt)\n"
 		"  -r, --rng-seed SEED    random seed (default 0)\n"
@@ -50,7 +50,9 @@ static void usage(FILE *fp) {
 		"  -i, --prof-agg FILE    export aggregate per-kernel times\n"
 		"  -D, --dump PREFIX      dump the state of every iteration to\n"
 		"                         PREFIX_agents.txt and PREFIX_cells.txt\n"
-		"  -T, --dump-type TYPE   dump type bits (0x10: cells with agents)\n");
+		"  -T, --dump-type TYPE   dump type bits (0x10: cells with agents)\n"
+		"  -F, --f32-stats        statistics as the reference prints them\n"
+		"                         (32-bit counts, single-precision means)\n");
 }
 
 /* pp_sim_simulate one iteration at a time, dumping every state. */
@@ -85,16 +87,17 @@ int main(int argc, char **argv) {
 		{"prof-agg", required_argument, NULL, 'i'},
 		{"dump", required_argument, NULL, 'D'},
 		{"dump-type", required_argument, NULL, 'T'},
+		{"f32-stats", no_argument, NULL, 'F'},
 		{"help", no_argument, NULL, 'h'},
 		{NULL, 0, NULL, 0}};
 	const char *params_file = NULL, *stats_file = NULL, *prof_file = NULL;
 	const char *dump_prefix = NULL;
 	unsigned long dump_type = 0x01;
 	unsigned long long seed = 0, cap = 0;
-	int device = -1, opt;
+	int device = -1, opt, f32_stats = 0;
 	char *endp;
 
-	while ((opt = getopt_long(argc, argv, "p:s:r:m:d:i:D:T:h", opts, NULL)) != -1) {
+	while ((opt = getopt_long(argc, argv, "p:s:r:m:d:i:D:T:Fh", opts, NULL)) != -1) {
 		switch (opt) {
 		case 'p': params_file = optarg; break;
 		case 's': stats_file = optarg; break;
@@ -116,6 +119,7 @@ int main(int argc, char **argv) {
 			dump_type = strtoul(optarg, &endp, 0);
 			if (*endp) { fprintf(stderr, "Error: invalid -T\n"); return PP_INVALID_ARGS; }
 			break;
+		case 'F': f32_stats = 1; break;
 		case 'h': usage(stdout); return PP_SUCCESS;
 		default: usage(stderr); return PP_UNKNOWN_ARGS;
 		}
@@ -171,7 +175,8 @@ int main(int argc, char **argv) {
 		else
 			fprintf(stderr, "Error: %s\n", pp_strerror(rc));
 	} else {
-		rc = pp_stats_save(stats_file, rows, p.iters + 1, &p);
+		rc = f32_stats ? pp_stats_save_f32(stats_file, rows, p.iters + 1, &p)
+			: pp_stats_save(stats_file, rows, p.iters + 1, &p);
 		if (rc) fprintf(stderr, "Error: %s\n", pp_strerror(rc));
 		if (prof_file && !rc) {
 			rc = pp_sim_prof_export(sim, prof_file);

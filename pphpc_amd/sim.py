@@ -106,6 +106,8 @@ def _lib():
     sig = {
         "pp_load_params": (i32, [ctypes.c_char_p, P(Params)]),
         "pp_stats_save": (i32, [ctypes.c_char_p, P(Stats), u32, P(Params)]),
+        "pp_stats_save_f32": (i32, [ctypes.c_char_p, P(Stats), u32,
+                                    P(Params)]),
         "pp_strerror": (ctypes.c_char_p, [i32]),
         "pp_sim_create": (i32, [P(Params), u64, u64, i32, P(vp)]),
         "pp_sim_init": (i32, [vp, P(Stats)]),
@@ -186,12 +188,15 @@ def load_params(path: Optional[str] = None) -> Params:
     return p
 
 
-def stats_save(path: Optional[str], rows, params: Params) -> None:
-    """Write the statistics file (pp_stats_save, pp_common.c:205-240)."""
+def stats_save(path: Optional[str], rows, params: Params,
+               f32: bool = False) -> None:
+    """Write the statistics file (pp_stats_save, pp_common.c:205-240); f32:
+    as the reference prints it (32-bit counts, single-precision means)."""
     arr = (Stats * len(rows))(*rows) if not isinstance(rows, ctypes.Array) \
         else rows
-    rc = _lib().pp_stats_save(path.encode() if path else None, arr,
-                              len(arr), ctypes.byref(params))
+    fn = _lib().pp_stats_save_f32 if f32 else _lib().pp_stats_save
+    rc = fn(path.encode() if path else None, arr, len(arr),
+            ctypes.byref(params))
     if rc:
         raise PPError(rc, f"stats_save({path!r})")
 

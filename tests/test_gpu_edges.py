@@ -66,6 +66,27 @@ def test_cli_writes_the_oracle_stats_file(tmp_path):
     assert abs(sum(float(ln.split("\t")[2]) for ln in lines) - 1.0) < 1e-5
 
 
+def test_cli_f32_stats_equal_the_reference_format_of_the_oracle(tmp_path):
+    """`pp -F` writes the statistics as the reference prints them (32-bit
+    counts, single-precision means): the same bytes as pp_stats_save_f32 of
+    the oracle's rows."""
+    import pphpc_amd as pp
+    from oracle.pp_oracle import simulate
+    _case("cli config1 -F")
+    params = os.path.join(ROOT, "configs", "v1_100.txt")
+    ours, ref = str(tmp_path / "gpu.txt"), str(tmp_path / "oracle.txt")
+    r = subprocess.run([os.path.join(ROOT, "pphpc_amd", "bin", "pp"), "-p",
+                        params, "-s", ours, "-r", "0", "-d", "0", "-F"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    p = pp.load_params(params)
+    d = {f: getattr(p, f) for f in pp.sim.PARAM_FIELDS}
+    rows = simulate(d, seed=0)
+    pp.stats_save(ref, [pp.Stats(*map(int, row)) for row in rows], p,
+                  f32=True)
+    assert open(ours, "rb").read() == open(ref, "rb").read()
+
+
 @pytest.mark.parametrize("dump_type", [0x01, 0x10])
 def test_cli_dump_equals_the_oracle_dump(tmp_path, dump_type):
     """`pp -D PREFIX` dumps every state (after init and each iteration) with

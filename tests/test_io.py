@@ -134,6 +134,29 @@ def test_stats_writer_closed_form(tmp_path):
     assert open(path).read() == want
 
 
+def test_stats_writer_f32_known_answers(tmp_path):
+    """pp_stats_save_f32 prints what the reference's pp_stats_save
+    (pp_common.c:216-224) prints for the same rows: 32-bit cl_uint fields,
+    "%d" counts, single-precision quotients. Hand-worked answers: 2^24 + 1
+    becomes 2^24 as a float, so 16777217/3 is 5592405.5 (the float nearest
+    16777216/3), not 5592405.666667; 1234567/1000 is the float nearest
+    1234.567, 1234.5670166015625; 9/20 rounds back to 0.450000; sums above
+    2^32 - 1 wrap (2^32 + 6 -> 6, 2^33 -> 0) and a count of 2^31 + 5 prints
+    as -2147483643."""
+    p = pp.Params.from_values(**{f: 1 for f in pp.sim.PARAM_FIELDS
+                                 if f != "grid_xy"} | {"grid_x": 4,
+                                                       "grid_y": 5})
+    rows = [pp.Stats(1000, 3, 7, 1234567, 16777217, 9, 0),
+            pp.Stats(0, 0, 20, 0, 0, 0, 0),
+            pp.Stats(2**31 + 5, 1, 0, 2**32 + 6, 7, 2**33, 0)]
+    path = str(tmp_path / "s.txt")
+    pp.stats_save(path, rows, p, f32=True)
+    want = ("1000\t3\t7\t1234.567017\t5592405.500000\t0.450000\n"
+            "0\t0\t20\t0.000000\t0.000000\t0.000000\n"
+            "-2147483643\t1\t0\t0.000000\t7.000000\t0.000000\n")
+    assert open(path).read() == want
+
+
 def test_stats_writers_agree(tmp_path):
     """Product writer and oracle writer give identical bytes."""
     rng = np.random.default_rng(5)
