@@ -3,8 +3,9 @@
 * the energy limit: an agent above 2^24 - 1 after its action fails the
   simulation (in a batch, only its replica) with PP_ENERGY_OVERFLOW at the
   iteration the oracle fails, and every row before it is the oracle's;
-* agent capacities above 33.5 M, where the end-of-tick scan of the action
-  tiles' block sums takes its 1024-thread form;
+* agent capacities above 33.5 M: one replica (the emit sums the block sums
+  before its tiles) and a batch of two, where the end-of-tick scan of the
+  action tiles' block sums takes its 1024-thread form;
 * the full 12800 x 12800 grid (163.84 M cells, 39.3 M agents): the state
   and the statistics after two iterations.
 """
@@ -91,9 +92,10 @@ def test_energy_overflow_is_isolated_in_a_batch():
 
 @pytest.mark.parametrize("capacity", [34_000_000])
 def test_wide_tile_scan_matches_oracle(capacity):
-    """Capacities above 32768 blocks of action tiles (33.5 M agents) scan
-    the tiles' block sums with 1024 threads instead of one wave; a small
-    grid in such a batch runs the oracle's rows and state."""
+    """A small grid in buffers of more than 32768 blocks of action tiles
+    (33.5 M agents) runs the oracle's rows and state. (One replica: the
+    emit's groups sum the block sums before their tiles; the scan kernel
+    is not launched.)"""
     from pphpc_amd import Simulation
     from oracle.pp_oracle import Oracle
     d = v2s(100, iters=300)
@@ -109,6 +111,26 @@ def test_wide_tile_scan_matches_oracle(capacity):
     assert np.array_equal(rows, np.vstack([np.array(ref, np.uint64), more]))
     assert_state_equal(state, o.state(), "final state")
     o.close()
+
+
+def test_wide_tile_scan_of_a_batch_matches_oracle():
+    """Two replicas in a batch of 34 M agents (above 32768 blocks of action
+    tiles): the end-of-tick kernel scans the block sums with 1024 threads
+    instead of one wave; both replicas run the oracle's rows."""
+    from pphpc_amd import Batch
+    from pphpc_amd.workloads import reserve_for as res
+    from oracle.pp_oracle import simulate
+    reps = [(v2s(100, iters=300), 12), (v1(64, iters=300), 5)]
+    cells = sum(d["grid_x"] * d["grid_y"] for d, _ in reps)
+    _case("two replicas in a batch of 34 M agents")
+    with Batch(34_000_000, cells, 2, 10, device=0) as b:
+        ids = [b.add(gpu_params(d), s, res(d)) for d, s in reps]
+        b.run(300)
+        for rid, (d, s) in zip(ids, reps):
+            st = b.replica(rid)
+            assert st["state"] == 2 and st["steps"] == 300, st
+            assert np.array_equal(b.rows_array(rid),
+                                  simulate(d, seed=s, capacity=res(d)))
 
 
 def test_full_size_12800_equals_oracle():
