@@ -2586,7 +2586,8 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	const uint32_t ntiles = (L + A_TILE - 1) / A_TILE;
 	const uint32_t S = P.bshift;
 	unsigned long long live = 0;
-	/* A batch of one replica (every plain simulation) needs no lookups. */
+	/* A batch of one replica (every plain simulation) needs no lookups; one
+	 * of several replicas reads its replica's constants per tile. */
 	RM m_one{};
 	if (solo) {
 		/* The end of the acted tick without k_act_scan: every group sums
@@ -2615,8 +2616,6 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		m_one.active = m_one.active && !failed && tick - t0 < iters;
 		m_one.base = 0;
 		if (blockIdx.x == 0 && tid < WAVE) end_tick_solo(ctl, T, P, tick - 1, L);
-	} else if (nrep == 1 && ntiles > blockIdx.x) {
-		m_one = load_rm_uniform(T, 0, tick, S, P.one_slot);
 	}
 
 	for (uint32_t i = tid; i < 2 * BWIN; i += WG) s_win[i] = 0;
