@@ -2599,9 +2599,12 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		if (ntiles <= blockIdx.x && blockIdx.x != 0) return;
 		const unsigned long long *q = P.part + (tid & (WAVE - 1)) * PART_STRIDE
 			+ part_acc(tick - 1);
-		const unsigned long long a_out = wave_sum(q[AC_OUT]);
-		const unsigned long long a_occ = wave_sum(q[AC_OCC]);
-		const uint32_t out = (uint32_t)a_out;
+		/* the outputs (low halves: their sum stays below 2^32) and whether
+		 * any slot counted an agent above the energy limit (high halves of
+		 * the occupancy words: the occupied cells, in the low halves, stay
+		 * below 2^32 in all, so no carry reaches them) */
+		const uint32_t out = wave_sum32(lo32(q[AC_OUT]));
+		const bool over = __ballot(hi32(q[AC_OCC]) != 0) != 0;
 		if (out > P.cap) {
 			if (blockIdx.x == 0 && tid == 0)
 				set_error(ctl, PP_OUT_OF_RESOURCES, tick - 1, -1);
@@ -2612,7 +2615,7 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		const uint32_t cap = __builtin_amdgcn_readfirstlane(rp.cap);
 		const uint32_t t0 = __builtin_amdgcn_readfirstlane(rp.t0);
 		const uint32_t iters = __builtin_amdgcn_readfirstlane(rp.iters);
-		const bool failed = hi32(a_occ) != 0 || out > cap;
+		const bool failed = over || out > cap;
 		m_one.active = m_one.active && !failed && tick - t0 < iters;
 		m_one.base = 0;
 		if (blockIdx.x == 0 && tid < WAVE) end_tick_solo(ctl, T, P, tick - 1, L);
@@ -2669,12 +2672,11 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	 * the one before. */
 	uint32_t run_off = 0;
 	if (solo && t_beg < t_end) {
-		const uint32_t nbl = t_beg >> 6;
-		uint32_t v = tid < (int)(t_beg & 63u) ? act_off[(t_beg & ~63u) + tid] : 0u;
-		for (uint32_t i = tid; i < nbl; i += WG) v += act_off[P.act_l1 + i];
-		uint32_t tot;
-		(void)wg_excl_scan<WG>(v, s_tmp, tot);
-		run_off = tot;
+		/* each wave on its own (no barrier) */
+		const uint32_t nbl = t_beg >> 6, ln = (uint32_t)(tid & (WAVE - 1));
+		uint32_t v = ln < (t_beg & 63u) ? act_off[(t_beg & ~63u) + ln] : 0u;
+		for (uint32_t i = ln; i < nbl; i += WAVE) v += act_off[P.act_l1 + i];
+		run_off = wave_sum32(v);
 	}
 	for (uint32_t tile = t_beg; tile < t_end; tile++) {
 		const uint32_t base = tile * A_TILE;
