@@ -1220,7 +1220,8 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 		h1 = rows_per_bucket(gx1, S);
 		nby1 = (gy1 + h1 - 1) / h1;
 	}
-	/* Geometry of bucket b: its replica (largest k with bbase[k] <= b), grid
+	/* Geometry of bucket b: its replica (the largest k with bbase[k] <= b,
+	 * from the table after the survivor counts), grid
 	 * rows [y h, y h + h), block j (nb >= 2: h = 1), and the buckets its
 	 * agents come from: the bucket rows above and below, its own, and
 	 * (nb >= 2) those of the cells left and right of the block. */
@@ -1238,12 +1239,7 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 		uint32_t nby = nby1;
 		uint32_t bb = 0;
 		if (nrep != 1) {
-			uint32_t lo = 0, hi = nrep;
-			while (hi - lo > 1) {
-				const uint32_t mid = (lo + hi) >> 1;
-				if (T.bbase[mid] <= b) lo = mid;
-				else hi = mid;
-			}
+			const uint32_t lo = ncnt[cnt_words(P.nb_cap) + b];
 			const Rep &rp = T.rep[T.order[lo]];
 			g.gx = rp.gx;
 			g.gy = rp.gy;
@@ -2953,6 +2949,10 @@ struct pp_batch {
 	 * levels; first bucket of each replica of the table */
 	uint32_t *ocnt = nullptr, *ncnt = nullptr;
 	uint32_t *bbase_dev = nullptr;
+	/* table index of each bucket's replica: nb_cap words after the
+	 * survivor counts, in their allocation (the bucket sort reads it
+	 * through ncnt, with no argument of its own) */
+	uint32_t *brep_dev = nullptr;
 	int need_radix = 1;   /* the next tick sorts with the radix sort */
 	/* 0: radix sort every tick; 1: automatic; 8..11: row buckets of
 	 * 2^mode cells */
@@ -3336,6 +3336,13 @@ static int upload_table(pp_batch_t *b) {
 			agents >= BUCKET_AUTO_DENSITY * nbk)) ? 1u : 0u;
 	uint32_t top = 0;
 	while (top + 1 < (uint32_t)CNT_LEVELS && (nbk >> (6 * (top + 1))) > 0) top++;
+	/* the replica of each bucket, for the bucket sort of several */
+	if (on && nrep > 1) {
+		std::vector<uint32_t> brep(nbk);
+		for (uint32_t i = 0; i < nrep; i++)
+			std::fill(brep.begin() + bbase[i], brep.begin() + bbase[i + 1], i);
+		HIP_OK(hipMemcpy(b->brep_dev, brep.data(), nbk * 4, hipMemcpyHostToDevice));
+	}
 	const uint32_t one = nrep == 1 ? order[0] : 0u;
 	const uint32_t solo = nrep == 1 ? 1u : 0u;
 	if (on != b->P.bucket_on || shift != b->P.bshift || top != b->P.cnt_top ||
@@ -3537,6 +3544,7 @@ static uint64_t batch_bytes(uint64_t A, uint64_t C, uint32_t max_rep,
 	const uint64_t A4 = (A + 3) & ~3ull;
 	uint64_t t = A4 * (4 + 4 + 1) + 6 * A4 * 4;
 	t += 8ull * cnt_words(nb_cap_for(C)) + 4ull * (max_rep + 1);
+	t += 4ull * nb_cap_for(C);   /* the replica of each bucket */
 	t += C * grass_bytes_for(R);
 	t += (uint64_t)max_rep * (R + 1) * 8;
 	t += hist_words * 4 + RADIX_MAX * 4;
@@ -3589,7 +3597,8 @@ static int batch_buffers(pp_batch_t *b, Take &&take) {
 	HIP_OK(take((void **)&b->mixed_list, (size_t)act_tiles * 4));
 	const size_t nbw = cnt_words(b->P.nb_cap);
 	HIP_OK(take((void **)&b->ocnt, nbw * 4));
-	HIP_OK(take((void **)&b->ncnt, nbw * 4));
+	HIP_OK(take((void **)&b->ncnt, (nbw + b->P.nb_cap) * 4));
+	b->brep_dev = b->ncnt + nbw;
 	HIP_OK(take((void **)&b->bbase_dev, ((size_t)b->max_rep + 1) * 4));
 	HIP_OK(take((void **)&b->ctl, sizeof(Ctl)));
 	HIP_OK(take((void **)&b->P.part, PART_SLOTS * PART_STRIDE * 8));
