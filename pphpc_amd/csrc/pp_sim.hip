@@ -280,7 +280,11 @@ struct Tables {
 	const uint32_t *cbs;
 	const uint32_t *order;
 	const uint32_t *bbase;  /* first bucket of the k-th replica; [nrep] = all */
+	/* per chunk of 2^CREP_SHIFT cells: the last table entry whose first cell
+	 * is at or before the chunk's first cell (0 before the first) */
+	const uint32_t *crep;
 };
+constexpr uint32_t CREP_SHIFT = 12;
 
 /* ---------------------------------------------------------------- helpers */
 
@@ -459,13 +463,11 @@ __device__ __forceinline__ void set_error(Ctl *ctl, int32_t code,
 __device__ __forceinline__ uint32_t rep_index(const Tables &T, uint32_t nrep,
 	uint32_t c) {
 	if (nrep <= 1) return 0;
-	uint32_t lo = 0, hi = nrep;
-	while (hi - lo > 1) {
-		const uint32_t mid = (lo + hi) >> 1;
-		if (T.cbs[mid] <= c) lo = mid;
-		else hi = mid;
-	}
-	return lo;
+	/* the chunk's entry, then past the replicas that start inside the
+	 * chunk before c (none where replicas span more than a chunk) */
+	uint32_t k = T.crep[c >> CREP_SHIFT];
+	while (k + 1 < nrep && T.cbs[k + 1] <= c) k++;
+	return k;
 }
 
 /* Grass of a cell, stored as its regrow tick a (the cell is alive at tick t
@@ -2966,6 +2968,7 @@ struct pp_batch {
 	RepCtl *rctl_dev = nullptr;
 	RepCtl *rctl_host = nullptr;    /* pinned mirror */
 	uint32_t *cbs_dev = nullptr;
+	uint32_t *crep_dev = nullptr;   /* Tables::crep, ceil(C / 2^CREP_SHIFT) words */
 	uint32_t *order_dev = nullptr;
 	pp_stats64_t *stats_dev = nullptr;
 	pp_stats64_t *stats_host = nullptr; /* pinned, same layout */
@@ -2989,7 +2992,7 @@ struct pp_batch {
 	uint64_t table_version = 0;   /* replica table uploads */
 
 	Tables tables() const {
-		return Tables{rep_dev, rctl_dev, cbs_dev, order_dev, bbase_dev};
+		return Tables{rep_dev, rctl_dev, cbs_dev, order_dev, bbase_dev, crep_dev};
 	}
 };
 
@@ -3308,6 +3311,18 @@ static int upload_table(pp_batch_t *b) {
 			hipMemcpyHostToDevice));
 	}
 	const uint32_t nrep = (uint32_t)t.size();
+	/* the cell-chunk table of rep_index, for a table of several */
+	if (nrep > 1) {
+		std::vector<uint32_t> crep((size_t)(b->P.C >> CREP_SHIFT) + 1);
+		uint32_t k = 0;
+		for (size_t i = 0; i < crep.size(); i++) {
+			const uint64_t c0 = (uint64_t)i << CREP_SHIFT;
+			while (k + 1 < nrep && cbs[k + 1] <= c0) k++;
+			crep[i] = k;
+		}
+		HIP_OK(hipMemcpy(b->crep_dev, crep.data(), crep.size() * 4,
+			hipMemcpyHostToDevice));
+	}
 	/* bucket layout: rows of blocks of 2^bshift cells, replicas in table
 	 * order */
 	const uint32_t shift = b->sort_mode >= 8 ? b->sort_mode : 11u;
@@ -3522,6 +3537,7 @@ static void batch_free(pp_batch_t *b) {
 	(void)hipFree(b->rep_dev);
 	(void)hipFree(b->rctl_dev);
 	(void)hipFree(b->cbs_dev);
+	(void)hipFree(b->crep_dev);
 	(void)hipFree(b->order_dev);
 	(void)hipFree(b->stats_dev);
 	if (b->stats_host) (void)hipHostFree(b->stats_host);
@@ -3545,6 +3561,7 @@ static uint64_t batch_bytes(uint64_t A, uint64_t C, uint32_t max_rep,
 	uint64_t t = A4 * (4 + 4 + 1) + 6 * A4 * 4;
 	t += 8ull * cnt_words(nb_cap_for(C)) + 4ull * (max_rep + 1);
 	t += 4ull * nb_cap_for(C);   /* the replica of each bucket */
+	t += 4ull * ((C >> CREP_SHIFT) + 1);   /* the cell-chunk table */
 	t += C * grass_bytes_for(R);
 	t += (uint64_t)max_rep * (R + 1) * 8;
 	t += hist_words * 4 + RADIX_MAX * 4;
@@ -3605,6 +3622,7 @@ static int batch_buffers(pp_batch_t *b, Take &&take) {
 	HIP_OK(take((void **)&b->rep_dev, (size_t)b->max_rep * sizeof(Rep)));
 	HIP_OK(take((void **)&b->rctl_dev, (size_t)b->max_rep * sizeof(RepCtl)));
 	HIP_OK(take((void **)&b->cbs_dev, (size_t)b->max_rep * 4));
+	HIP_OK(take((void **)&b->crep_dev, ((size_t)(C >> CREP_SHIFT) + 1) * 4));
 	HIP_OK(take((void **)&b->order_dev, (size_t)b->max_rep * 4));
 	const size_t srows = (size_t)b->max_rep * b->stats_rows;
 	HIP_OK(take((void **)&b->stats_dev, srows * sizeof(pp_stats64_t)));
