@@ -1189,7 +1189,8 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	 * counters instead) */
 	__shared__ uint32_t s_buf[2 * BCAP];
 	__shared__ uint16_t s_ix[BCAP];        /* gather index of each sorted slot */
-	__shared__ uint32_t s_cnt2[W_MAX / 2]; /* per cell, 16 bits: count, start, end */
+	/* per cell, 16 bits: count, start, end */
+	__shared__ __align__(16) uint32_t s_cnt2[W_MAX / 2];
 	uint32_t *s_ka = s_buf, *s_va = s_buf + BCAP;
 	__shared__ uint32_t s_need[2][14];
 	__shared__ uint4 s_wc[2][BG_IPT];     /* per element row: the waves' counts */
@@ -1468,23 +1469,27 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 			}
 			__syncthreads();
 			{
-				/* thread t: words t pw .. t pw + pw - 1 */
-				const uint32_t nw = W / 2, pw = nw > WG ? nw / WG : 1u;
-				const uint32_t w0 = tid * pw;
-				uint32_t v[W_MAX / 2 / WG], sum = 0;
+				/* thread t: words 4t .. 4t + 3 (one vector; W / 2 is a
+				 * multiple of 4, so a thread has all four or none) */
+				static_assert(W_MAX / 2 == 4 * WG, "four counter words a thread");
+				const uint32_t w0 = 4u * (uint32_t)tid;
+				const bool mine = w0 < W / 2;
+				uint4 *p4 = reinterpret_cast<uint4 *>(s_cnt2 + w0);
+				const uint4 v4 = mine ? *p4 : make_uint4(0u, 0u, 0u, 0u);
+				const uint32_t v[4] = {v4.x, v4.y, v4.z, v4.w};
+				uint32_t sum = 0;
 #pragma unroll
-				for (uint32_t i = 0; i < W_MAX / 2 / WG; i++) {
-					v[i] = i < pw && w0 + i < nw ? s_cnt2[w0 + i] : 0u;
-					sum += (v[i] & 0xFFFFu) + (v[i] >> 16);
-				}
+				for (int i = 0; i < 4; i++) sum += (v[i] & 0xFFFFu) + (v[i] >> 16);
 				uint32_t tot;
 				uint32_t run = wg_excl_scan<WG>(sum, s_tmp, tot);
+				uint32_t o[4];
 #pragma unroll
-				for (uint32_t i = 0; i < W_MAX / 2 / WG; i++) {
+				for (int i = 0; i < 4; i++) {
 					const uint32_t lo = v[i] & 0xFFFFu;
-					if (i < pw && w0 + i < nw) s_cnt2[w0 + i] = run | ((run + lo) << 16);
+					o[i] = run | ((run + lo) << 16);
 					run += lo + (v[i] >> 16);
 				}
+				if (mine) *p4 = make_uint4(o[0], o[1], o[2], o[3]);
 			}
 			__syncthreads();
 			for (uint32_t i = tid; i < cnt; i += WG) {
@@ -1526,12 +1531,7 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 				const uint32_t per = W / WAVE;   /* W >= 64 */
 				uint32_t sum = 0;
 				for (uint32_t i = 0; i < per; i++) sum += s_run[lane * per + i];
-				uint32_t x = sum;
-				for (int o = 1; o < WAVE; o <<= 1) {
-					const uint32_t t = __shfl_up(x, o);
-					if (lane >= o) x += t;
-				}
-				uint32_t run = x - sum;
+				uint32_t run = wave_incl_scan32(sum) - sum;
 				for (uint32_t i = 0; i < per; i++) {
 					const uint32_t v = s_run[lane * per + i];
 					s_run[lane * per + i] = run;
