@@ -1382,7 +1382,10 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 			s_nrng = m;
 			s_nsrc = seq;
 		}
-		for (uint32_t i = tid; i < W / 2; i += WG) s_cnt2[i] = 0;
+		/* thread t clears counter words 4t .. 4t + 3 (W / 2 is a multiple
+		 * of 4), as the scan below reads them */
+		if (4u * (uint32_t)tid < W / 2)
+			*reinterpret_cast<uint4 *>(s_cnt2 + 4u * (uint32_t)tid) = make_uint4(0u, 0u, 0u, 0u);
 		__syncthreads();
 		/* gather the agents landing in [cs, ce), in range order: the ranges
 		 * as one sequence of positions, element e of a round at lane-
