@@ -1163,6 +1163,46 @@ __global__ __launch_bounds__(WG) void k_bscan_clear(uint32_t *ocnt,
 	}
 }
 
+/* Prefix tables of a large layout (at least BPRE_MIN buckets), after the old-
+ * cell table's words: opre[b], the moved agents of buckets [0, b), and npre[b],
+ * the survivors bound for buckets [0, b), for b in [0, NB]. k_bprefix writes
+ * them between the emit and the bucket sort, which then reads a range's
+ * bounds with two loads instead of summing up to 3 x 63 entries of the count
+ * levels in every bucket. */
+constexpr uint64_t BPRE_MIN = 16384;
+__host__ __device__ __forceinline__ uint32_t opre_off(uint32_t nbc) {
+	return cnt_words(nbc);
+}
+__host__ __device__ __forceinline__ uint32_t npre_off(uint32_t nbc) {
+	return cnt_words(nbc) + nbc + 1;
+}
+
+/* One wave per table (old cells, new cells) and block of 64 buckets: the sum
+ * of the buckets before the block from the count levels 1 and 2 (one load a
+ * lane, one round), then the block's own from a wave scan of level 0. */
+__global__ __launch_bounds__(WG) void k_bprefix(const uint32_t *ocnt,
+	const uint32_t *ncnt, uint32_t *pre, const Ctl *ctl, Params P) {
+	const CtlHead H = ctl_head(ctl);
+	if (H.error) return;
+	const uint32_t NB = H.nbuckets;
+	const uint32_t wv = blockIdx.x * NWAVE + threadIdx.x / WAVE;
+	const uint32_t q1 = wv >> 1, t = wv & 1u;
+	if (q1 > (NB >> 6)) return;
+	const uint32_t lane = threadIdx.x & (WAVE - 1);
+	const uint32_t *cnt = t ? ncnt : ocnt;
+	uint32_t before = 0;
+	if (P.cnt_top >= 1) {
+		const uint32_t q2 = q1 >> 6;
+		if (P.cnt_top >= 2 && lane < q2) before += cnt[cnt_off(P.nb_cap, 2) + lane];
+		if (lane < (q1 & 63u)) before += cnt[cnt_off(P.nb_cap, 1) + (q2 << 6) + lane];
+	}
+	before = wave_sum32(before);
+	const uint32_t bk = (q1 << 6) + lane;
+	const uint32_t v = bk < NB ? cnt[bk] : 0u;
+	const uint32_t excl = before + wave_incl_scan32(v) - v;
+	if (bk <= NB) pre[(t ? npre_off(P.nb_cap) : opre_off(P.nb_cap)) + bk] = excl;
+}
+
 /* Stable rank of digit d (0 .. nd; nd = no element) among the lanes of the
  * wave: peers with the same digit, and this lane's rank among them. */
 __device__ __forceinline__ void wave_match(uint32_t d, uint32_t bits,
@@ -1178,7 +1218,9 @@ __device__ __forceinline__ void wave_match(uint32_t d, uint32_t bits,
 	count = (uint32_t)__popcll(peers);
 }
 
-template <typename GT>
+/* PRE: the range bounds come from the prefix tables (k_bprefix ran before);
+ * else from the count levels. */
+template <typename GT, bool PRE>
 __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
 	const uint32_t *__restrict__ ocnt, const uint32_t *__restrict__ ncnt,
@@ -1275,7 +1317,34 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	 * overlap the previous bucket's sort. Edge counts: agents on the last
 	 * cell of need[4] (left of the block) and on the first of need[5]. */
 	uint32_t pl[2][CNT_LEVELS], pc[2], pe[2];
+	/* With the prefix tables, lane s of wave 0 loads need s's bounds. */
+	uint32_t r_lo = 0, r_hi = 0, r_e = 0;
+	auto issue_pre = [&](const Geo &g) {
+		if (w != 0) return;
+		const uint32_t s = (uint32_t)lane;
+		const uint32_t q = s == 0 ? g.need[0] : s == 1 ? g.need[1]
+			: s == 2 ? g.need[2] : s == 3 ? g.need[3] : s == 4 ? g.need[4]
+			: g.need[5];
+		if (s < 6) {
+			const uint32_t off = (s == 0 ? npre_off(P.nb_cap) : opre_off(P.nb_cap)) + q;
+			r_lo = at32(ocnt, off);
+			r_hi = at32(ocnt, off + 1u);
+			if (s >= 4)
+				r_e = at32(ocnt, (s == 4 ? edge_last_off(P.nb_cap)
+					: edge_first_off(P.nb_cap)) + q);
+		}
+	};
+	auto publish_pre = [&](uint32_t *dst) {
+		if (w != 0 || lane >= 6) return;
+		dst[2 * lane] = r_lo;
+		dst[2 * lane + 1] = r_hi;
+		if (lane >= 4) dst[8 + lane] = r_e;
+	};
 	auto issue = [&](const Geo &g) {
+		if constexpr (PRE) {
+			issue_pre(g);
+			return;
+		}
 #pragma unroll
 		for (int i = 0; i < 2; i++) {
 			const int s = w + NWAVE * i;
@@ -1301,6 +1370,10 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 		}
 	};
 	auto publish = [&](uint32_t *dst) {
+		if constexpr (PRE) {
+			publish_pre(dst);
+			return;
+		}
 #pragma unroll
 		for (int i = 0; i < 2; i++) {
 			const int s = w + NWAVE * i;
@@ -2963,6 +3036,10 @@ struct pp_batch {
 	 * 2^mode cells */
 	uint32_t sort_mode = 1;
 	uint32_t grid_bucket = 0;
+	/* the bucket sort reads prefix tables that k_bprefix (grid_bpre groups)
+	 * writes into the old-cell table's allocation */
+	int bpre_on = 0;
+	uint32_t grid_bpre = 0;
 	Ctl *ctl = nullptr;
 	Ctl *ctl_host = nullptr;        /* pinned mirror */
 	int ctl_fresh = 0;              /* mirror matches the device */
@@ -3121,9 +3198,14 @@ static void launch_act(pp_batch_t *b, int sorted) {
 
 template <typename GT>
 static void launch_bucket_sort(pp_batch_t *b) {
-	k_bucket_sort<GT><<<b->grid_bucket, WG, 0, b->stream>>>(b->keys[0],
-		b->vals[0], b->ocnt, b->ncnt, b->keys[1], b->vals[1],
-		b->ctl, b->tables(), b->P, b->act_cnt, b->act_words, (GT *)b->grass_at);
+	if (b->bpre_on)
+		k_bucket_sort<GT, true><<<b->grid_bucket, WG, 0, b->stream>>>(b->keys[0],
+			b->vals[0], b->ocnt, b->ncnt, b->keys[1], b->vals[1],
+			b->ctl, b->tables(), b->P, b->act_cnt, b->act_words, (GT *)b->grass_at);
+	else
+		k_bucket_sort<GT, false><<<b->grid_bucket, WG, 0, b->stream>>>(b->keys[0],
+			b->vals[0], b->ocnt, b->ncnt, b->keys[1], b->vals[1],
+			b->ctl, b->tables(), b->P, b->act_cnt, b->act_words, (GT *)b->grass_at);
 }
 
 /* Enqueue one tick (no synchronisation): the sort of the pairs the previous
@@ -3145,6 +3227,12 @@ static int enqueue_step(pp_batch_t *b, int radix) {
 			launch_sort_pass(b, pass);
 		sorted = pass_dst(b->P.passes - 1);
 	} else {
+		if (b->bpre_on) {
+			prof_begin(b, KI_BUCKET_SCAN, pe);
+			k_bprefix<<<b->grid_bpre, WG, 0, b->stream>>>(b->ocnt, b->ncnt, b->ocnt,
+				b->ctl, b->P);
+			prof_end(b, pe);
+		}
 		prof_begin(b, KI_BUCKET_SORT, pe);
 		if (b->grass_bytes == 1) launch_bucket_sort<uint8_t>(b);
 		else if (b->grass_bytes == 2) launch_bucket_sort<uint16_t>(b);
@@ -3201,7 +3289,7 @@ static void set_grids_gt(pp_batch_t *b) {
 	b->grid_act = min_u32(ceil_div(P.cap, A_TILE),
 		b->tile_rounds * resident(b, (const void *)k_act<GT>));
 	b->grid_bucket = min_u32(P.nb_cap, std::max<uint32_t>(1u,
-		b->bucket_eighths * resident(b, (const void *)k_bucket_sort<GT>) / 8));
+		b->bucket_eighths * resident(b, (const void *)k_bucket_sort<GT, false>) / 8));
 }
 
 /* Sets the sort shape and the launch grids that depend on it. */
@@ -3363,6 +3451,17 @@ static int upload_table(pp_batch_t *b) {
 	}
 	const uint32_t one = nrep == 1 ? order[0] : 0u;
 	const uint32_t solo = nrep == 1 ? 1u : 0u;
+	/* prefix tables for a large layout (PPHPC_BPREFIX_MIN sets the bucket
+	 * count from which they are used; results do not depend on it) */
+	uint64_t bpre_min = BPRE_MIN;
+	if (const char *e = getenv("PPHPC_BPREFIX_MIN")) bpre_min = strtoull(e, nullptr, 10);
+	const int bpre = on && nbk >= bpre_min ? 1 : 0;
+	const uint32_t gpre = bpre ? ceil_div(2 * ((nbk >> 6) + 1), NWAVE) : 0u;
+	if (bpre != b->bpre_on || gpre != b->grid_bpre) {
+		b->bpre_on = bpre;
+		b->grid_bpre = gpre;
+		drop_graph(b);
+	}
 	if (on != b->P.bucket_on || shift != b->P.bshift || top != b->P.cnt_top ||
 		one != b->P.one_slot || solo != b->P.solo) {
 		b->P.bucket_on = on;
@@ -3564,6 +3663,7 @@ static uint64_t batch_bytes(uint64_t A, uint64_t C, uint32_t max_rep,
 	uint64_t t = A4 * (4 + 4 + 1) + 6 * A4 * 4;
 	t += 8ull * cnt_words(nb_cap_for(C)) + 4ull * (max_rep + 1);
 	t += 4ull * nb_cap_for(C);   /* the replica of each bucket */
+	t += 8ull * (nb_cap_for(C) + 1ull);   /* the prefix tables */
 	t += 4ull * ((C >> CREP_SHIFT) + 1);   /* the cell-chunk table */
 	t += C * grass_bytes_for(R);
 	t += (uint64_t)max_rep * (R + 1) * 8;
@@ -3616,7 +3716,7 @@ static int batch_buffers(pp_batch_t *b, Take &&take) {
 	HIP_OK(take((void **)&b->act_cnt, (size_t)b->act_words * 4));
 	HIP_OK(take((void **)&b->mixed_list, (size_t)act_tiles * 4));
 	const size_t nbw = cnt_words(b->P.nb_cap);
-	HIP_OK(take((void **)&b->ocnt, nbw * 4));
+	HIP_OK(take((void **)&b->ocnt, (nbw + 2 * ((size_t)b->P.nb_cap + 1)) * 4));
 	HIP_OK(take((void **)&b->ncnt, (nbw + b->P.nb_cap) * 4));
 	b->brep_dev = b->ncnt + nbw;
 	HIP_OK(take((void **)&b->bbase_dev, ((size_t)b->max_rep + 1) * 4));

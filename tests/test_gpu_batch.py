@@ -72,10 +72,16 @@ HETERO = [
 ]
 
 
-def test_heterogeneous_batch_equals_alone():
+@pytest.mark.parametrize("prefix", [False, True], ids=["levels", "prefix"])
+def test_heterogeneous_batch_equals_alone(prefix, monkeypatch):
     """Replicas of different grids, parameter sets, seeds and iteration
-    counts, admitted at ticks 0, 37 and 95, each equal to its run alone."""
-    _case("heterogeneous batch of 9 replicas, staggered admission")
+    counts, admitted at ticks 0, 37 and 95, each equal to its run alone;
+    the bucket sort reading its range bounds from the count levels, and from
+    the prefix tables (used by default from 16,384 buckets, here for any
+    layout)."""
+    if prefix:
+        monkeypatch.setenv("PPHPC_BPREFIX_MIN", "0")
+    _case(f"heterogeneous batch of 9 replicas, staggered admission, prefix={prefix}")
     cells = sum(d["grid_x"] * d["grid_y"] for d, _, _ in HETERO)
     agents = sum(_reserve(d) for d, _, _ in HETERO)
     with _batch(agents, cells, 16) as b:

@@ -241,6 +241,35 @@ def test_sort_variants_agree(shift):
         sim.close()
 
 
+@pytest.mark.parametrize("shift", [8, 11])
+def test_bucket_prefix_tables_agree(shift, monkeypatch):
+    """The bucket sort reading its range bounds from the prefix tables that
+    k_bprefix writes (used by default from 16,384 buckets, here forced for
+    every layout) gives the oracle's rows and state: grids narrower and wider
+    than a block, a dense corner, a bucket above the LDS capacity and a
+    layout with more than 4096 buckets (all three count levels)."""
+    from oracle.pp_oracle import Oracle
+    monkeypatch.setenv("PPHPC_BPREFIX_MIN", "0")
+    _case(f"bucket prefix tables, shift {shift}")
+    cases = [(v2s(600), 1), (v2s(gx=3000, gy=20), 2), (v2s(gx=257, gy=1), 3),
+             (dict(v2ref(60), grid_x=60, grid_y=60, init_sheep=6000,
+                   init_wolves=300), 5),
+             (dict(v2ref(60), grid_x=2048, grid_y=4, grid_xy=8192,
+                   init_sheep=16000, init_wolves=4000), 6),
+             (v2s(gx=300, gy=2500), 7)]
+    for d, seed in cases:
+        sim = _sim(d, seed=seed)
+        sim.set_sort(shift)
+        rows = _rows(sim, 120)
+        o = Oracle(d, seed=seed, capacity=sim.capacity)
+        rc, orows = o.run(120)
+        assert rc == 0
+        assert np.array_equal(rows[1:], orows), (d["grid_x"], d["grid_y"])
+        assert_state_equal(sim.state(), o.state(), f"prefix {shift} {d['grid_x']}")
+        o.close()
+        sim.close()
+
+
 def test_step_by_step_equals_run():
     """Rows and final agent state agree whether 50 iterations run as one
     call (graphs of 8 and single ticks; only the last tick writes the
