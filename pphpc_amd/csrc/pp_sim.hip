@@ -1206,6 +1206,106 @@ __global__ __launch_bounds__(WG) void k_bprefix(const uint32_t *ocnt,
 	if (bk <= NB) pre[(t ? npre_off(P.nb_cap) : opre_off(P.nb_cap)) + bk] = excl;
 }
 
+/* Bucket records of a large layout, after the prefix tables (64-byte
+ * aligned): 16 words per bucket that k_brec writes and the bucket sort reads
+ * with one load a lane, in place of working out its geometry and ordering its
+ * source ranges itself: the cells [cs, ce), the output start and count, the
+ * gathered sequence's length and range count, and per range r its first
+ * sequence index cum[r] (cum[0] = 0; unused: all ones) and the position
+ * minus sequence index delta[r] (unused: 0). */
+enum { BR_CS = 0, BR_CE, BR_OUT0, BR_EXPECT, BR_NSRC, BR_NR, BR_CUM1,
+	BR_DELTA0 = BR_CUM1 + 4, BR_WORDS = 16 };
+static_assert(BR_DELTA0 + 5 <= BR_WORDS, "record layout");
+__host__ __device__ __forceinline__ uint32_t brec_off(uint32_t nbc) {
+	return (npre_off(nbc) + nbc + 1 + BR_WORDS - 1) & ~(uint32_t)(BR_WORDS - 1);
+}
+
+/* One thread per bucket: the bucket's replica, grid rows and block, the
+ * buckets its agents come from (the rows above and below, its own, and for
+ * rows wider than a block the cells just left and right of the block), their
+ * ranges of the moved order from the prefix tables, ordered by start without
+ * empty ones or repeats (a sorting network), written as the bucket's record.
+ * The layout rules are those of bucket_of and the emit's edge counts. */
+__global__ __launch_bounds__(WG) void k_brec(uint32_t *ocnt, const uint32_t *ncnt,
+	const Ctl *ctl, Tables T, Params P) {
+	const CtlHead H = ctl_head(ctl);
+	if (H.error) return;
+	const uint32_t NB = H.nbuckets, nrep = H.nrep, S = P.bshift;
+	const uint32_t W = 1u << S;
+	const uint32_t b = blockIdx.x * WG + threadIdx.x;
+	if (b >= NB) return;
+	uint32_t slot = P.one_slot, bb = 0;
+	if (nrep != 1) {
+		const uint32_t lo = ncnt[cnt_words(P.nb_cap) + b];
+		slot = T.order[lo];
+		bb = T.bbase[lo];
+	}
+	const Rep &rp = T.rep[slot];
+	const uint32_t gx = rp.gx, gy = rp.gy, cb = rp.cb;
+	const uint32_t nb = row_blocks(gx, S), h = rows_per_bucket(gx, S);
+	const uint32_t nby = (gy + h - 1) / h;
+	const uint32_t local = b - bb;
+	const uint32_t y = nb == 1 ? local : local / nb, j = local - y * nb;
+	const uint32_t x0 = j << S, x1 = min(x0 + W, gx);
+	const uint32_t xl = x0 ? x0 - 1 : gx - 1, xr = x1 < gx ? x1 : 0;
+	const uint32_t ym = y ? y - 1 : nby - 1, yp = y + 1 < nby ? y + 1 : 0;
+	const uint32_t need[5] = {bb + ym * nb + j, b, bb + yp * nb + j,
+		bb + y * nb + (xl >> S), bb + y * nb + (xr >> S)};
+	const uint32_t row0 = cb + y * h * gx;
+	const uint32_t *opre = ocnt + opre_off(P.nb_cap), *npre = ocnt + npre_off(P.nb_cap);
+	uint32_t rs[5], re[5];
+#pragma unroll
+	for (int a = 0; a < 5; a++) {
+		rs[a] = opre[need[a]];
+		re[a] = opre[need[a] + 1];
+	}
+	if (nb >= 2) {
+		rs[3] = re[3] - ocnt[edge_last_off(P.nb_cap) + need[3]];
+		re[4] = rs[4] + ocnt[edge_first_off(P.nb_cap) + need[4]];
+	} else {
+		rs[3] = re[3] = rs[4] = re[4] = 0xFFFFFFFFu;
+	}
+	auto ce2 = [&](int a, int c) {
+		if (rs[c] < rs[a]) {
+			const uint32_t ts = rs[a], te = re[a];
+			rs[a] = rs[c]; re[a] = re[c];
+			rs[c] = ts; re[c] = te;
+		}
+	};
+	ce2(0, 3); ce2(1, 4); ce2(0, 2); ce2(1, 3); ce2(0, 1);
+	ce2(2, 4); ce2(1, 2); ce2(3, 4); ce2(2, 3);
+	uint32_t r[BR_WORDS];
+#pragma unroll
+	for (int i = 0; i < BR_WORDS; i++) r[i] = 0;
+#pragma unroll
+	for (int i = 0; i < 4; i++) r[BR_CUM1 + i] = 0xFFFFFFFFu;
+	uint32_t m = 0, last = 0xFFFFFFFFu, seq = 0;
+#pragma unroll
+	for (int a = 0; a < 5; a++) {
+		if (rs[a] >= re[a] || rs[a] == last) continue;
+		/* (m <= a: constant indices after unrolling select, not index) */
+#pragma unroll
+		for (int c = 0; c <= a; c++) {
+			if ((uint32_t)c != m) continue;
+			if (c > 0) r[BR_CUM1 + c - 1] = seq;
+			r[BR_DELTA0 + c] = rs[a] - seq;
+		}
+		seq += re[a] - rs[a];
+		last = rs[a];
+		m++;
+	}
+	r[BR_CS] = row0 + x0;
+	r[BR_CE] = h == 1 ? row0 + x1 : cb + min(y * h + h, gy) * gx;
+	r[BR_OUT0] = npre[b];
+	r[BR_EXPECT] = npre[b + 1] - npre[b];
+	r[BR_NSRC] = seq;
+	r[BR_NR] = m;
+	uint4 *dst = reinterpret_cast<uint4 *>(ocnt + brec_off(P.nb_cap) + b * BR_WORDS);
+#pragma unroll
+	for (int i = 0; i < BR_WORDS / 4; i++)
+		dst[i] = make_uint4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
+}
+
 /* Stable rank of digit d (0 .. nd; nd = no element) among the lanes of the
  * wave: peers with the same digit, and this lane's rank among them. */
 __device__ __forceinline__ void wave_match(uint32_t d, uint32_t bits,
@@ -1237,7 +1337,7 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	/* per cell, 16 bits: count, start, end */
 	__shared__ __align__(16) uint32_t s_cnt2[W_MAX / 2];
 	uint32_t *s_ka = s_buf, *s_va = s_buf + BCAP;
-	__shared__ uint32_t s_need[2][14];
+	__shared__ uint32_t s_need[2][BR_WORDS];
 	__shared__ uint4 s_wc[2][BG_IPT];     /* per element row: the waves' counts */
 	__shared__ uint32_t s_rng[10];
 	__shared__ uint32_t s_nrng;
@@ -1320,34 +1420,16 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	 * overlap the previous bucket's sort. Edge counts: agents on the last
 	 * cell of need[4] (left of the block) and on the first of need[5]. */
 	uint32_t pl[2][CNT_LEVELS], pc[2], pe[2];
-	/* With the prefix tables, lane s of wave 0 loads need s's bounds. */
-	uint32_t r_lo = 0, r_hi = 0, r_e = 0;
-	auto issue_pre = [&](const Geo &g) {
-		if (w != 0) return;
-		const uint32_t s = (uint32_t)lane;
-		const uint32_t q = s == 0 ? g.need[0] : s == 1 ? g.need[1]
-			: s == 2 ? g.need[2] : s == 3 ? g.need[3] : s == 4 ? g.need[4]
-			: g.need[5];
-		if (s < 6) {
-			const uint32_t off = (s == 0 ? npre_off(P.nb_cap) : opre_off(P.nb_cap)) + q;
-			r_lo = at32(ocnt, off);
-			r_hi = at32(ocnt, off + 1u);
-			if (s >= 4)
-				r_e = at32(ocnt, (s == 4 ? edge_last_off(P.nb_cap)
-					: edge_first_off(P.nb_cap)) + q);
-		}
+	/* With the records, lane i of wave 0 loads word i of the bucket's. */
+	uint32_t r_rec = 0;
+	auto issue_rec = [&](uint32_t bk) {
+		if (w == 0 && lane < (int)BR_WORDS)
+			r_rec = at32(ocnt, brec_off(P.nb_cap) + bk * BR_WORDS + (uint32_t)lane);
 	};
-	auto publish_pre = [&](uint32_t *dst) {
-		if (w != 0 || lane >= 6) return;
-		dst[2 * lane] = r_lo;
-		dst[2 * lane + 1] = r_hi;
-		if (lane >= 4) dst[8 + lane] = r_e;
+	auto publish_rec = [&](uint32_t *dst) {
+		if (w == 0 && lane < (int)BR_WORDS) dst[lane] = r_rec;
 	};
 	auto issue = [&](const Geo &g) {
-		if constexpr (PRE) {
-			issue_pre(g);
-			return;
-		}
 #pragma unroll
 		for (int i = 0; i < 2; i++) {
 			const int s = w + NWAVE * i;
@@ -1374,7 +1456,7 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	};
 	auto publish = [&](uint32_t *dst) {
 		if constexpr (PRE) {
-			publish_pre(dst);
+			publish_rec(dst);
 			return;
 		}
 #pragma unroll
@@ -1403,84 +1485,121 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 	uint32_t b = blockIdx.x < gpx * NXCD ? xcd * share + blockIdx.x / NXCD : NB;
 	if (b >= b_end) b = NB;
 	if (b < NB) {
-		issue(geo(b));
+		if constexpr (PRE) issue_rec(b);
+		else issue(geo(b));
 		publish(s_need[0]);
 	}
 	__syncthreads();
 	for (uint32_t it = 0; b < NB; b += gpx, it++) {
 		const uint32_t *nd = s_need[it & 1];
-		const Geo g = geo(b);
-		const uint32_t x0 = g.j << S, x1 = min(x0 + W, g.gx);
-		const uint32_t row0 = g.cb + g.y * g.h * g.gx;
-		const uint32_t cs = row0 + x0;
-		const uint32_t ce = g.h == 1 ? row0 + x1 : g.cb + min(g.y * g.h + g.h, g.gy) * g.gx;
-		const uint32_t out0 = nd[0], expect = nd[1] - out0;
-		/* source ranges of the moved order in position order (a sorting
-		 * network on the starts), without empty ones or repeats */
-		if (tid == 0) {
-			uint32_t rs[5], re[5];
+		uint32_t cs, ce, out0, expect, nr, nsrc;
+		uint32_t cum1, cum2, cum3, cum4, d0, d1, d2, d3, d4;
+		if constexpr (PRE) {
+			/* the bucket's record (k_brec) */
+			cs = __builtin_amdgcn_readfirstlane(nd[BR_CS]);
+			ce = __builtin_amdgcn_readfirstlane(nd[BR_CE]);
+			out0 = __builtin_amdgcn_readfirstlane(nd[BR_OUT0]);
+			expect = __builtin_amdgcn_readfirstlane(nd[BR_EXPECT]);
+			nsrc = __builtin_amdgcn_readfirstlane(nd[BR_NSRC]);
+			nr = __builtin_amdgcn_readfirstlane(nd[BR_NR]);
+			cum1 = __builtin_amdgcn_readfirstlane(nd[BR_CUM1]);
+			cum2 = __builtin_amdgcn_readfirstlane(nd[BR_CUM1 + 1]);
+			cum3 = __builtin_amdgcn_readfirstlane(nd[BR_CUM1 + 2]);
+			cum4 = __builtin_amdgcn_readfirstlane(nd[BR_CUM1 + 3]);
+			d0 = __builtin_amdgcn_readfirstlane(nd[BR_DELTA0]);
+			d1 = __builtin_amdgcn_readfirstlane(nd[BR_DELTA0 + 1]);
+			d2 = __builtin_amdgcn_readfirstlane(nd[BR_DELTA0 + 2]);
+			d3 = __builtin_amdgcn_readfirstlane(nd[BR_DELTA0 + 3]);
+			d4 = __builtin_amdgcn_readfirstlane(nd[BR_DELTA0 + 4]);
+			/* the ranges' positions, for a dense bucket's path */
+			if (tid < 5 && (uint32_t)tid < nr) {
+				const uint32_t c0 = tid ? nd[BR_CUM1 + tid - 1] : 0u;
+				const uint32_t c1 = (uint32_t)tid + 1 < nr ? nd[BR_CUM1 + tid] : nsrc;
+				s_rng[2 * tid] = c0 + nd[BR_DELTA0 + tid];
+				s_rng[2 * tid + 1] = c1 + nd[BR_DELTA0 + tid];
+			}
+			/* thread t clears counter words 4t .. 4t + 3 (W / 2 is a
+			 * multiple of 4), as the scan below reads them (ordered
+			 * before the counting by the gather's barriers) */
+			if (4u * (uint32_t)tid < W / 2)
+				*reinterpret_cast<uint4 *>(s_cnt2 + 4u * (uint32_t)tid) =
+					make_uint4(0u, 0u, 0u, 0u);
+		} else {
+			const Geo g = geo(b);
+			const uint32_t x0 = g.j << S, x1 = min(x0 + W, g.gx);
+			const uint32_t row0 = g.cb + g.y * g.h * g.gx;
+			cs = row0 + x0;
+			ce = g.h == 1 ? row0 + x1 : g.cb + min(g.y * g.h + g.h, g.gy) * g.gx;
+			out0 = nd[0];
+			expect = nd[1] - out0;
+			/* source ranges of the moved order in position order (a sorting
+			 * network on the starts), without empty ones or repeats */
+			if (tid == 0) {
+				uint32_t rs[5], re[5];
 #pragma unroll
-			for (int s = 1; s < 6; s++) {
-				rs[s - 1] = nd[2 * s];
-				re[s - 1] = nd[2 * s + 1];
-			}
-			if (g.nb >= 2) {
-				rs[3] = re[3] - nd[12];
-				re[4] = rs[4] + nd[13];
-			} else {
-				rs[3] = re[3] = rs[4] = re[4] = 0xFFFFFFFFu;
-			}
-			auto ce2 = [&](int a, int c) {
-				if (rs[c] < rs[a]) {
-					const uint32_t ts = rs[a], te = re[a];
-					rs[a] = rs[c]; re[a] = re[c];
-					rs[c] = ts; re[c] = te;
+				for (int s = 1; s < 6; s++) {
+					rs[s - 1] = nd[2 * s];
+					re[s - 1] = nd[2 * s + 1];
 				}
-			};
-			ce2(0, 3); ce2(1, 4); ce2(0, 2); ce2(1, 3); ce2(0, 1);
-			ce2(2, 4); ce2(1, 2); ce2(3, 4); ce2(2, 3);
-			uint32_t m = 0, last = 0xFFFFFFFFu, seq = 0;
+				if (g.nb >= 2) {
+					rs[3] = re[3] - nd[12];
+					re[4] = rs[4] + nd[13];
+				} else {
+					rs[3] = re[3] = rs[4] = re[4] = 0xFFFFFFFFu;
+				}
+				auto ce2 = [&](int a, int c) {
+					if (rs[c] < rs[a]) {
+						const uint32_t ts = rs[a], te = re[a];
+						rs[a] = rs[c]; re[a] = re[c];
+						rs[c] = ts; re[c] = te;
+					}
+				};
+				ce2(0, 3); ce2(1, 4); ce2(0, 2); ce2(1, 3); ce2(0, 1);
+				ce2(2, 4); ce2(1, 2); ce2(3, 4); ce2(2, 3);
+				uint32_t m = 0, last = 0xFFFFFFFFu, seq = 0;
 #pragma unroll
-			for (int a = 0; a < 5; a++) {
-				if (rs[a] >= re[a] || rs[a] == last) continue;
-				s_rng[2 * m] = rs[a];
-				s_rng[2 * m + 1] = re[a];
-				s_cum[m] = seq;
-				s_delta[m] = rs[a] - seq;
-				seq += re[a] - rs[a];
-				last = rs[a];
-				m++;
+				for (int a = 0; a < 5; a++) {
+					if (rs[a] >= re[a] || rs[a] == last) continue;
+					s_rng[2 * m] = rs[a];
+					s_rng[2 * m + 1] = re[a];
+					s_cum[m] = seq;
+					s_delta[m] = rs[a] - seq;
+					seq += re[a] - rs[a];
+					last = rs[a];
+					m++;
+				}
+				for (uint32_t a = m; a < 5; a++) {
+					s_cum[a] = 0xFFFFFFFFu;
+					s_delta[a] = 0;
+				}
+				s_nrng = m;
+				s_nsrc = seq;
 			}
-			for (uint32_t a = m; a < 5; a++) {
-				s_cum[a] = 0xFFFFFFFFu;
-				s_delta[a] = 0;
-			}
-			s_nrng = m;
-			s_nsrc = seq;
+			/* thread t clears counter words 4t .. 4t + 3 (W / 2 is a
+			 * multiple of 4), as the scan below reads them */
+			if (4u * (uint32_t)tid < W / 2)
+				*reinterpret_cast<uint4 *>(s_cnt2 + 4u * (uint32_t)tid) =
+					make_uint4(0u, 0u, 0u, 0u);
+			__syncthreads();
+			nr = s_nrng;
+			nsrc = __builtin_amdgcn_readfirstlane(s_nsrc);
+			/* range r of sequence index e: the last r with cum[r] <= e (uniform
+			 * bounds, so a few compares and selects per element) */
+			cum1 = __builtin_amdgcn_readfirstlane(s_cum[1]);
+			cum2 = __builtin_amdgcn_readfirstlane(s_cum[2]);
+			cum3 = __builtin_amdgcn_readfirstlane(s_cum[3]);
+			cum4 = __builtin_amdgcn_readfirstlane(s_cum[4]);
+			d0 = __builtin_amdgcn_readfirstlane(s_delta[0]);
+			d1 = __builtin_amdgcn_readfirstlane(s_delta[1]);
+			d2 = __builtin_amdgcn_readfirstlane(s_delta[2]);
+			d3 = __builtin_amdgcn_readfirstlane(s_delta[3]);
+			d4 = __builtin_amdgcn_readfirstlane(s_delta[4]);
 		}
-		/* thread t clears counter words 4t .. 4t + 3 (W / 2 is a multiple
-		 * of 4), as the scan below reads them */
-		if (4u * (uint32_t)tid < W / 2)
-			*reinterpret_cast<uint4 *>(s_cnt2 + 4u * (uint32_t)tid) = make_uint4(0u, 0u, 0u, 0u);
-		__syncthreads();
 		/* gather the agents landing in [cs, ce), in range order: the ranges
 		 * as one sequence of positions, element e of a round at lane-
 		 * consecutive e = v0 + j WG + tid (coalesced), all loads of a round
 		 * issued together; the compaction keeps the sequence order */
 		uint32_t cnt = 0;
-		const uint32_t nr = s_nrng;
-		const uint32_t nsrc = __builtin_amdgcn_readfirstlane(s_nsrc);
-		/* range r of sequence index e: the last r with cum[r] <= e (uniform
-		 * bounds, so a few compares and selects per element) */
-		const uint32_t cum1 = __builtin_amdgcn_readfirstlane(s_cum[1]);
-		const uint32_t cum2 = __builtin_amdgcn_readfirstlane(s_cum[2]);
-		const uint32_t cum3 = __builtin_amdgcn_readfirstlane(s_cum[3]);
-		const uint32_t cum4 = __builtin_amdgcn_readfirstlane(s_cum[4]);
-		const uint32_t d0 = __builtin_amdgcn_readfirstlane(s_delta[0]);
-		const uint32_t d1 = __builtin_amdgcn_readfirstlane(s_delta[1]);
-		const uint32_t d2 = __builtin_amdgcn_readfirstlane(s_delta[2]);
-		const uint32_t d3 = __builtin_amdgcn_readfirstlane(s_delta[3]);
-		const uint32_t d4 = __builtin_amdgcn_readfirstlane(s_delta[4]);
 		const uint64_t lt_mask = (1ull << lane) - 1ull;
 		for (uint32_t v0 = 0, rnd = 0; v0 < nsrc; v0 += WG * BG_IPT, rnd++) {
 			uint32_t kk[BG_IPT], vv[BG_IPT];
@@ -1536,7 +1655,10 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 		}
 		/* the next bucket's loads, in flight during this one's sort */
 		const uint32_t bn = b + gpx < b_end ? b + gpx : NB;
-		if (bn < NB) issue(geo(bn));
+		if (bn < NB) {
+			if constexpr (PRE) issue_rec(bn);
+			else issue(geo(bn));
+		}
 		if (cnt <= BCAP) {
 			/* counting sort by cell offset in LDS: counts, exclusive scan,
 			 * scatter of the gather indices, then each cell's few slots put
@@ -3039,10 +3161,10 @@ struct pp_batch {
 	 * 2^mode cells */
 	uint32_t sort_mode = 1;
 	uint32_t grid_bucket = 0;
-	/* the bucket sort reads prefix tables that k_bprefix (grid_bpre groups)
-	 * writes into the old-cell table's allocation */
+	/* the bucket sort reads records that k_bprefix (grid_bpre groups) and
+	 * k_brec (grid_brec) write into the old-cell table's allocation */
 	int bpre_on = 0;
-	uint32_t grid_bpre = 0;
+	uint32_t grid_bpre = 0, grid_brec = 0;
 	Ctl *ctl = nullptr;
 	Ctl *ctl_host = nullptr;        /* pinned mirror */
 	int ctl_fresh = 0;              /* mirror matches the device */
@@ -3234,6 +3356,8 @@ static int enqueue_step(pp_batch_t *b, int radix) {
 			prof_begin(b, KI_BUCKET_SCAN, pe);
 			k_bprefix<<<b->grid_bpre, WG, 0, b->stream>>>(b->ocnt, b->ncnt, b->ocnt,
 				b->ctl, b->P);
+			k_brec<<<b->grid_brec, WG, 0, b->stream>>>(b->ocnt, b->ncnt, b->ctl,
+				b->tables(), b->P);
 			prof_end(b, pe);
 		}
 		prof_begin(b, KI_BUCKET_SORT, pe);
@@ -3460,9 +3584,11 @@ static int upload_table(pp_batch_t *b) {
 	if (const char *e = getenv("PPHPC_BPREFIX_MIN")) bpre_min = strtoull(e, nullptr, 10);
 	const int bpre = on && nbk >= bpre_min ? 1 : 0;
 	const uint32_t gpre = bpre ? ceil_div(2 * ((nbk >> 6) + 1), NWAVE) : 0u;
-	if (bpre != b->bpre_on || gpre != b->grid_bpre) {
+	const uint32_t grec = bpre ? ceil_div(nbk, WG) : 0u;
+	if (bpre != b->bpre_on || gpre != b->grid_bpre || grec != b->grid_brec) {
 		b->bpre_on = bpre;
 		b->grid_bpre = gpre;
+		b->grid_brec = grec;
 		drop_graph(b);
 	}
 	if (on != b->P.bucket_on || shift != b->P.bshift || top != b->P.cnt_top ||
@@ -3666,7 +3792,9 @@ static uint64_t batch_bytes(uint64_t A, uint64_t C, uint32_t max_rep,
 	uint64_t t = A4 * (4 + 4 + 1) + 6 * A4 * 4;
 	t += 8ull * cnt_words(nb_cap_for(C)) + 4ull * (max_rep + 1);
 	t += 4ull * nb_cap_for(C);   /* the replica of each bucket */
-	t += 8ull * (nb_cap_for(C) + 1ull);   /* the prefix tables */
+	/* the prefix tables and bucket records */
+	t += 4ull * (brec_off(nb_cap_for(C)) - cnt_words(nb_cap_for(C)))
+		+ 4ull * BR_WORDS * nb_cap_for(C);
 	t += 4ull * ((C >> CREP_SHIFT) + 1);   /* the cell-chunk table */
 	t += C * grass_bytes_for(R);
 	t += (uint64_t)max_rep * (R + 1) * 8;
@@ -3719,7 +3847,9 @@ static int batch_buffers(pp_batch_t *b, Take &&take) {
 	HIP_OK(take((void **)&b->act_cnt, (size_t)b->act_words * 4));
 	HIP_OK(take((void **)&b->mixed_list, (size_t)act_tiles * 4));
 	const size_t nbw = cnt_words(b->P.nb_cap);
-	HIP_OK(take((void **)&b->ocnt, (nbw + 2 * ((size_t)b->P.nb_cap + 1)) * 4));
+	/* the old-cell table, then the prefix tables and bucket records */
+	HIP_OK(take((void **)&b->ocnt,
+		((size_t)brec_off(b->P.nb_cap) + (size_t)BR_WORDS * b->P.nb_cap) * 4));
 	HIP_OK(take((void **)&b->ncnt, (nbw + b->P.nb_cap) * 4));
 	b->brep_dev = b->ncnt + nbw;
 	HIP_OK(take((void **)&b->bbase_dev, ((size_t)b->max_rep + 1) * 4));
