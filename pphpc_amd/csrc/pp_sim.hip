@@ -1163,16 +1163,17 @@ __global__ __launch_bounds__(WG) void k_bscan_clear(uint32_t *ocnt,
 	}
 }
 
-/* Prefix tables of a large layout (at least BPRE_MIN buckets: from there the
- * count tables have three levels; one replica, per iteration: 3200 x 3200,
- * 6400 buckets, 114.5 -> 113.4 us; 1600 x 1600, 1600 buckets, 46.6 -> 48.1
- * us, so smaller layouts keep the levels), after the old-
+/* Prefix tables of a large layout (at least BPRE_MIN buckets; with the
+ * bucket records below, one replica per iteration: 6400 x 6400, 25,600
+ * buckets, 286 -> 260 us; 3200 x 3200, 6400 buckets, 116.6 -> 114.7 us;
+ * 2400 x 2400, 4800 buckets, 84.5 -> 85.0 us; 1600 x 1600, 46.5 -> 49.5 us:
+ * two more launches outweigh what smaller layouts save), after the old-
  * cell table's words: opre[b], the moved agents of buckets [0, b), and npre[b],
  * the survivors bound for buckets [0, b), for b in [0, NB]. k_bprefix writes
  * them between the emit and the bucket sort, which then reads a range's
  * bounds with two loads instead of summing up to 3 x 63 entries of the count
  * levels in every bucket. */
-constexpr uint64_t BPRE_MIN = 4096;
+constexpr uint64_t BPRE_MIN = 6144;
 __host__ __device__ __forceinline__ uint32_t opre_off(uint32_t nbc) {
 	return cnt_words(nbc);
 }

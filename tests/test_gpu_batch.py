@@ -77,7 +77,7 @@ def test_heterogeneous_batch_equals_alone(prefix, monkeypatch):
     """Replicas of different grids, parameter sets, seeds and iteration
     counts, admitted at ticks 0, 37 and 95, each equal to its run alone;
     the bucket sort reading its range bounds from the count levels, and from
-    the prefix tables (used by default from 4096 buckets, here for any
+    the bucket records (used by default from 6144 buckets, here for any
     layout)."""
     if prefix:
         monkeypatch.setenv("PPHPC_BPREFIX_MIN", "0")

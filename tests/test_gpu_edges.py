@@ -243,8 +243,8 @@ def test_sort_variants_agree(shift):
 
 @pytest.mark.parametrize("shift", [8, 11])
 def test_bucket_prefix_tables_agree(shift, monkeypatch):
-    """The bucket sort reading its range bounds from the prefix tables that
-    k_bprefix writes (used by default from 4096 buckets, here forced for
+    """The bucket sort reading its bucket records, which k_bprefix and k_brec
+    write (used by default from 6144 buckets, here forced for
     every layout) gives the oracle's rows and state: grids narrower and wider
     than a block, a dense corner, a bucket above the LDS capacity and a
     layout with more than 4096 buckets (all three count levels)."""
