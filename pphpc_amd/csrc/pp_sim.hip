@@ -1141,7 +1141,7 @@ static_assert(W_MAX <= 2 * BCAP, "a dense bucket's counters reuse the staging");
 constexpr uint32_t BG_IPT = 6;   /* source positions per thread per round (1536: the three bucket rows of 12800^2) */
 static_assert(NWAVE == 4, "the gather reads the waves' counts as one uint4");
 /* The automatic choice (sort mode 1): one replica sorts by buckets at any
- * size (12800 x 12800: 89,600 buckets, 0.38 ms against 1.15 ms for four
+ * size (12800 x 12800: 89,600 buckets, 0.28 ms against 1.15 ms for four
  * radix passes); a batch of several sorts by buckets while it has at most
  * BUCKET_AUTO_MAX buckets, or at least BUCKET_AUTO_DENSITY agents a bucket
  * on average, else by the radix sort. Each bucket costs a few dependent
