@@ -47,6 +47,18 @@ def test_noalloc_equals_simulate_call_after_call():
         assert len(bad) == 0, (i, int(bad[0]), got[bad[0]], ref[bad[0]])
 
 
+def test_noalloc_at_its_limits():
+    """A run at every limit at once (262,144 cells, 262,144 agents of capacity,
+    grass restart 4096) fits the scratch and equals the plain run."""
+    from pphpc_amd import simulate_noalloc
+    d = dict(v2s(512, iters=40), grass_restart=4096)
+    _case("noalloc at its limits: 512x512, capacity 262144, grass restart 4096")
+    got = simulate_noalloc(gpu_params(d), 11, 1 << 18, device=0)
+    ref = _plain(d, 11, 1 << 18)
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref)
+
+
 def test_noalloc_rejects_sizes_beyond_its_scratch():
     from pphpc_amd import PP_INVALID_ARGS, PPError, simulate_noalloc
     _case("noalloc limits")
