@@ -2764,10 +2764,12 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	int32_t *__restrict__ oenergy, uint8_t *__restrict__ otype,
 	uint32_t *__restrict__ mkeys, uint32_t *__restrict__ mvals,
 	uint32_t *ocnt, uint32_t *ncnt, Ctl *ctl, Tables T, Params P) {
-	__shared__ uint32_t s_oc[A_STAGE];
+	/* staged outputs: a tile's, after fewer than WG left from the tile
+	 * before (one replica) */
+	__shared__ uint32_t s_oc[A_STAGE + WG];
 	__shared__ uint32_t s_l1[2 * L1W];
 	__shared__ uint32_t s_l2[2 * L2W];
-	__shared__ uint32_t s_ov[A_STAGE];
+	__shared__ uint32_t s_ov[A_STAGE + WG];
 	__shared__ uint32_t s_win[2 * BWIN];
 	__shared__ uint32_t s_tmp[NWAVE];
 	__shared__ unsigned long long s_red[NWAVE];
@@ -2878,6 +2880,11 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		for (uint32_t i = ln; i < nbl; i += WAVE) v += act_off[P.act_l1 + i];
 		run_off = wave_sum32(v);
 	}
+	/* One replica: the outputs of the group's run of tiles have consecutive
+	 * ids, so they are emitted as one stream in rounds of WG; the fewer than
+	 * WG that do not fill a round wait in front of the next tile's (carry,
+	 * ids from cbase), and only the run's last round is partial. */
+	uint32_t carry = 0, cbase = 0;
 	for (uint32_t tile = t_beg; tile < t_end; tile++) {
 		const uint32_t base = tile * A_TILE;
 		/* the tile's output offset: with several replicas, its block's
@@ -2920,7 +2927,8 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		for (int j = 0; j < A_IPT; j++)
 			nout += ((v[j] & V_KILLED) == 0) + ((v[j] & V_CHILD) != 0);
 		uint32_t total;
-		uint32_t o = wg_excl_scan<WG>(nout, s_tmp, total);
+		uint32_t o = carry + wg_excl_scan<WG>(nout, s_tmp, total);
+		if (carry == 0) cbase = goff;
 		run_off += total;
 #pragma unroll
 		for (int j = 0; j < A_IPT; j++) {
@@ -2969,15 +2977,29 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 			u0 = (w0 >> 6) > L1W / 2 ? (w0 >> 6) - L1W / 2 : 0u;
 			have_win = true;
 		}
+		const uint32_t avail = carry + total;
+		const uint32_t n_emit = solo && tile + 1 < t_end ? avail & ~(uint32_t)(WG - 1)
+			: avail;
 		if (!mixed)
-			live += emit_outputs<false>(s_oc, s_ov, s_win, s_l1, w0, u0, total, goff, ocell,
-				oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
+			live += emit_outputs<false>(s_oc, s_ov, s_win, s_l1, w0, u0, n_emit, cbase,
+				ocell, oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
 				keep);
 		else
-			live += emit_outputs<true>(s_oc, s_ov, s_win, s_l1, w0, u0, total, goff, ocell,
-				oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
+			live += emit_outputs<true>(s_oc, s_ov, s_win, s_l1, w0, u0, n_emit, cbase,
+				ocell, oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
 				keep);
 		__syncthreads();
+		/* what did not fill a round moves to the front (from at least WG
+		 * entries on: no overlap) */
+		carry = avail - n_emit;
+		cbase += n_emit;
+		if (carry && n_emit) {
+			if ((uint32_t)tid < carry) {
+				s_oc[tid] = s_oc[n_emit + tid];
+				s_ov[tid] = s_ov[n_emit + tid];
+			}
+			__syncthreads();
+		}
 	}
 	if (P.bucket_on && have_win) flush();
 	const int lane = tid & (WAVE - 1), w = tid / WAVE;
