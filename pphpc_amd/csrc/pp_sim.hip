@@ -1126,8 +1126,10 @@ __global__ __launch_bounds__(WG) void k_sort_down(
  * cells just left and right of the block in row y (rows and columns wrap on
  * the torus): five contiguous ranges of the moved order. The emit counts, per
  * bucket, the moved agents by old cell (ocnt) and the survivors by new cell
- * (ncnt); the prefix sums of both give the start positions (k_bscan_*, or
- * the emit's last work-group when the buckets fit one scan chunk).
+ * (ncnt) in three levels of 64-ary sums; the prefix sums of both give the
+ * start positions. Each bucket's group sums the levels itself, or, from
+ * BPRE_MIN buckets, k_bprefix writes the prefix sums of all buckets and
+ * k_brec a record per bucket with its ranges already ordered.
  * k_bucket_sort gives every bucket one work-group, which
  * gathers the agents of its ranges that land in it, in range order (= agent
  * id order), sorts them stably by cell in LDS (a counting sort) and writes
