@@ -190,7 +190,10 @@ int pp_dump_state(const char *agents_path, const char *cells_path, int append,
  * on the simulation's stream and no graph is used. */
 int pp_sim_set_profiling(pp_sim_t *s, int enable);
 /* Number of kernels tracked, and for kernel i its name, launch count and
- * total / average duration in nanoseconds. */
+ * total / average duration in nanoseconds. A name can cover more than one
+ * kernel: "bucket_scan" is the count tables' clear before a radix-sorted
+ * tick, or the prefix tables and bucket records (k_bprefix, k_brec) before
+ * the bucket sort of a large layout. */
 int pp_sim_prof_count(const pp_sim_t *s);
 int pp_sim_prof_get(pp_sim_t *s, int i, const char **name, uint64_t *launches,
 	double *total_ns);
