@@ -106,7 +106,13 @@ constexpr uint32_t K_REG = 5;       /* agents of a cell acted on in registers */
  * bucket sort takes one round of resident groups (eighths of a round). */
 constexpr uint32_t TILE_ROUNDS = 4;
 constexpr uint32_t BUCKET_EIGHTHS = 8;
-constexpr int SCAN_WG = 1024;       /* threads of the one-group scans */
+/* Emit groups of 512 threads (two positions each) when a tick's tiles fit in
+ * one round of such groups: each tile's output rounds, its serial part, are
+ * then half as many. One 1600 x 1600 replica: 45.1 us an iteration against
+ * 46.4; at 2000 x 2000 and above, and for several replicas, 256 threads are
+ * as fast or faster (12800 x 12800: 8%). */
+constexpr int EMIT_WIDE = 2 * WG;
+constexpr int SCAN_WG = 1024;      /* threads of the one-group scans */
 constexpr int SCAN_UNROLL = 8;
 /* Grass regrow ticks are stored modulo 2^(8 sizeof(GT)); alive cells are
  * re-stamped once every REFRESH ticks (1/REFRESH of the cells per tick), so
@@ -2635,7 +2641,7 @@ constexpr uint32_t BWIN = 128;   /* buckets per LDS count window */
  * (ncnt), through LDS windows of BWIN buckets from w0. MIXED = false: every
  * output belongs to replica m_u (its constants stay uniform across the
  * group); true: each looks up its own. Returns the survivors of the move. */
-template <bool MIXED>
+template <bool MIXED, int NT>
 __device__ __forceinline__ unsigned long long emit_outputs(
 	const uint32_t *s_oc, const uint32_t *s_ov, uint32_t *s_win,
 	uint32_t *s_l1, uint32_t w0, uint32_t u0,
@@ -2646,7 +2652,7 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 	uint32_t tick, const RM &m_u, const Params &P, bool keep) {
 	unsigned long long live = 0;
 	const uint32_t S = P.bshift;
-	for (uint32_t i0 = 0; i0 < total; i0 += WG) {
+	for (uint32_t i0 = 0; i0 < total; i0 += NT) {
 		const uint32_t i = i0 + threadIdx.x;
 		uint32_t ob = NO_DIGIT, nbk = NO_DIGIT;
 		if (i < total) {
@@ -2760,21 +2766,25 @@ __device__ void end_tick_solo(Ctl *ctl, const Tables &T, const Params &P,
  * written with coalesced stores to the SoA state and immediately moved for
  * tick t+1; the bucket counts of the next sort are summed on the way.
  * Agents of a replica that is done or failed get DEAD_KEY. */
-__global__ __launch_bounds__(WG) void k_emit_move(
+template <int NT>
+__global__ __launch_bounds__(NT) void k_emit_move(
 	const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
 	const uint32_t *__restrict__ act_off, uint32_t *__restrict__ ocell,
 	int32_t *__restrict__ oenergy, uint8_t *__restrict__ otype,
 	uint32_t *__restrict__ mkeys, uint32_t *__restrict__ mvals,
 	uint32_t *ocnt, uint32_t *ncnt, Ctl *ctl, Tables T, Params P) {
-	/* staged outputs: a tile's, after fewer than WG left from the tile
+	/* NT threads, IPT sorted positions each (the action tile) */
+	constexpr int IPT = A_TILE / NT, NW = NT / WAVE;
+	static_assert(IPT * NT == A_TILE && (IPT == 2 || IPT == 4), "emit shape");
+	/* staged outputs: a tile's, after fewer than NT left from the tile
 	 * before (one replica) */
-	__shared__ uint32_t s_oc[A_STAGE + WG];
+	__shared__ uint32_t s_oc[A_STAGE + NT];
 	__shared__ uint32_t s_l1[2 * L1W];
 	__shared__ uint32_t s_l2[2 * L2W];
-	__shared__ uint32_t s_ov[A_STAGE + WG];
+	__shared__ uint32_t s_ov[A_STAGE + NT];
 	__shared__ uint32_t s_win[2 * BWIN];
-	__shared__ uint32_t s_tmp[NWAVE];
-	__shared__ unsigned long long s_red[NWAVE];
+	__shared__ uint32_t s_tmp[NW];
+	__shared__ unsigned long long s_red[NW];
 
 	const CtlHead H = ctl_head(ctl);
 	const uint32_t state_tick = ctl->state_tick;
@@ -2825,8 +2835,8 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		if (blockIdx.x == 0 && tid < WAVE) end_tick_solo(ctl, T, P, tick - 1, L);
 	}
 
-	for (uint32_t i = tid; i < 2 * BWIN; i += WG) s_win[i] = 0;
-	for (uint32_t i = tid; i < 2 * L1W; i += WG) s_l1[i] = 0;
+	for (uint32_t i = tid; i < 2 * BWIN; i += NT) s_win[i] = 0;
+	for (uint32_t i = tid; i < 2 * L1W; i += NT) s_l1[i] = 0;
 	if (tid < 2 * L2W) s_l2[tid] = 0;
 	/* Flushes the count windows (bases w0, u0): level 0; level 1, summed
 	 * into the level-2 window; levels 2 and 3. The windows persist over a
@@ -2834,7 +2844,7 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	uint32_t w0 = 0, u0 = 0;
 	bool have_win = false;
 	auto flush = [&]() {
-		for (uint32_t i = tid; i < 2 * BWIN; i += WG) {
+		for (uint32_t i = tid; i < 2 * BWIN; i += NT) {
 			const uint32_t h = s_win[i];
 			if (h) {
 				atomicAdd(&(i < BWIN ? ocnt : ncnt)[w0 + (i % BWIN)], h);
@@ -2842,7 +2852,7 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 			}
 		}
 		if (P.cnt_top >= 1) {
-			for (uint32_t i = tid; i < 2 * L1W; i += WG) {
+			for (uint32_t i = tid; i < 2 * L1W; i += NT) {
 				const uint32_t h = s_l1[i];
 				if (h) {
 					const uint32_t k = i / L1W, q1 = u0 + (i % L1W);
@@ -2883,8 +2893,8 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		run_off = wave_sum32(v);
 	}
 	/* One replica: the outputs of the group's run of tiles have consecutive
-	 * ids, so they are emitted as one stream in rounds of WG; the fewer than
-	 * WG that do not fill a round wait in front of the next tile's (carry,
+	 * ids, so they are emitted as one stream in rounds of NT; the fewer than
+	 * NT that do not fill a round wait in front of the next tile's (carry,
 	 * ids from cbase), and only the run's last round is partial. */
 	uint32_t carry = 0, cbase = 0;
 	for (uint32_t tile = t_beg; tile < t_end; tile++) {
@@ -2902,10 +2912,10 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		/* the tile's first cell, requested with its pairs (uniform: a
 		 * scalar load) */
 		const uint32_t c_first = __builtin_amdgcn_readfirstlane(keys[base]);
-		const uint32_t p0 = base + tid * A_IPT;
-		uint32_t c[A_IPT], v[A_IPT];
-		if (p0 + A_IPT <= L) {
-			if constexpr (A_IPT == 4) {
+		const uint32_t p0 = base + tid * IPT;
+		uint32_t c[IPT], v[IPT];
+		if (p0 + IPT <= L) {
+			if constexpr (IPT == 4) {
 				const uint4 kv = at32(reinterpret_cast<const uint4 *>(keys), p0 / 4);
 				const uint4 vv = at32(reinterpret_cast<const uint4 *>(vals), p0 / 4);
 				c[0] = kv.x; c[1] = kv.y; c[2] = kv.z; c[3] = kv.w;
@@ -2918,7 +2928,7 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 			}
 		} else {
 #pragma unroll
-			for (int j = 0; j < A_IPT; j++) {
+			for (int j = 0; j < IPT; j++) {
 				const bool in = p0 + j < L;
 				c[j] = in ? keys[p0 + j] : 0u;
 				v[j] = in ? vals[p0 + j] : V_KILLED;
@@ -2926,14 +2936,14 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 		}
 		uint32_t nout = 0;
 #pragma unroll
-		for (int j = 0; j < A_IPT; j++)
+		for (int j = 0; j < IPT; j++)
 			nout += ((v[j] & V_KILLED) == 0) + ((v[j] & V_CHILD) != 0);
 		uint32_t total;
-		uint32_t o = carry + wg_excl_scan<WG>(nout, s_tmp, total);
+		uint32_t o = carry + wg_excl_scan<NT>(nout, s_tmp, total);
 		if (carry == 0) cbase = goff;
 		run_off += total;
 #pragma unroll
-		for (int j = 0; j < A_IPT; j++) {
+		for (int j = 0; j < IPT; j++) {
 			const uint32_t e = v[j] & V_ENERGY;
 			const uint32_t t = (v[j] & V_WOLF) ? 1u : 0u;
 			if (!(v[j] & V_KILLED)) {
@@ -2980,18 +2990,18 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 			have_win = true;
 		}
 		const uint32_t avail = carry + total;
-		const uint32_t n_emit = solo && tile + 1 < t_end ? avail & ~(uint32_t)(WG - 1)
+		const uint32_t n_emit = solo && tile + 1 < t_end ? avail & ~(uint32_t)(NT - 1)
 			: avail;
 		if (!mixed)
-			live += emit_outputs<false>(s_oc, s_ov, s_win, s_l1, w0, u0, n_emit, cbase,
+			live += emit_outputs<false, NT>(s_oc, s_ov, s_win, s_l1, w0, u0, n_emit, cbase,
 				ocell, oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
 				keep);
 		else
-			live += emit_outputs<true>(s_oc, s_ov, s_win, s_l1, w0, u0, n_emit, cbase,
+			live += emit_outputs<true, NT>(s_oc, s_ov, s_win, s_l1, w0, u0, n_emit, cbase,
 				ocell, oenergy, otype, mkeys, mvals, ocnt, ncnt, T, nrep, tick, m_t, P,
 				keep);
 		__syncthreads();
-		/* what did not fill a round moves to the front (from at least WG
+		/* what did not fill a round moves to the front (from at least NT
 		 * entries on: no overlap) */
 		carry = avail - n_emit;
 		cbase += n_emit;
@@ -3012,7 +3022,7 @@ __global__ __launch_bounds__(WG) void k_emit_move(
 	/* the next tick's first kernel adds up the slots into live[tick & 1] */
 	if (tid == 0) {
 		unsigned long long sum = 0;
-		for (int i = 0; i < NWAVE; i++) sum += s_red[i];
+		for (int i = 0; i < NW; i++) sum += s_red[i];
 		if (sum)
 			atomicAdd(&P.part[(blockIdx.x % PART_SLOTS) * PART_STRIDE + PART_LIVE],
 				sum);
@@ -3158,6 +3168,9 @@ struct pp_batch {
 	/* launch shape: rounds of resident groups of the action and emit grids,
 	 * eighths of a round of the bucket sort's */
 	uint32_t tile_rounds = TILE_ROUNDS, bucket_eighths = BUCKET_EIGHTHS;
+	/* emit groups of EMIT_WIDE threads instead of WG (chosen from the live
+	 * agents at each replica set) */
+	int emit_wide = 0;
 
 	uint32_t *cell = nullptr;
 	int32_t *energy = nullptr;
@@ -3410,9 +3423,15 @@ static int enqueue_step(pp_batch_t *b, int radix) {
 		prof_end(b, pe);
 	}
 	prof_begin(b, KI_EMIT_MOVE, pe);
-	k_emit_move<<<b->grid_emit, WG, 0, b->stream>>>(b->keys[sorted],
-		b->vals[sorted], b->act_cnt, b->cell, b->energy, b->type, b->keys[0],
-		b->vals[0], b->ocnt, b->ncnt, b->ctl, b->tables(), b->P);
+	if (b->emit_wide)
+		k_emit_move<EMIT_WIDE><<<b->grid_emit, EMIT_WIDE, 0, b->stream>>>(
+			b->keys[sorted], b->vals[sorted], b->act_cnt, b->cell, b->energy,
+			b->type, b->keys[0], b->vals[0], b->ocnt, b->ncnt, b->ctl,
+			b->tables(), b->P);
+	else
+		k_emit_move<WG><<<b->grid_emit, WG, 0, b->stream>>>(b->keys[sorted],
+			b->vals[sorted], b->act_cnt, b->cell, b->energy, b->type, b->keys[0],
+			b->vals[0], b->ocnt, b->ncnt, b->ctl, b->tables(), b->P);
 	prof_end(b, pe);
 	HIP_OK(hipGetLastError());
 	return PP_SUCCESS;
@@ -3427,9 +3446,9 @@ static void drop_graph(pp_batch_t *b) {
 }
 
 /* Work-groups of kernel f (WG threads) that the GPU holds at once. */
-static uint32_t resident(const pp_batch_t *b, const void *f) {
+static uint32_t resident(const pp_batch_t *b, const void *f, int nt = WG) {
 	int per_cu = 0;
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, WG, 0) !=
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, nt, 0) !=
 			hipSuccess || per_cu <= 0)
 		per_cu = 2;
 	return (uint32_t)per_cu * (uint32_t)b->num_cu;
@@ -3460,11 +3479,19 @@ static void apply_shape(pp_batch_t *b, const SortShape &sh) {
 	/* (8-bit digits: three groups a CU, as many as its LDS holds; 11-bit
 	 * digits: one) */
 	b->grid_down = min_u32(P.hist_cols, (b->rbits == 11 ? 1 : 3) * cu);
-	b->grid_emit = min_u32(ceil_div(P.cap, A_TILE),
-		b->tile_rounds * resident(b, (const void *)k_emit_move));
+	b->grid_emit = min_u32(ceil_div(P.cap, A_TILE), b->tile_rounds *
+		(b->emit_wide ? resident(b, (const void *)k_emit_move<EMIT_WIDE>, EMIT_WIDE)
+			: resident(b, (const void *)k_emit_move<WG>)));
 	if (b->grass_bytes == 1) set_grids_gt<uint8_t>(b);
 	else if (b->grass_bytes == 2) set_grids_gt<uint16_t>(b);
 	else set_grids_gt<uint32_t>(b);
+}
+
+/* Wide emit groups while the tiles of `agents` fit in one round of them. */
+static int emit_wide_auto(const pp_batch_t *b, uint64_t agents) {
+	const uint64_t tiles = (agents + A_TILE - 1) / A_TILE;
+	return tiles <= resident(b, (const void *)k_emit_move<EMIT_WIDE>, EMIT_WIDE)
+		? 1 : 0;
 }
 
 /* Sets the launch shape (grids of the action, emit and bucket-sort kernels). */
@@ -3615,6 +3642,14 @@ static int upload_table(pp_batch_t *b) {
 		b->grid_bpre = gpre;
 		b->grid_brec = grec;
 		drop_graph(b);
+	}
+	/* emit width: PPHPC_EMIT_WIDE (0 or 1), or wide while the agents'
+	 * tiles fit in one round of wide groups (results do not depend on it) */
+	int wide = emit_wide_auto(b, agents);
+	if (const char *e = getenv("PPHPC_EMIT_WIDE")) wide = atoi(e) ? 1 : 0;
+	if (wide != b->emit_wide) {
+		b->emit_wide = wide;
+		apply_shape(b, b->shape);    /* the emit grid */
 	}
 	if (on != b->P.bucket_on || shift != b->P.bshift || top != b->P.cnt_top ||
 		one != b->P.one_slot || solo != b->P.solo) {
@@ -4920,7 +4955,9 @@ int pp_batch_tune(pp_batch_t *b, uint32_t trial_ticks, uint32_t *bucket_shift,
 		}
 		/* Then the launch shape under the chosen sort: rounds of resident
 		 * groups of the action and emit grids, and the bucket sort's half
-		 * or whole round. */
+		 * or whole round. (The emit width is left to its rule, which follows
+		 * the replica set: choosing it here, in trials with every replica
+		 * running, made the 64-replica workload's run 8% slower.) */
 		if (best_t >= 0) {
 			uint32_t best_r = b->tile_rounds, best_e = b->bucket_eighths;
 			double best_l = -1.0;
