@@ -99,6 +99,14 @@ static_assert(A_IPT == 2 || A_IPT == 4, "the emit loads a thread's pairs as one 
 constexpr int A_STAGE = 2 * A_TILE; /* staged outputs per action tile */
 constexpr int MIX_SLOTS = 16;       /* replicas per action tile summed in LDS */
 constexpr uint32_t K_REG = 5;       /* agents of a cell acted on in registers */
+/* ... in the second, rarely taken register path: at 1600 x 1600 the six to
+ * nine tiles an iteration that hold a cell of six or seven agents took 12-21 us
+ * in the general path against 8.7 for the others, and set the kernel's end.
+ * Six: 45.4 -> 43.5 us an iteration (12800 x 12800: 645 -> 640). Seven or
+ * eight take more registers than six waves leave (scratch spills): 0.1 us
+ * more gained at 1600 x 1600, the 5 us at 12800 x 12800 lost. */
+constexpr uint32_t K_REG2 = 6;
+template <uint32_t N> struct KConst { static constexpr uint32_t value = N; };
 /* Launch shape defaults (pp_batch_set_launch; the tuner tries the others):
  * the action and emit kernels take four times the groups the GPU holds at
  * once, so that groups that finish early are replaced, which evens out tiles
@@ -2117,28 +2125,39 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 		const uint32_t k = end - lp;
 		unsigned long long sc[NACC] = {0, 0, 0, 0, 0, 0, 0};
 		sc[AC_OCC] = 1;
-		if (k <= K_REG) {
-			/* The cell in registers (one Philox block covers the k - 1 <= 4
-			 * draws), with the same steps as the general path below. The
-			 * Fisher-Yates swaps act on a permutation of 4-bit indices, and
-			 * the values are read from the tile already in acting order;
-			 * sheep and eaten agents are bit masks over acting positions. */
-			const uint4 r = pp_philox(c - ra.cb, ra.step, PP_STREAM_SHUFFLE, 0,
-				ra.key0, ra.key1);
-			uint32_t perm = 0x43210u;
+		/* A cell of at most KR agents in registers, with the same steps as
+		 * the general path below: Philox block 0 covers the k - 1 <= 4
+		 * draws of a cell of up to five, block 1 the rest (KR up to 8).
+		 * The Fisher-Yates swaps act on a permutation of 4-bit indices, and
+		 * the values are read from the tile already in acting order; sheep
+		 * and eaten agents are bit masks over acting positions. */
+		auto reg_cell = [&](auto kr) {
+			constexpr uint32_t KR = decltype(kr)::value;
+			static_assert(KR <= 8, "4-bit indices in one word");
+			uint32_t perm = 0x76543210u;
+			/* the swaps of draws m0..m0+3 (i descending across both calls;
+			 * one block live at a time) */
+			auto swaps = [&](const uint4 w, uint32_t m0) {
 #pragma unroll
-			for (int i = K_REG - 1; i >= 1; i--) {
-				if ((uint32_t)i >= k) continue;
-				const uint32_t m = k - 1 - (uint32_t)i;
-				const uint32_t word = m == 0 ? r.x : m == 1 ? r.y : m == 2 ? r.z : r.w;
-				const uint32_t jj = pp_bounded(word, (uint32_t)i + 1);
-				const uint32_t x = ((perm >> (4 * i)) ^ (perm >> (4 * jj))) & 0xFu;
-				perm ^= (x << (4 * i)) | (x << (4 * jj));
-			}
-			uint32_t V[K_REG];
+				for (int i = KR - 1; i >= 1; i--) {
+					if ((uint32_t)i >= k) continue;
+					const uint32_t m = k - 1 - (uint32_t)i - m0;
+					if constexpr (KR > 5)
+						if (m >= 4) continue;   /* another block's (m0 <= m' < m0+4) */
+					const uint32_t word = m == 0 ? w.x : m == 1 ? w.y : m == 2 ? w.z : w.w;
+					const uint32_t jj = pp_bounded(word, (uint32_t)i + 1);
+					const uint32_t x = ((perm >> (4 * i)) ^ (perm >> (4 * jj))) & 0xFu;
+					perm ^= (x << (4 * i)) | (x << (4 * jj));
+				}
+			};
+			swaps(pp_philox(c - ra.cb, ra.step, PP_STREAM_SHUFFLE, 0, ra.key0, ra.key1), 0);
+			if constexpr (KR > 5)
+				swaps(pp_philox(c - ra.cb, ra.step, PP_STREAM_SHUFFLE, 1, ra.key0, ra.key1),
+					4);
+			uint32_t V[KR];
 			uint32_t sheep = 0;
 #pragma unroll
-			for (uint32_t q = 0; q < K_REG; q++) {
+			for (uint32_t q = 0; q < KR; q++) {
 				V[q] = q < k ? sv.get(lp + ((perm >> (4 * q)) & 0xFu)) : 0u;
 				if (q < k && !(V[q] & V_WOLF)) sheep |= 1u << q;
 			}
@@ -2146,7 +2165,7 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 			bool ate = false;
 			uint32_t killed = 0, ovf = 0;
 #pragma unroll
-			for (uint32_t q = 0; q < K_REG; q++) {
+			for (uint32_t q = 0; q < KR; q++) {
 				if (q >= k || ((killed >> q) & 1u)) continue;
 				const uint32_t v = V[q];
 				int32_t e = (int32_t)(v & V_ENERGY);
@@ -2176,12 +2195,12 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 				V[q] = nv | ((uint32_t)e & V_ENERGY);
 			}
 			if (ate) grass_cell(grass_at, c) = (GT)(tick + 1 + ra.R);
-			/* per-cell sums fit 32 bits: at most 2 K_REG outputs below
+			/* per-cell sums fit 32 bits: at most 2 KR outputs below
 			 * 2^24 each */
 			uint32_t n_sheep = 0, n_wolves = 0, e_sheep = 0, e_wolves = 0,
 				births = 0;
 #pragma unroll
-			for (uint32_t q = 0; q < K_REG; q++) {
+			for (uint32_t q = 0; q < KR; q++) {
 				if (q >= k) continue;
 				const uint32_t v = V[q] | (((killed >> q) & 1u) ? V_KILLED : 0u);
 				sv.set(lp + q, v);
@@ -2207,6 +2226,12 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 			sc[AC_OUT] = (n_sheep + n_wolves) + births * HI32;
 			sc[AC_EATEN] = ate ? 1 : 0;
 			sc[AC_OCC] = 1 + ovf * HI32;
+		};
+		if (k <= K_REG) {
+			reg_cell(KConst<K_REG>{});
+		} else if (k <= K_REG2) {
+			/* (a wave takes this branch only for its rare cells of six) */
+			reg_cell(KConst<K_REG2>{});
 		} else {
 			if (k > 1) {
 				uint4 r = make_uint4(0, 0, 0, 0);
