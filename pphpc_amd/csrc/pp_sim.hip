@@ -1720,7 +1720,12 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 			/* Each counter is now the end of its cell's slots, which hold the
 			 * cell's agents in scatter order: an agent's place among them is
 			 * the number of its cell's agents gathered before it (few: cells
-			 * hold a handful of agents), and it is written there directly. */
+			 * hold a handful of agents), and it is written there directly.
+			 * The sorted pairs, and the emit's moved pairs, are stored
+			 * non-temporally: nothing in the writing kernel reads them
+			 * again (with the emit's: 1600 x 1600 43.3 -> 42.3 us an
+			 * iteration, 12800 x 12800 636 -> 631; the act's write-back
+			 * the same way gained nothing). */
 			for (uint32_t q = tid; q < cnt; q += WG) {
 				const uint32_t i = s_ix[q];
 				const uint32_t c = s_ka[i];
@@ -1728,8 +1733,8 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 				const uint32_t s0 = c ? (s_cnt2[(c - 1) >> 1] >> (((c - 1) & 1u) * 16)) & 0xFFFFu : 0u;
 				uint32_t at = s0;
 				for (uint32_t a = s0; a < e; a++) at += s_ix[a] < i;
-				at32(okeys, out0 + at) = cs + c;
-				at32(ovals, out0 + at) = s_va[i];
+				__builtin_nontemporal_store(cs + c, &at32(okeys, out0 + at));
+				__builtin_nontemporal_store(s_va[i], &at32(ovals, out0 + at));
 			}
 		} else {
 			/* More agents than LDS holds (a dense bucket): count per cell,
@@ -1768,8 +1773,8 @@ __global__ __launch_bounds__(WG, 8) void k_bucket_sort(
 						if (in) {
 							const uint32_t dst = s_run[off] + rank;
 							if (dst < expect) {
-								okeys[out0 + dst] = key;
-								ovals[out0 + dst] = vals[p];
+								__builtin_nontemporal_store(key, &okeys[out0 + dst]);
+								__builtin_nontemporal_store(vals[p], &ovals[out0 + dst]);
 							} else {
 								set_error(ctl, PP_LIBRARY_ERROR, tick,
 									-(int32_t)(1u << 30) - (int32_t)b);
@@ -2702,8 +2707,8 @@ __device__ __forceinline__ unsigned long long emit_outputs(
 				if (x + 1 == min(((x >> S) + 1) << S, m.gx))
 					atomicAdd(&ocnt[edge_last_off(P.nb_cap) + ob], 1u);
 			}
-			at32(mkeys, id) = mk;
-			at32(mvals, id) = mv;
+			__builtin_nontemporal_store(mk, &at32(mkeys, id));
+			__builtin_nontemporal_store(mv, &at32(mvals, id));
 			live += mk != DEAD_KEY;
 		}
 		if (P.bucket_on) {
