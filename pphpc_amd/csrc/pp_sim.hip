@@ -114,12 +114,13 @@ template <uint32_t N> struct KConst { static constexpr uint32_t value = N; };
  * bucket sort takes one round of resident groups (eighths of a round). */
 constexpr uint32_t TILE_ROUNDS = 4;
 constexpr uint32_t BUCKET_EIGHTHS = 8;
-/* Emit groups of 512 threads (two positions each) when a tick's tiles fit in
- * one round of such groups: each tile's output rounds, its serial part, are
- * then half as many. One 1600 x 1600 replica: 45.1 us an iteration against
- * 46.4; at 2000 x 2000 and above, and for several replicas, 256 threads are
- * as fast or faster (12800 x 12800: 8%). */
-constexpr int EMIT_WIDE = 2 * WG;
+/* Action and emit groups of 512 threads (two positions each) when a tick's
+ * tiles fit in one round of such groups: a tile's serial parts (the emit's
+ * output rounds, the act's single-agent cells) are then half as long. One
+ * 1600 x 1600 replica: 46.4 -> 45.1 us an iteration for the emit, 42.4 ->
+ * 41.7 for the act; at 2000 x 2000 and above, and for several replicas, 256
+ * threads are as fast or faster (12800 x 12800: 8% for the emit). */
+constexpr int WIDE_WG = 2 * WG;
 constexpr int SCAN_WG = 1024;      /* threads of the one-group scans */
 constexpr int SCAN_UNROLL = 8;
 /* Grass regrow ticks are stored modulo 2^(8 sizeof(GT)); alive cells are
@@ -1959,6 +1960,7 @@ __device__ __forceinline__ void tile_add(uint32_t *act_cnt, uint32_t l1o,
  * and each energy sum as two 32-bit pieces split at bit 24 (exact while
  * every lane's sum is below 2^48; a wave holding a larger one sums it in 64
  * bits). acc is cleared. */
+template <int NW>
 __device__ __forceinline__ void act_wg_add(unsigned long long (&acc)[NACC],
 	unsigned long long (*s_red)[NACC], unsigned long long *dst) {
 	const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
@@ -1992,7 +1994,7 @@ __device__ __forceinline__ void act_wg_add(unsigned long long (&acc)[NACC],
 	__syncthreads();
 	if (threadIdx.x < NACC) {
 		unsigned long long t = 0;
-		for (int i = 0; i < NWAVE; i++) t += s_red[i][threadIdx.x];
+		for (int i = 0; i < NW; i++) t += s_red[i][threadIdx.x];
 		if (t) atomicAdd(&dst[threadIdx.x], t);
 	}
 	__syncthreads();
@@ -2006,8 +2008,8 @@ struct ActLds {
 	uint32_t slow_lp[A_TILE];
 	GT slow_g[A_TILE];
 	uint32_t nslow;
-	uint32_t tmp[NWAVE];
-	unsigned long long red[NWAVE][NACC];
+	uint32_t tmp[WIDE_WG / WAVE];
+	unsigned long long red[WIDE_WG / WAVE][NACC];
 	unsigned long long mix[MIX_SLOTS][NACC];
 };
 
@@ -2017,27 +2019,28 @@ struct ActLds {
  * MIXED = true: several, each cell looks up its replica and counters go
  * through LDS slots. See k_act for the rules. Agents whose energy after
  * their action exceeds V_ENERGY are counted in the high half of AC_OCC. */
-template <typename GT, bool MIXED>
+template <typename GT, bool MIXED, int NT = WG>
 __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 	uint32_t cnt, const uint32_t *__restrict__ keys, uint32_t *vals,
 	GT *grass_at, uint32_t *act_cnt, uint32_t l1o, const Tables &T, uint32_t nrep,
 	uint32_t tick, uint32_t L, uint32_t Rmax, uint32_t k_first,
 	uint32_t k_last, const RA &ra_u, unsigned long long (&acc)[NACC]) {
 	const int tid = threadIdx.x;
+	constexpr int IPT = A_TILE / NT;
 	const uint32_t base = tile * A_TILE;
 	const uint32_t alive_at = tick + 1;
 	const SegVals sv{S.val, vals + base};
 	/* Heads of this thread's positions; their grass loads are issued
 	 * together before any segment is processed. */
-	uint32_t cj[A_IPT];
-	GT gj[A_IPT];
-	bool hj[A_IPT];
+	uint32_t cj[IPT];
+	GT gj[IPT];
+	bool hj[IPT];
 	/* every lane loads (the tile's first cell where it holds no head), so
 	 * the loads are issued unconditionally */
 	const uint32_t c_safe = __builtin_amdgcn_readfirstlane(S.key[1]);
 #pragma unroll
-	for (int j = 0; j < A_IPT; j++) {
-		const uint32_t lp = tid * A_IPT + j;
+	for (int j = 0; j < IPT; j++) {
+		const uint32_t lp = tid * IPT + j;
 		cj[j] = S.key[1 + lp];
 		hj[j] = lp < cnt && S.key[lp] != cj[j];
 		gj[j] = grass_cell(grass_at, hj[j] ? cj[j] : c_safe);
@@ -2050,19 +2053,19 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 	 * path. */
 	if (tid == 0) S.nslow = 0;
 	if (MIXED)
-		for (int i = tid; i < MIX_SLOTS * NACC; i += WG) (&S.mix[0][0])[i] = 0;
+		for (int i = tid; i < MIX_SLOTS * NACC; i += NT) (&S.mix[0][0])[i] = 0;
 	__syncthreads();
 	/* the grass values (and the next tile's pairs, issued before them) */
 	wait_global();
 	uint32_t nout = 0;
-	/* sums over this thread's single-agent cells: at most A_IPT outputs
+	/* sums over this thread's single-agent cells: at most IPT outputs
 	 * pairs below 2^25 each, so 32 bits hold them */
 	uint32_t f_sheep = 0, f_wolves = 0, f_se = 0, f_we = 0, f_eaten = 0,
 		f_occ = 0, f_ovf = 0, f_birth = 0;
 #pragma unroll
-	for (int j = 0; j < A_IPT; j++) {
+	for (int j = 0; j < IPT; j++) {
 		if (!hj[j]) continue;
-		const uint32_t lp = tid * A_IPT + j;
+		const uint32_t lp = tid * IPT + j;
 		const uint32_t c = cj[j];
 		if (!MIXED && lp + 1 < cnt && S.key[2 + lp] != c) {
 			/* A cell with one agent: nothing to shuffle, a lone wolf finds no
@@ -2120,7 +2123,7 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 	__syncthreads();
 	const uint32_t nslow = S.nslow;
 #pragma unroll 1
-	for (uint32_t si = tid; si < nslow; si += WG) {
+	for (uint32_t si = tid; si < nslow; si += NT) {
 		const uint32_t lp = S.slow_lp[si];
 		const uint32_t c = S.key[1 + lp];
 		const GT gslow = S.slow_g[si];
@@ -2319,14 +2322,14 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 	__syncthreads();
 	/* Write back the positions this tile owns (not those of the previous
 	 * tile's last segment, which its lane wrote). */
-	for (uint32_t i = tid; i < cnt; i += WG)
+	for (uint32_t i = tid; i < cnt; i += NT)
 		if (S.key[1 + i] != S.key[0]) at32(vals, base + i) = S.val[i];
 	uint32_t total;
-	(void)wg_excl_scan<WG>(nout, S.tmp, total);
+	(void)wg_excl_scan<NT>(nout, S.tmp, total);
 	if (tid == 0 && total) tile_add(act_cnt, l1o, tile, total);
 	if (MIXED) {
 		const uint32_t nk = min(k_last - k_first + 1, (uint32_t)MIX_SLOTS);
-		for (uint32_t i = tid; i < nk * NACC; i += WG) {
+		for (uint32_t i = tid; i < nk * NACC; i += NT) {
 			const unsigned long long v = (&S.mix[0][0])[i];
 			if (v)
 				atomicAdd(&T.rctl[T.order[k_first + i / NACC]].acc[i % NACC], v);
@@ -2360,10 +2363,11 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
  * replica. Tiles holding agents of several replicas are listed for
  * k_act_mixed (segments never span replicas, so the two kernels touch
  * disjoint cells and values). */
-template <typename GT>
-__global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys,
+template <typename GT, int NT>
+__global__ __launch_bounds__(NT, 6) void k_act(const uint32_t *__restrict__ keys,
 	uint32_t *vals, GT *grass_at, uint32_t *act_cnt, uint32_t *mixed_list,
 	uint32_t *ocnt, uint32_t *ncnt, Ctl *ctl, Tables T, Params P) {
+	constexpr int IPT = A_TILE / NT;
 	__shared__ ActLds<GT> S;
 	const CtlHead H = ctl_head(ctl);
 	if (H.error) return;
@@ -2375,12 +2379,12 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 		const uint32_t NB = H.nbuckets;
 		for (uint32_t k = 0; k <= P.cnt_top; k++) {
 			const uint32_t o = cnt_off(P.nb_cap, (int)k), n = (NB >> (6 * k)) + 1;
-			for (uint32_t i = blockIdx.x * WG + tid; i < n; i += gridDim.x * WG) {
+			for (uint32_t i = blockIdx.x * NT + tid; i < n; i += gridDim.x * NT) {
 				ocnt[o + i] = 0;
 				ncnt[o + i] = 0;
 			}
 		}
-		for (uint32_t i = blockIdx.x * WG + tid; i <= NB; i += gridDim.x * WG) {
+		for (uint32_t i = blockIdx.x * NT + tid; i <= NB; i += gridDim.x * NT) {
 			ocnt[edge_first_off(P.nb_cap) + i] = 0;
 			ocnt[edge_last_off(P.nb_cap) + i] = 0;
 		}
@@ -2402,12 +2406,12 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 	const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
 	const uint32_t t_beg = blockIdx.x * per;
 	const uint32_t t_end = min(ntiles, t_beg + per);
-	uint32_t pk[A_IPT], pv[A_IPT], pprev = DEAD_KEY;
+	uint32_t pk[IPT], pv[IPT], pprev = DEAD_KEY;
 	auto fetch = [&](uint32_t t) {
 		const uint32_t base = t * A_TILE;
 #pragma unroll
-		for (int j = 0; j < A_IPT; j++) {
-			const uint32_t i = base + tid + j * WG;
+		for (int j = 0; j < IPT; j++) {
+			const uint32_t i = base + tid + j * NT;
 			pk[j] = DEAD_KEY;
 			pv[j] = 0u;
 			if (i < L) {
@@ -2431,9 +2435,9 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 	for (uint32_t tile = t_beg; tile < t_end; tile++) {
 		const uint32_t cnt = min((uint32_t)A_TILE, L - tile * A_TILE);
 #pragma unroll
-		for (int j = 0; j < A_IPT; j++) {
-			S.key[1 + tid + j * WG] = pk[j];
-			S.val[tid + j * WG] = pv[j];
+		for (int j = 0; j < IPT; j++) {
+			S.key[1 + tid + j * NT] = pk[j];
+			S.val[tid + j * NT] = pv[j];
 		}
 		if (tid == 0) S.key[0] = pprev;
 		__syncthreads();
@@ -2465,14 +2469,14 @@ __global__ __launch_bounds__(WG, 6) void k_act(const uint32_t *__restrict__ keys
 				ra = load_ra(T, k_first, tick);
 			}
 			if (tile + 1 < t_end) fetch(tile + 1);
-			act_tile<GT, false>(S, tile, cnt, keys, vals, grass_at,
+			act_tile<GT, false, NT>(S, tile, cnt, keys, vals, grass_at,
 				act_cnt, P.act_l1, T, nrep, tick, L, P.R, k_first, k_last, ra, acc);
 		}
 	}
 	/* one replica: a partial-sum slot, words of the tick's parity (the
 	 * emit adds them up) */
 	if (acc_k != NO_REP)
-		act_wg_add(acc, S.red, nrep == 1
+		act_wg_add<NT / WAVE>(acc, S.red, nrep == 1
 			? P.part + (blockIdx.x % PART_SLOTS) * PART_STRIDE + part_acc(tick)
 			: &T.rctl[T.order[acc_k]].acc[0]);
 }
@@ -3198,9 +3202,9 @@ struct pp_batch {
 	/* launch shape: rounds of resident groups of the action and emit grids,
 	 * eighths of a round of the bucket sort's */
 	uint32_t tile_rounds = TILE_ROUNDS, bucket_eighths = BUCKET_EIGHTHS;
-	/* emit groups of EMIT_WIDE threads instead of WG (chosen from the live
-	 * agents at each replica set) */
-	int emit_wide = 0;
+	/* action and emit groups of WIDE_WG threads instead of WG (chosen from
+	 * the live agents at each replica set) */
+	int wide = 0;
 
 	uint32_t *cell = nullptr;
 	int32_t *energy = nullptr;
@@ -3379,9 +3383,14 @@ static void launch_sort_pass(pp_batch_t *b, uint32_t pass) {
 
 template <typename GT>
 static void launch_act(pp_batch_t *b, int sorted) {
-	k_act<GT><<<b->grid_act, WG, 0, b->stream>>>(b->keys[sorted],
-		b->vals[sorted], (GT *)b->grass_at, b->act_cnt, b->mixed_list, b->ocnt,
-		b->ncnt, b->ctl, b->tables(), b->P);
+	if (b->wide)
+		k_act<GT, WIDE_WG><<<b->grid_act, WIDE_WG, 0, b->stream>>>(b->keys[sorted],
+			b->vals[sorted], (GT *)b->grass_at, b->act_cnt, b->mixed_list, b->ocnt,
+			b->ncnt, b->ctl, b->tables(), b->P);
+	else
+		k_act<GT, WG><<<b->grid_act, WG, 0, b->stream>>>(b->keys[sorted],
+			b->vals[sorted], (GT *)b->grass_at, b->act_cnt, b->mixed_list, b->ocnt,
+			b->ncnt, b->ctl, b->tables(), b->P);
 	/* A batch of one replica never has a tile of several. */
 	if (b->max_rep > 1)
 		k_act_mixed<GT><<<min_u32(b->max_rep, 256), WG, 0, b->stream>>>(
@@ -3453,8 +3462,8 @@ static int enqueue_step(pp_batch_t *b, int radix) {
 		prof_end(b, pe);
 	}
 	prof_begin(b, KI_EMIT_MOVE, pe);
-	if (b->emit_wide)
-		k_emit_move<EMIT_WIDE><<<b->grid_emit, EMIT_WIDE, 0, b->stream>>>(
+	if (b->wide)
+		k_emit_move<WIDE_WG><<<b->grid_emit, WIDE_WG, 0, b->stream>>>(
 			b->keys[sorted], b->vals[sorted], b->act_cnt, b->cell, b->energy,
 			b->type, b->keys[0], b->vals[0], b->ocnt, b->ncnt, b->ctl,
 			b->tables(), b->P);
@@ -3488,7 +3497,9 @@ template <typename GT>
 static void set_grids_gt(pp_batch_t *b) {
 	const Params &P = b->P;
 	b->grid_act = min_u32(ceil_div(P.cap, A_TILE),
-		b->tile_rounds * resident(b, (const void *)k_act<GT>));
+		b->tile_rounds * (b->wide
+			? resident(b, (const void *)k_act<GT, WIDE_WG>, WIDE_WG)
+			: resident(b, (const void *)k_act<GT, WG>)));
 	b->grid_bucket = min_u32(P.nb_cap, std::max<uint32_t>(1u,
 		b->bucket_eighths * resident(b, (const void *)k_bucket_sort<GT, false>) / 8));
 }
@@ -3510,17 +3521,18 @@ static void apply_shape(pp_batch_t *b, const SortShape &sh) {
 	 * digits: one) */
 	b->grid_down = min_u32(P.hist_cols, (b->rbits == 11 ? 1 : 3) * cu);
 	b->grid_emit = min_u32(ceil_div(P.cap, A_TILE), b->tile_rounds *
-		(b->emit_wide ? resident(b, (const void *)k_emit_move<EMIT_WIDE>, EMIT_WIDE)
+		(b->wide ? resident(b, (const void *)k_emit_move<WIDE_WG>, WIDE_WG)
 			: resident(b, (const void *)k_emit_move<WG>)));
 	if (b->grass_bytes == 1) set_grids_gt<uint8_t>(b);
 	else if (b->grass_bytes == 2) set_grids_gt<uint16_t>(b);
 	else set_grids_gt<uint32_t>(b);
 }
 
-/* Wide emit groups while the tiles of `agents` fit in one round of them. */
-static int emit_wide_auto(const pp_batch_t *b, uint64_t agents) {
+/* Wide action and emit groups while the tiles of `agents` fit in one round
+ * of wide emit groups. */
+static int wide_auto(const pp_batch_t *b, uint64_t agents) {
 	const uint64_t tiles = (agents + A_TILE - 1) / A_TILE;
-	return tiles <= resident(b, (const void *)k_emit_move<EMIT_WIDE>, EMIT_WIDE)
+	return tiles <= resident(b, (const void *)k_emit_move<WIDE_WG>, WIDE_WG)
 		? 1 : 0;
 }
 
@@ -3673,13 +3685,14 @@ static int upload_table(pp_batch_t *b) {
 		b->grid_brec = grec;
 		drop_graph(b);
 	}
-	/* emit width: PPHPC_EMIT_WIDE (0 or 1), or wide while the agents'
-	 * tiles fit in one round of wide groups (results do not depend on it) */
-	int wide = emit_wide_auto(b, agents);
-	if (const char *e = getenv("PPHPC_EMIT_WIDE")) wide = atoi(e) ? 1 : 0;
-	if (wide != b->emit_wide) {
-		b->emit_wide = wide;
-		apply_shape(b, b->shape);    /* the emit grid */
+	/* group width: PPHPC_WIDE_GROUPS (0 or 1), or wide while the agents'
+	 * tiles fit in one round of wide emit groups (results do not depend on
+	 * it) */
+	int wide = wide_auto(b, agents);
+	if (const char *e = getenv("PPHPC_WIDE_GROUPS")) wide = atoi(e) ? 1 : 0;
+	if (wide != b->wide) {
+		b->wide = wide;
+		apply_shape(b, b->shape);    /* the action and emit grids */
 	}
 	if (on != b->P.bucket_on || shift != b->P.bshift || top != b->P.cnt_top ||
 		one != b->P.one_slot || solo != b->P.solo) {

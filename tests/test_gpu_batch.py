@@ -73,20 +73,20 @@ HETERO = [
 
 
 @pytest.mark.parametrize("prefix,narrow", [(False, False), (True, False), (False, True)],
-                         ids=["levels", "prefix", "narrow-emit"])
+                         ids=["levels", "prefix", "narrow"])
 def test_heterogeneous_batch_equals_alone(prefix, narrow, monkeypatch):
     """Replicas of different grids, parameter sets, seeds and iteration
     counts, admitted at ticks 0, 37 and 95, each equal to its run alone;
     the bucket sort reading its range bounds from the count levels, and from
     the bucket records (used by default from 6144 buckets, here for any
-    layout); the emit in groups of 512 threads (its default for this few
-    agents) and of 256."""
+    layout); the act and emit in groups of 512 threads (their default for
+    this few agents) and of 256."""
     if prefix:
         monkeypatch.setenv("PPHPC_BPREFIX_MIN", "0")
     if narrow:
-        monkeypatch.setenv("PPHPC_EMIT_WIDE", "0")
+        monkeypatch.setenv("PPHPC_WIDE_GROUPS", "0")
     _case(f"heterogeneous batch of 9 replicas, staggered admission, prefix={prefix}, "
-          f"narrow emit={narrow}")
+          f"narrow groups={narrow}")
     cells = sum(d["grid_x"] * d["grid_y"] for d, _, _ in HETERO)
     agents = sum(_reserve(d) for d, _, _ in HETERO)
     with _batch(agents, cells, 16) as b:

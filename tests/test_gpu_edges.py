@@ -270,15 +270,15 @@ def test_bucket_prefix_tables_agree(shift, monkeypatch):
         sim.close()
 
 
-@pytest.mark.parametrize("wide", ["0", "1"], ids=["emit256", "emit512"])
-def test_emit_width_agrees(wide, monkeypatch):
-    """One replica with the emit in groups of 256 and of 512 threads (chosen
-    by the agents' tile count; here forced): rows and state equal the
-    oracle's. Among the cases, a grid whose tick outputs end mid-round, a
+@pytest.mark.parametrize("wide", ["0", "1"], ids=["wg256", "wg512"])
+def test_group_width_agrees(wide, monkeypatch):
+    """One replica with the act and emit in groups of 256 and of 512 threads
+    (chosen by the agents' tile count; here forced): rows and state equal
+    the oracle's. Among the cases, a grid whose tick outputs end mid-round, a
     one-row grid and a dense corner whose tiles emit many children."""
     from oracle.pp_oracle import Oracle
-    monkeypatch.setenv("PPHPC_EMIT_WIDE", wide)
-    _case(f"emit width {wide}")
+    monkeypatch.setenv("PPHPC_WIDE_GROUPS", wide)
+    _case(f"group width {wide}")
     cases = [(v2s(600), 11), (v2s(gx=3000, gy=20), 12), (v2s(gx=257, gy=1), 13),
              (dict(v2ref(60), grid_x=60, grid_y=60, init_sheep=6000,
                    init_wolves=300), 14), (v2s(gx=1000, gy=1000), 15)]
@@ -289,7 +289,7 @@ def test_emit_width_agrees(wide, monkeypatch):
         rc, orows = o.run(60)
         assert rc == 0
         assert np.array_equal(rows[1:], orows), (d["grid_x"], d["grid_y"])
-        assert_state_equal(sim.state(), o.state(), f"emit {wide} {d['grid_x']}")
+        assert_state_equal(sim.state(), o.state(), f"width {wide} {d['grid_x']}")
         o.close()
         sim.close()
 
