@@ -2027,6 +2027,10 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 	uint32_t k_last, const RA &ra_u, unsigned long long (&acc)[NACC]) {
 	const int tid = threadIdx.x;
 	constexpr int IPT = A_TILE / NT;
+	/* wide groups hold two positions a thread, which leaves the registers
+	 * for cells of up to eight in the second path (1600 x 1600: 41.6 ->
+	 * 41.4 us an iteration) */
+	constexpr uint32_t KR2 = NT > WG ? 8u : K_REG2;
 	const uint32_t base = tile * A_TILE;
 	const uint32_t alive_at = tick + 1;
 	const SegVals sv{S.val, vals + base};
@@ -2237,9 +2241,10 @@ __device__ __forceinline__ void act_tile(ActLds<GT> &S, uint32_t tile,
 		};
 		if (k <= K_REG) {
 			reg_cell(KConst<K_REG>{});
-		} else if (k <= K_REG2) {
-			/* (a wave takes this branch only for its rare cells of six) */
-			reg_cell(KConst<K_REG2>{});
+		} else if (k <= KR2) {
+			/* (a wave takes this branch only for its rare cells of six, or
+			 * six to eight in wide groups) */
+			reg_cell(KConst<KR2>{});
 		} else {
 			if (k > 1) {
 				uint4 r = make_uint4(0, 0, 0, 0);
