@@ -118,8 +118,9 @@ constexpr uint32_t BUCKET_EIGHTHS = 8;
  * tiles fit in one round of such groups: a tile's serial parts (the emit's
  * output rounds, the act's single-agent cells) are then half as long. One
  * 1600 x 1600 replica: 46.4 -> 45.1 us an iteration for the emit, 42.4 ->
- * 41.7 for the act; at 2000 x 2000 and above, and for several replicas, 256
- * threads are as fast or faster (12800 x 12800: 8% for the emit). */
+ * 41.7 for the act. From 2000 x 2000 to 3200 x 3200, at 12800 x 12800 (8% for
+ * the emit) and for several replicas 256 threads are faster; 4000 x 4000 is
+ * the exception measured (wide 4% faster). */
 constexpr int WIDE_WG = 2 * WG;
 constexpr int SCAN_WG = 1024;      /* threads of the one-group scans */
 constexpr int SCAN_UNROLL = 8;
