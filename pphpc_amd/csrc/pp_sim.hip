@@ -122,7 +122,7 @@ constexpr uint32_t BUCKET_EIGHTHS = 8;
  * the emit) and for several replicas 256 threads are faster; 4000 x 4000 is
  * the exception measured (wide 4% faster). */
 constexpr int WIDE_WG = 2 * WG;
-constexpr int SCAN_WG = 1024;      /* threads of the one-group scans */
+constexpr int SCAN_WG = 1024;       /* threads of the one-group scans */
 constexpr int SCAN_UNROLL = 8;
 /* Grass regrow ticks are stored modulo 2^(8 sizeof(GT)); alive cells are
  * re-stamped once every REFRESH ticks (1/REFRESH of the cells per tick), so
